@@ -1,0 +1,121 @@
+// Internal definitions shared by the device-layer translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "feelpp_hip.h"
+
+namespace feelpp_amd {
+
+int setError(int code, const std::string& msg);
+
+#define FEEL_HIP_CHECK(expr)                                                                   \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return ::feelpp_amd::setError(FEEL_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Solver state shared by the CG kernels; written only by block 0 of a kernel,
+// read by the following kernels (kernel boundaries order the accesses).
+struct CgState {
+    double rtol, atol, dtol;
+    double ttol, rnorm0, dp, beta, dpi_prev;
+    int maxit, its, done, reason;
+    int iter;       // kernel-side iteration counter (number of p-updates launched and not skipped)
+    int pad[3];
+};
+
+constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying kernels
+constexpr int kBlock = 256;
+
+}  // namespace feelpp_amd
+
+struct feel_hip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t bytes = 0;
+    // timing
+    bool timing = false;
+    struct Ev { hipEvent_t a, b; std::string name; };
+    std::vector<Ev> events;
+    size_t evUsed = 0;
+    std::map<std::string, std::pair<int64_t, double>> timers;
+    // CG scratch: partial sums (3 arrays of kReduceBlocks doubles) and state
+    double* part = nullptr;
+    feelpp_amd::CgState* state = nullptr;
+    feelpp_amd::CgState* hstate = nullptr;   // pinned host mirror
+
+    int alloc(void** p, size_t bytes);
+    void release(void* p, size_t bytes);
+    void tic(const char* name);   // records the start event of a timed launch
+    void toc();                   // records the stop event
+    void collectTimers();
+};
+
+struct feel_hip_space {
+    feel_hip_ctx* ctx = nullptr;
+    int dim = 0, order = 0, ncomp = 1, nloc = 0, nldof = 0;
+    int64_t nv = 0, ne = 0, ndof = 0;
+    double* coords = nullptr;
+    int32_t* elems = nullptr;
+    int32_t* e2d = nullptr;
+    std::vector<int32_t> hElem2dof;   // kept to build scatter maps
+    // dof -> (element, local) incidence, element order, for the linear-form gather
+    uint32_t* rptr = nullptr;
+    uint32_t* rentry = nullptr;
+    // per-element scratch (geometric factors / element vectors)
+    double* geo = nullptr;
+    size_t geoBytes = 0;
+    double* elemvec = nullptr;
+    // reference tensors per quadrature order: [Rsym(NS) | M | m] device copies
+    std::map<int, double*> rtab;
+    std::map<int, std::vector<double>> hrtab;
+    // per quadrature order: weights, basis values, P1 geometry basis values
+    std::map<int, double*> qtab;
+    std::map<int, int> qnpts;
+};
+
+struct feel_hip_graph {
+    feel_hip_space* space = nullptr;
+    int64_t n = 0, nnz = 0;
+    int32_t* ia = nullptr;
+    int32_t* ja = nullptr;
+    int32_t* diagpos = nullptr;
+    uint32_t* cptr = nullptr;     // nnz+1, into centry
+    uint32_t* centry = nullptr;   // element-matrix entry ids e*nldof^2 + i*nldof + j
+    int64_t ncontrib = 0;
+    std::vector<int32_t> hia, hja;
+};
+
+struct feel_hip_mat {
+    feel_hip_graph* g = nullptr;
+    int precision = FEEL_HIP_FP64;
+    double* v64 = nullptr;
+    float* v32 = nullptr;
+    double* dinv64 = nullptr;
+    float* dinv32 = nullptr;
+    bool dinvValid = false;
+    double* elemmat = nullptr;    // materialised element matrices (elasticity path)
+    size_t elemmatBytes = 0;
+    // solver work vectors, allocated on first use and kept with the matrix
+    double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *w64 = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *w32 = nullptr, *x32 = nullptr, *b32 = nullptr;
+};
+
+struct feel_hip_vec {
+    feel_hip_ctx* ctx = nullptr;
+    int64_t n = 0;
+    double* d = nullptr;
+};
+
+struct feel_hip_rows {
+    feel_hip_ctx* ctx = nullptr;
+    int64_t n = 0, nrows = 0;
+    int32_t* rows = nullptr;
+    uint8_t* flag = nullptr;
+};

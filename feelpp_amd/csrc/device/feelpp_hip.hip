@@ -1,0 +1,786 @@
+// C-ABI implementation of the MI355X device layer: handles, uploads, assembly,
+// Dirichlet elimination and the Jacobi-preconditioned CG (FP64, and FP32 inside
+// FP64 iterative refinement).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../host/fe.hpp"
+#include "../host/space.hpp"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace feelpp_amd;
+
+namespace feelpp_amd {
+thread_local std::string gLastError;
+int setError(int code, const std::string& msg) {
+    gLastError = msg;
+    return code;
+}
+}  // namespace feelpp_amd
+
+namespace {
+
+int gridFor(int64_t n, int cap = 1 << 16) {
+    int64_t g = (n + kBlock - 1) / kBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(g, cap));
+}
+
+// Number of scalar reference-tensor slots for the symmetric gradient form
+int numSym(int dim) { return dim == 3 ? 6 : 3; }
+
+#define TRY_CALL(expr)                       \
+    do {                                     \
+        int rc_ = (expr);                    \
+        if (rc_ != FEEL_HIP_OK) return rc_;  \
+    } while (0)
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                         \
+    }                                                                     \
+    catch (const std::bad_alloc&) {                                       \
+        return setError(FEEL_HIP_ENOMEM, "host allocation failed");       \
+    }                                                                     \
+    catch (const std::invalid_argument& e) {                              \
+        return setError(FEEL_HIP_EUNSUPPORTED, e.what());                 \
+    }                                                                     \
+    catch (const std::exception& e) {                                     \
+        return setError(FEEL_HIP_EINVAL, e.what());                       \
+    }
+
+template <typename T>
+int devAlloc(feel_hip_ctx* ctx, T** p, size_t count) {
+    return ctx->alloc((void**)p, count * sizeof(T));
+}
+
+template <typename T>
+int upload(feel_hip_ctx* ctx, T** p, const T* host, size_t count) {
+    TRY_CALL(devAlloc(ctx, p, count));
+    if (count) FEEL_HIP_CHECK(hipMemcpyAsync(*p, host, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    FEEL_HIP_CHECK(hipStreamSynchronize(ctx->stream));
+    return FEEL_HIP_OK;
+}
+
+}  // namespace
+
+// ============================================================== context
+int feel_hip_ctx::alloc(void** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) return FEEL_HIP_OK;
+    hipError_t e = hipMalloc(p, n);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return setError(FEEL_HIP_ENOMEM, "hipMalloc(" + std::to_string(n) + " bytes): " + hipGetErrorString(e));
+    }
+    bytes += (int64_t)n;
+    return FEEL_HIP_OK;
+}
+
+void feel_hip_ctx::release(void* p, size_t n) {
+    if (!p) return;
+    (void)hipFree(p);
+    bytes -= (int64_t)n;
+}
+
+void feel_hip_ctx::tic(const char* name) {
+    if (!timing) return;
+    if (evUsed == events.size()) {
+        Ev ev;
+        if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) return;
+        events.push_back(ev);
+    }
+    events[evUsed].name = name;
+    (void)hipEventRecord(events[evUsed].a, stream);
+}
+
+void feel_hip_ctx::toc() {
+    if (!timing || evUsed >= events.size()) return;
+    (void)hipEventRecord(events[evUsed].b, stream);
+    ++evUsed;
+}
+
+void feel_hip_ctx::collectTimers() {
+    if (evUsed == 0) return;
+    (void)hipStreamSynchronize(stream);
+    for (size_t i = 0; i < evUsed; ++i) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, events[i].a, events[i].b) == hipSuccess) {
+            auto& t = timers[events[i].name];
+            t.first += 1;
+            t.second += ms;
+        }
+    }
+    evUsed = 0;
+}
+
+extern "C" {
+
+const char* feel_hip_last_error(void) { return gLastError.c_str(); }
+
+int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
+    if (!out) return setError(FEEL_HIP_EINVAL, "null output handle");
+    *out = nullptr;
+    int ndev = 0;
+    FEEL_HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return setError(FEEL_HIP_EINVAL, "no such HIP device");
+    FEEL_HIP_CHECK(hipSetDevice(device));
+    auto* c = new feel_hip_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return setError(FEEL_HIP_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    int rc = c->alloc((void**)&c->part, sizeof(double) * 4 * kReduceBlocks);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->state, sizeof(CgState) + 4 * sizeof(double));
+    if (rc == FEEL_HIP_OK && hipHostMalloc((void**)&c->hstate, sizeof(CgState), hipHostMallocDefault) != hipSuccess)
+        rc = setError(FEEL_HIP_ENOMEM, "hipHostMalloc failed");
+    if (rc != FEEL_HIP_OK) {
+        feel_hip_ctx_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_ctx_destroy(feel_hip_ctx* c) {
+    if (!c) return FEEL_HIP_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->release(c->part, sizeof(double) * 4 * kReduceBlocks);
+    c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
+    if (c->hstate) (void)hipHostFree(c->hstate);
+    for (auto& ev : c->events) {
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_ctx_synchronize(feel_hip_ctx* c) {
+    if (!c) return setError(FEEL_HIP_EINVAL, "null context");
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return FEEL_HIP_OK;
+}
+
+void* feel_hip_ctx_stream(feel_hip_ctx* c) { return c ? (void*)c->stream : nullptr; }
+int64_t feel_hip_ctx_device_bytes(feel_hip_ctx* c) { return c ? c->bytes : -1; }
+
+int feel_hip_ctx_set_timing(feel_hip_ctx* c, int enable) {
+    if (!c) return setError(FEEL_HIP_EINVAL, "null context");
+    c->collectTimers();
+    c->timing = enable != 0;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_ctx_timing(feel_hip_ctx* c, const char* name, int64_t* launches, double* total_ms) {
+    if (!c || !name) return setError(FEEL_HIP_EINVAL, "null argument");
+    c->collectTimers();
+    auto it = c->timers.find(name);
+    if (launches) *launches = it == c->timers.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == c->timers.end() ? 0.0 : it->second.second;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_ctx_reset_timing(feel_hip_ctx* c) {
+    if (!c) return setError(FEEL_HIP_EINVAL, "null context");
+    c->collectTimers();
+    c->timers.clear();
+    return FEEL_HIP_OK;
+}
+
+// ============================================================== space
+int feel_hip_space_create(feel_hip_ctx* ctx, int dim, int order, int ncomp, int64_t nverts, const double* coords,
+                          int64_t nelems, const int32_t* elems, const int32_t* elem2dof, int64_t ndof,
+                          feel_hip_space** out) {
+    GUARD_BEGIN
+    if (!ctx || !out || !coords || !elems || !elem2dof) return setError(FEEL_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (dim < 2 || dim > 3) return setError(FEEL_HIP_EUNSUPPORTED, "dim must be 2 or 3");
+    if (order < 1 || order > maxSupportedOrder(dim))
+        return setError(FEEL_HIP_EUNSUPPORTED, "order " + std::to_string(order) + " not supported in " +
+                                                   std::to_string(dim) + "D");
+    if (ncomp != 1 && ncomp != dim) return setError(FEEL_HIP_EINVAL, "ncomp must be 1 or dim");
+    if (nverts <= 0 || nelems <= 0 || ndof <= 0) return setError(FEEL_HIP_EINVAL, "empty mesh or space");
+    const int nloc = nLocalDof(dim, order), nldof = nloc * ncomp;
+    // validate connectivity before it reaches a kernel
+    for (int64_t k = 0; k < nelems * (dim + 1); ++k)
+        if (elems[k] < 0 || elems[k] >= nverts) return setError(FEEL_HIP_EINVAL, "element vertex index out of range");
+    for (int64_t k = 0; k < nelems * nldof; ++k)
+        if (elem2dof[k] < 0 || elem2dof[k] >= ndof) return setError(FEEL_HIP_EINVAL, "elem2dof entry out of range");
+    if ((uint64_t)nelems * nldof >= (1ULL << 32)) return setError(FEEL_HIP_EUNSUPPORTED, "space too large");
+    FEEL_HIP_CHECK(hipSetDevice(ctx->device));
+    auto* s = new feel_hip_space();
+    s->ctx = ctx;
+    s->dim = dim; s->order = order; s->ncomp = ncomp; s->nloc = nloc; s->nldof = nldof;
+    s->nv = nverts; s->ne = nelems; s->ndof = ndof;
+    s->hElem2dof.assign(elem2dof, elem2dof + nelems * nldof);
+    int rc = upload(ctx, &s->coords, coords, (size_t)nverts * dim);
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &s->elems, elems, (size_t)nelems * (dim + 1));
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &s->e2d, elem2dof, (size_t)nelems * nldof);
+    if (rc == FEEL_HIP_OK) {
+        DofTable dt;
+        dt.dim = dim; dt.order = order; dt.ncomp = ncomp; dt.nloc = nloc; dt.nldof = nldof;
+        dt.ne = nelems; dt.ndof = ndof;
+        dt.elem2dof = s->hElem2dof;
+        Incidence inc = buildIncidence(dt);
+        std::vector<uint32_t> p32(inc.ptr.begin(), inc.ptr.end());
+        rc = upload(ctx, &s->rptr, p32.data(), p32.size());
+        if (rc == FEEL_HIP_OK) rc = upload(ctx, &s->rentry, inc.entry.data(), inc.entry.size());
+    }
+    if (rc == FEEL_HIP_OK) rc = devAlloc(ctx, &s->elemvec, (size_t)nelems * nldof);
+    if (rc != FEEL_HIP_OK) {
+        feel_hip_space_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return FEEL_HIP_OK;
+    GUARD_END
+}
+
+int feel_hip_space_destroy(feel_hip_space* s) {
+    if (!s) return FEEL_HIP_OK;
+    feel_hip_ctx* c = s->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    c->release(s->coords, sizeof(double) * s->nv * s->dim);
+    c->release(s->elems, sizeof(int32_t) * s->ne * (s->dim + 1));
+    c->release(s->e2d, sizeof(int32_t) * s->ne * s->nldof);
+    c->release(s->rptr, sizeof(uint32_t) * (s->ndof + 1));
+    c->release(s->rentry, sizeof(uint32_t) * s->ne * s->nldof);
+    c->release(s->elemvec, sizeof(double) * s->ne * s->nldof);
+    c->release(s->geo, s->geoBytes);
+    for (auto& kv : s->rtab) c->release(kv.second, sizeof(double) * s->hrtab[kv.first].size());
+    for (auto& kv : s->qtab)
+        c->release(kv.second, sizeof(double) * (size_t)s->qnpts[kv.first] * (1 + s->nloc + s->dim + 1));
+    delete s;
+    return FEEL_HIP_OK;
+}
+
+int64_t feel_hip_space_ndof(const feel_hip_space* s) { return s ? s->ndof : -1; }
+int64_t feel_hip_space_nelements(const feel_hip_space* s) { return s ? s->ne : -1; }
+
+}  // extern "C"
+
+namespace {
+
+// reference tensors [Rsym(NS) | M] for the scalar forms, cached per quadrature order
+int scalarTensors(feel_hip_space* s, int qorder, const double** out) {
+    auto it = s->rtab.find(qorder);
+    if (it != s->rtab.end()) { *out = it->second; return FEEL_HIP_OK; }
+    ReferenceTensors T = referenceTensors(s->dim, s->order, qorder);
+    const int n = T.n, nn = n * n, d = s->dim, ns = numSym(d);
+    std::vector<double> h((size_t)(ns + 1) * nn);
+    auto R = [&](int a, int b, int ij) { return T.R[(size_t)(a * d + b) * nn + ij]; };
+    for (int ij = 0; ij < nn; ++ij) {
+        if (d == 3) {
+            h[0 * nn + ij] = R(0, 0, ij);
+            h[1 * nn + ij] = R(1, 1, ij);
+            h[2 * nn + ij] = R(2, 2, ij);
+            h[3 * nn + ij] = R(0, 1, ij) + R(1, 0, ij);
+            h[4 * nn + ij] = R(0, 2, ij) + R(2, 0, ij);
+            h[5 * nn + ij] = R(1, 2, ij) + R(2, 1, ij);
+        } else {
+            h[0 * nn + ij] = R(0, 0, ij);
+            h[1 * nn + ij] = R(1, 1, ij);
+            h[2 * nn + ij] = R(0, 1, ij) + R(1, 0, ij);
+        }
+        h[(size_t)ns * nn + ij] = T.M[ij];
+    }
+    double* dptr = nullptr;
+    TRY_CALL(upload(s->ctx, &dptr, h.data(), h.size()));
+    s->rtab[qorder] = dptr;
+    s->hrtab[qorder] = std::move(h);
+    *out = dptr;
+    return FEEL_HIP_OK;
+}
+
+// quadrature table for the linear form: w[Q] | phi[nloc*Q] | lambda[(dim+1)*Q]
+int rhsTable(feel_hip_space* s, int qorder, const double** out, int* Q) {
+    auto it = s->qtab.find(qorder);
+    if (it != s->qtab.end()) { *out = it->second; *Q = s->qnpts[qorder]; return FEEL_HIP_OK; }
+    QuadratureRule rule = quadrature(s->dim, qorder);
+    Tabulation T = tabulate(s->dim, s->order, rule.pts);
+    const int q = rule.npts, d = s->dim;
+    std::vector<double> h((size_t)q * (1 + s->nloc + d + 1));
+    for (int i = 0; i < q; ++i) h[i] = rule.w[i];
+    for (int i = 0; i < s->nloc; ++i)
+        for (int k = 0; k < q; ++k) h[q + (size_t)i * q + k] = T.phi[(size_t)i * q + k];
+    double* lam = h.data() + q + (size_t)s->nloc * q;
+    for (int k = 0; k < q; ++k) {
+        double sum = 0;
+        for (int c = 0; c < d; ++c) {
+            lam[(c + 1) * q + k] = 0.5 * (1.0 + rule.pts[(size_t)k * d + c]);
+            sum += lam[(c + 1) * q + k];
+        }
+        lam[k] = 1.0 - sum;
+    }
+    double* dptr = nullptr;
+    TRY_CALL(upload(s->ctx, &dptr, h.data(), h.size()));
+    s->qtab[qorder] = dptr;
+    s->qnpts[qorder] = q;
+    *out = dptr;
+    *Q = q;
+    return FEEL_HIP_OK;
+}
+
+int ensureGeo(feel_hip_space* s, size_t bytes) {
+    if (s->geoBytes >= bytes) return FEEL_HIP_OK;
+    s->ctx->release(s->geo, s->geoBytes);
+    s->geo = nullptr;
+    s->geoBytes = 0;
+    TRY_CALL(s->ctx->alloc((void**)&s->geo, bytes));
+    s->geoBytes = bytes;
+    return FEEL_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ============================================================== graph
+int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, const int32_t* ja,
+                          feel_hip_graph** out) {
+    GUARD_BEGIN
+    if (!s || !ia || !ja || !out) return setError(FEEL_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (nrows != s->ndof) return setError(FEEL_HIP_EINVAL, "graph rows must equal the space's dofs");
+    if (ia[0] != 0) return setError(FEEL_HIP_EINVAL, "ia[0] must be 0");
+    for (int64_t r = 0; r < nrows; ++r) {
+        if (ia[r + 1] < ia[r]) return setError(FEEL_HIP_EINVAL, "ia not monotone");
+        for (int32_t p = ia[r]; p < ia[r + 1]; ++p) {
+            if (ja[p] < 0 || ja[p] >= nrows) return setError(FEEL_HIP_EINVAL, "column index out of range");
+            if (p > ia[r] && ja[p] <= ja[p - 1]) return setError(FEEL_HIP_EINVAL, "columns not strictly ascending");
+        }
+    }
+    feel_hip_ctx* ctx = s->ctx;
+    FEEL_HIP_CHECK(hipSetDevice(ctx->device));
+    auto* g = new feel_hip_graph();
+    g->space = s;
+    g->n = nrows;
+    g->nnz = ia[nrows];
+    g->hia.assign(ia, ia + nrows + 1);
+    g->hja.assign(ja, ja + g->nnz);
+    std::vector<int32_t> diag(nrows, -1);
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int32_t* b = ja + ia[r];
+        const int32_t* e = ja + ia[r + 1];
+        const int32_t* p = std::lower_bound(b, e, (int32_t)r);
+        if (p == e || *p != r) {
+            delete g;
+            return setError(FEEL_HIP_EPATTERN, "graph row without its diagonal entry");
+        }
+        diag[r] = (int32_t)(p - ja);
+    }
+    DofTable dt;
+    dt.dim = s->dim; dt.order = s->order; dt.ncomp = s->ncomp; dt.nloc = s->nloc; dt.nldof = s->nldof;
+    dt.ne = s->ne; dt.ndof = s->ndof;
+    dt.elem2dof = s->hElem2dof;
+    GraphCSR gc;
+    gc.nrows = nrows; gc.nnz = g->nnz; gc.ia = g->hia; gc.ja = g->hja;
+    GatherMap gm;
+    try {
+        gm = buildGatherMap(dt, gc);
+    } catch (const std::runtime_error& e) {
+        delete g;
+        return setError(FEEL_HIP_EPATTERN, e.what());
+    }
+    g->ncontrib = (int64_t)gm.entry.size();
+    std::vector<uint32_t> cptr(gm.ptr.begin(), gm.ptr.end());
+    int rc = upload(ctx, &g->ia, ia, (size_t)nrows + 1);
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->ja, ja, (size_t)g->nnz);
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->diagpos, diag.data(), diag.size());
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->cptr, cptr.data(), cptr.size());
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->centry, gm.entry.data(), gm.entry.size());
+    if (rc != FEEL_HIP_OK) {
+        feel_hip_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FEEL_HIP_OK;
+    GUARD_END
+}
+
+int feel_hip_graph_destroy(feel_hip_graph* g) {
+    if (!g) return FEEL_HIP_OK;
+    feel_hip_ctx* c = g->space->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    c->release(g->ia, sizeof(int32_t) * (g->n + 1));
+    c->release(g->ja, sizeof(int32_t) * g->nnz);
+    c->release(g->diagpos, sizeof(int32_t) * g->n);
+    c->release(g->cptr, sizeof(uint32_t) * (g->nnz + 1));
+    c->release(g->centry, sizeof(uint32_t) * g->ncontrib);
+    delete g;
+    return FEEL_HIP_OK;
+}
+
+int64_t feel_hip_graph_nnz(const feel_hip_graph* g) { return g ? g->nnz : -1; }
+
+// ============================================================== matrix
+int feel_hip_mat_create(feel_hip_graph* g, int precision, feel_hip_mat** out) {
+    if (!g || !out) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (precision < 1 || precision > 3) return setError(FEEL_HIP_EINVAL, "bad precision");
+    *out = nullptr;
+    feel_hip_ctx* c = g->space->ctx;
+    auto* A = new feel_hip_mat();
+    A->g = g;
+    A->precision = precision;
+    int rc = FEEL_HIP_OK;
+    if (precision & FEEL_HIP_FP64) rc = devAlloc(c, &A->v64, (size_t)g->nnz);
+    if (rc == FEEL_HIP_OK && (precision & FEEL_HIP_FP32)) rc = devAlloc(c, &A->v32, (size_t)g->nnz);
+    if (rc == FEEL_HIP_OK) rc = devAlloc(c, &A->dinv64, (size_t)g->n);
+    if (rc == FEEL_HIP_OK) rc = devAlloc(c, &A->dinv32, (size_t)g->n);
+    if (rc != FEEL_HIP_OK) {
+        feel_hip_mat_destroy(A);
+        return rc;
+    }
+    *out = A;
+    return feel_hip_mat_zero(A);
+}
+
+int feel_hip_mat_destroy(feel_hip_mat* A) {
+    if (!A) return FEEL_HIP_OK;
+    feel_hip_ctx* c = A->g->space->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    c->release(A->v64, sizeof(double) * A->g->nnz);
+    c->release(A->v32, sizeof(float) * A->g->nnz);
+    c->release(A->dinv64, sizeof(double) * A->g->n);
+    c->release(A->dinv32, sizeof(float) * A->g->n);
+    c->release(A->elemmat, A->elemmatBytes);
+    const int64_t n = A->g->n;
+    for (double* v : {A->r64, A->z64, A->p64, A->w64}) c->release(v, sizeof(double) * n);
+    for (float* v : {A->r32, A->z32, A->p32, A->w32, A->x32, A->b32}) c->release(v, sizeof(float) * n);
+    delete A;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_mat_zero(feel_hip_mat* A) {
+    if (!A) return setError(FEEL_HIP_EINVAL, "null matrix");
+    feel_hip_ctx* c = A->g->space->ctx;
+    if (A->v64) FEEL_HIP_CHECK(hipMemsetAsync(A->v64, 0, sizeof(double) * A->g->nnz, c->stream));
+    if (A->v32) FEEL_HIP_CHECK(hipMemsetAsync(A->v32, 0, sizeof(float) * A->g->nnz, c->stream));
+    A->dinvValid = false;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_mat_add_matrix(feel_hip_mat* A, int nrows, const int32_t* rows, int ncols, const int32_t* cols,
+                            const double* data) {
+    if (!A || !rows || !cols || !data || nrows < 0 || ncols < 0) return setError(FEEL_HIP_EINVAL, "bad argument");
+    feel_hip_graph* g = A->g;
+    feel_hip_ctx* c = g->space->ctx;
+    const int64_t m = (int64_t)nrows * ncols;
+    if (m == 0) return FEEL_HIP_OK;
+    std::vector<int64_t> pos(m);
+    for (int i = 0; i < nrows; ++i) {
+        if (rows[i] < 0 || rows[i] >= g->n) return setError(FEEL_HIP_EINVAL, "row out of range");
+        const int32_t* b = g->hja.data() + g->hia[rows[i]];
+        const int32_t* e = g->hja.data() + g->hia[rows[i] + 1];
+        for (int j = 0; j < ncols; ++j) {
+            const int32_t* p = std::lower_bound(b, e, cols[j]);
+            if (p == e || *p != cols[j]) return setError(FEEL_HIP_EPATTERN, "entry outside the sparsity graph");
+            pos[(size_t)i * ncols + j] = p - g->hja.data();
+        }
+    }
+    int64_t* dpos = nullptr;
+    double* dval = nullptr;
+    TRY_CALL(upload(c, &dpos, pos.data(), pos.size()));
+    int rc = upload(c, &dval, data, (size_t)m);
+    if (rc == FEEL_HIP_OK) {
+        hipLaunchKernelGGL(kAddAt, dim3(1), dim3(1), 0, c->stream, m, dpos, dval, A->v64, A->v32);
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = setError(FEEL_HIP_EDEVICE, hipGetErrorString(e));
+    }
+    c->release(dpos, sizeof(int64_t) * m);
+    c->release(dval, sizeof(double) * m);
+    A->dinvValid = false;
+    return rc;
+}
+
+int feel_hip_mat_get_values(feel_hip_mat* A, int precision, void* host) {
+    if (!A || !host) return setError(FEEL_HIP_EINVAL, "null argument");
+    feel_hip_ctx* c = A->g->space->ctx;
+    if (precision == FEEL_HIP_FP64) {
+        if (!A->v64) return setError(FEEL_HIP_EINVAL, "matrix holds no FP64 values");
+        FEEL_HIP_CHECK(hipMemcpyAsync(host, A->v64, sizeof(double) * A->g->nnz, hipMemcpyDeviceToHost, c->stream));
+    } else if (precision == FEEL_HIP_FP32) {
+        if (!A->v32) return setError(FEEL_HIP_EINVAL, "matrix holds no FP32 values");
+        FEEL_HIP_CHECK(hipMemcpyAsync(host, A->v32, sizeof(float) * A->g->nnz, hipMemcpyDeviceToHost, c->stream));
+    } else {
+        return setError(FEEL_HIP_EINVAL, "precision must be FP64 or FP32");
+    }
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_mat_set_values(feel_hip_mat* A, const double* host) {
+    if (!A || !host) return setError(FEEL_HIP_EINVAL, "null argument");
+    feel_hip_ctx* c = A->g->space->ctx;
+    const int64_t nnz = A->g->nnz;
+    if (A->v64) {
+        FEEL_HIP_CHECK(hipMemcpyAsync(A->v64, host, sizeof(double) * nnz, hipMemcpyHostToDevice, c->stream));
+        if (A->v32) hipLaunchKernelGGL(kCastF32, dim3(gridFor(nnz)), dim3(kBlock), 0, c->stream, nnz, A->v64, A->v32);
+    } else {
+        std::vector<float> f(host, host + nnz);
+        FEEL_HIP_CHECK(hipMemcpyAsync(A->v32, f.data(), sizeof(float) * nnz, hipMemcpyHostToDevice, c->stream));
+        FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    }
+    FEEL_HIP_CHECK(hipGetLastError());
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    A->dinvValid = false;
+    return FEEL_HIP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// spmv launch: lanes-per-row chosen from the mean row length
+int lanesFor(const feel_hip_graph* g) {
+    const double mean = g->n ? (double)g->nnz / (double)g->n : 0.0;
+    return mean > 48 ? 16 : (mean > 12 ? 8 : 4);
+}
+
+template <typename T>
+void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part,
+                const CgState* st, int grid) {
+    const int L = lanesFor(g);
+    if (L == 16)
+        hipLaunchKernelGGL((kSpmv<T, 16>), dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, val, x, y, part, st);
+    else if (L == 8)
+        hipLaunchKernelGGL((kSpmv<T, 8>), dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, val, x, y, part, st);
+    else
+        hipLaunchKernelGGL((kSpmv<T, 4>), dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, val, x, y, part, st);
+}
+
+int spmvGrid(const feel_hip_graph* g) {
+    const int L = lanesFor(g);
+    const int64_t groups = (g->n + (kBlock / L) - 1) / (kBlock / L);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(groups, 8192));
+}
+
+}  // namespace
+
+extern "C" {
+
+int feel_hip_mat_mult(feel_hip_mat* A, const feel_hip_vec* x, feel_hip_vec* y) {
+    if (!A || !x || !y) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (x->n != A->g->n || y->n != A->g->n) return setError(FEEL_HIP_EINVAL, "size mismatch");
+    if (!A->v64) return setError(FEEL_HIP_EINVAL, "matrix holds no FP64 values");
+    feel_hip_ctx* c = A->g->space->ctx;
+    c->tic("spmv_fp64");
+    launchSpmv<double>(c, A->g, A->v64, x->d, y->d, nullptr, nullptr, spmvGrid(A->g));
+    c->toc();
+    FEEL_HIP_CHECK(hipGetLastError());
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    return FEEL_HIP_OK;
+}
+
+// ============================================================== vectors
+int feel_hip_vec_create(feel_hip_ctx* c, int64_t n, feel_hip_vec** out) {
+    if (!c || !out || n < 0) return setError(FEEL_HIP_EINVAL, "bad argument");
+    auto* v = new feel_hip_vec();
+    v->ctx = c;
+    v->n = n;
+    int rc = devAlloc(c, &v->d, (size_t)n);
+    if (rc != FEEL_HIP_OK) { delete v; return rc; }
+    if (n) FEEL_HIP_CHECK(hipMemsetAsync(v->d, 0, sizeof(double) * n, c->stream));
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    *out = v;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_destroy(feel_hip_vec* v) {
+    if (!v) return FEEL_HIP_OK;
+    (void)hipStreamSynchronize(v->ctx->stream);
+    v->ctx->release(v->d, sizeof(double) * v->n);
+    delete v;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_set(feel_hip_vec* v, const double* host) {
+    if (!v || !host) return setError(FEEL_HIP_EINVAL, "null argument");
+    FEEL_HIP_CHECK(hipMemcpyAsync(v->d, host, sizeof(double) * v->n, hipMemcpyHostToDevice, v->ctx->stream));
+    FEEL_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_get(const feel_hip_vec* v, double* host) {
+    if (!v || !host) return setError(FEEL_HIP_EINVAL, "null argument");
+    FEEL_HIP_CHECK(hipMemcpyAsync(host, v->d, sizeof(double) * v->n, hipMemcpyDeviceToHost, v->ctx->stream));
+    FEEL_HIP_CHECK(hipStreamSynchronize(v->ctx->stream));
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_zero(feel_hip_vec* v) {
+    if (!v) return setError(FEEL_HIP_EINVAL, "null vector");
+    if (v->n) FEEL_HIP_CHECK(hipMemsetAsync(v->d, 0, sizeof(double) * v->n, v->ctx->stream));
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_dot(const feel_hip_vec* a, const feel_hip_vec* b, double* out) {
+    if (!a || !b || !out || a->n != b->n) return setError(FEEL_HIP_EINVAL, "bad argument");
+    feel_hip_ctx* c = a->ctx;
+    hipLaunchKernelGGL(kDotPartial, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, a->n, a->d, b->d, c->part);
+    FEEL_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(kReduceBlocks);
+    FEEL_HIP_CHECK(hipMemcpyAsync(h.data(), c->part, sizeof(double) * kReduceBlocks, hipMemcpyDeviceToHost, c->stream));
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    double s = 0;
+    for (double x : h) s += x;
+    *out = s;
+    return FEEL_HIP_OK;
+}
+
+int64_t feel_hip_vec_size(const feel_hip_vec* v) { return v ? v->n : -1; }
+void* feel_hip_vec_device_ptr(feel_hip_vec* v) { return v ? (void*)v->d : nullptr; }
+
+// ============================================================== assembly
+int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d, int add) {
+    GUARD_BEGIN
+    if (!A || !d) return setError(FEEL_HIP_EINVAL, "null argument");
+    feel_hip_graph* g = A->g;
+    feel_hip_space* s = g->space;
+    feel_hip_ctx* c = s->ctx;
+    const int wantPrec = d->precision ? d->precision : A->precision;
+    double* v64 = (wantPrec & FEEL_HIP_FP64) ? A->v64 : nullptr;
+    float* v32 = (wantPrec & FEEL_HIP_FP32) ? A->v32 : nullptr;
+    if (!v64 && !v32) return setError(FEEL_HIP_EINVAL, "matrix holds none of the requested precisions");
+    if (d->kind & FEEL_HIP_EXPR_ELASTICITY) {
+        return setError(FEEL_HIP_EUNSUPPORTED, "elasticity assembly is not available in this build");
+    }
+    if ((d->kind & ~(FEEL_HIP_EXPR_MASS | FEEL_HIP_EXPR_DIFFUSION)) || d->kind == 0)
+        return setError(FEEL_HIP_EUNSUPPORTED, "expression outside the supported subset");
+    if (s->ncomp != 1) return setError(FEEL_HIP_EUNSUPPORTED, "mass/diffusion need a scalar space");
+    auto coefPtr = [&](const feel_hip_vec* v, const char* what, const double** out) -> int {
+        *out = nullptr;
+        if (!v) return FEEL_HIP_OK;
+        if (v->n != s->ne) return setError(FEEL_HIP_EINVAL, std::string(what) + " must have one value per element");
+        *out = v->d;
+        return FEEL_HIP_OK;
+    };
+    const double *kd = nullptr, *km = nullptr;
+    TRY_CALL(coefPtr(d->diff_coef, "diffusion coefficient", &kd));
+    TRY_CALL(coefPtr(d->mass_coef, "mass coefficient", &km));
+    const double kds = (d->kind & FEEL_HIP_EXPR_DIFFUSION) ? d->diff_scale : 0.0;
+    const double kms = (d->kind & FEEL_HIP_EXPR_MASS) ? d->mass_scale : 0.0;
+    const double* rt = nullptr;
+    TRY_CALL(scalarTensors(s, d->quad_order, &rt));
+    const int stride = s->dim == 3 ? 8 : 4;
+    TRY_CALL(ensureGeo(s, sizeof(double) * stride * s->ne));
+    c->tic("assemble_geo");
+    if (s->dim == 3)
+        hipLaunchKernelGGL(kGeoScalar<3>, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->coords, s->elems, kd,
+                           kds, km, kms, s->geo);
+    else
+        hipLaunchKernelGGL(kGeoScalar<2>, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->coords, s->elems, kd,
+                           kds, km, kms, s->geo);
+    c->toc();
+    FEEL_HIP_CHECK(hipGetLastError());
+    const int grid = gridFor(g->nnz, 8192);
+    c->tic("assemble_gather");
+#define GATHER(NN, NS, ST)                                                                                      \
+    hipLaunchKernelGGL((kGatherScalar<NN, NS, ST>), dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, \
+                       s->geo, rt, v64, v32, add)
+    const int key = s->dim * 10 + s->order;
+    switch (key) {
+        case 21: GATHER(9, 3, 4); break;
+        case 22: GATHER(36, 3, 4); break;
+        case 31: GATHER(16, 6, 8); break;
+        case 32: GATHER(100, 6, 8); break;
+        case 33: GATHER(400, 6, 8); break;
+        default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no scatter kernel for this space");
+    }
+#undef GATHER
+    c->toc();
+    FEEL_HIP_CHECK(hipGetLastError());
+    if (v64 && A->v32 && !v32) hipLaunchKernelGGL(kCastF32, dim3(gridFor(g->nnz)), dim3(kBlock), 0, c->stream, g->nnz, v64, A->v32);
+    FEEL_HIP_CHECK(hipGetLastError());
+    A->dinvValid = false;
+    return FEEL_HIP_OK;
+    GUARD_END
+}
+
+int feel_hip_assemble_linear(feel_hip_space* s, const feel_hip_linear_desc* d, feel_hip_vec* b, int add) {
+    GUARD_BEGIN
+    if (!s || !d || !b) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (b->n != s->ndof) return setError(FEEL_HIP_EINVAL, "rhs size must equal the space's dofs");
+    if (d->kind < 0 || d->kind > 2) return setError(FEEL_HIP_EUNSUPPORTED, "unknown source kind");
+    if (d->kind == FEEL_HIP_SRC_VECTOR && s->ncomp != 3) return setError(FEEL_HIP_EINVAL, "vector source needs Pchv");
+    if (d->kind != FEEL_HIP_SRC_VECTOR && s->ncomp != 1) return setError(FEEL_HIP_EINVAL, "scalar source needs Pch");
+    feel_hip_ctx* c = s->ctx;
+    const double* qt = nullptr;
+    int Q = 0;
+    TRY_CALL(rhsTable(s, d->quad_order, &qt, &Q));
+    if (s->dim == 3)
+        hipLaunchKernelGGL(kElemRhs<3>, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->nloc, s->ncomp, Q,
+                           s->coords, s->elems, qt, d->kind, d->params[0], d->params[1], d->params[2], s->elemvec);
+    else
+        hipLaunchKernelGGL(kElemRhs<2>, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->nloc, s->ncomp, Q,
+                           s->coords, s->elems, qt, d->kind, d->params[0], d->params[1], d->params[2], s->elemvec);
+    FEEL_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(kGatherVec, dim3(gridFor(s->ndof, 8192)), dim3(kBlock), 0, c->stream, s->ndof, s->rptr, s->rentry,
+                       s->elemvec, b->d, add);
+    FEEL_HIP_CHECK(hipGetLastError());
+    return FEEL_HIP_OK;
+    GUARD_END
+}
+
+// ============================================================== Dirichlet
+int feel_hip_rows_create(feel_hip_ctx* c, int64_t n, int64_t nrows, const int32_t* rows, feel_hip_rows** out) {
+    if (!c || !out || (nrows && !rows) || n <= 0 || nrows < 0) return setError(FEEL_HIP_EINVAL, "bad argument");
+    std::vector<uint8_t> flag(n, 0);
+    for (int64_t i = 0; i < nrows; ++i) {
+        if (rows[i] < 0 || rows[i] >= n) return setError(FEEL_HIP_EINVAL, "row index out of range");
+        flag[rows[i]] = 1;
+    }
+    auto* r = new feel_hip_rows();
+    r->ctx = c;
+    r->n = n;
+    r->nrows = nrows;
+    int rc = upload(c, &r->rows, rows, (size_t)nrows);
+    if (rc == FEEL_HIP_OK) rc = upload(c, &r->flag, flag.data(), flag.size());
+    if (rc != FEEL_HIP_OK) { feel_hip_rows_destroy(r); return rc; }
+    *out = r;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_rows_destroy(feel_hip_rows* r) {
+    if (!r) return FEEL_HIP_OK;
+    (void)hipStreamSynchronize(r->ctx->stream);
+    r->ctx->release(r->rows, sizeof(int32_t) * r->nrows);
+    r->ctx->release(r->flag, r->n);
+    delete r;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const feel_hip_vec* values, feel_hip_vec* rhs,
+                           int mode, double diag) {
+    if (!A || !rows) return setError(FEEL_HIP_EINVAL, "null argument");
+    feel_hip_graph* g = A->g;
+    if (rows->n != g->n) return setError(FEEL_HIP_EINVAL, "row set built for another size");
+    if (values && values->n != g->n) return setError(FEEL_HIP_EINVAL, "values size mismatch");
+    if (rhs && rhs->n != g->n) return setError(FEEL_HIP_EINVAL, "rhs size mismatch");
+    if (mode != FEEL_HIP_ON_ELIMINATION && mode != FEEL_HIP_ON_ELIMINATION_SYMMETRIC)
+        return setError(FEEL_HIP_EINVAL, "unknown elimination mode");
+    feel_hip_ctx* c = g->space->ctx;
+    const int grid = gridFor(g->n, 8192);
+    const double* gv = values ? values->d : nullptr;
+    double* b = rhs ? rhs->d : nullptr;
+    const int sym = mode == FEEL_HIP_ON_ELIMINATION_SYMMETRIC;
+    if (A->v64)
+        hipLaunchKernelGGL(kDirichlet<double>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, A->v64, b,
+                           rows->flag, gv, sym, diag, 1);
+    if (A->v32)
+        hipLaunchKernelGGL(kDirichlet<float>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, A->v32, b,
+                           rows->flag, gv, sym, diag, A->v64 ? 0 : 1);
+    FEEL_HIP_CHECK(hipGetLastError());
+    A->dinvValid = false;
+    return FEEL_HIP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,451 @@
+// HIP kernels of the device layer (gfx950, 64-wide wavefronts).
+//
+// Reductions never use floating-point atomics or grid-wide fences: a kernel
+// writes one partial per block (fixed grid of kReduceBlocks blocks), and every
+// block of the NEXT kernel re-reduces those partials in the same fixed order,
+// so all blocks see bit-identical scalars and results are run-to-run
+// deterministic.  Scalars that must persist are written by block 0 into
+// double-buffered slots indexed by the iteration number the host passes in.
+#pragma once
+#include "common.hpp"
+
+namespace feelpp_amd {
+
+// ---------------------------------------------------------------- reductions
+__device__ inline double waveReduceSum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// sum over the block (blockDim.x == kBlock); result valid in every thread
+__device__ inline double blockReduceSum(double v) {
+    __shared__ double sh[kBlock / 64];
+    __shared__ double total;
+    v = waveReduceSum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+#pragma unroll
+        for (int i = 0; i < kBlock / 64; ++i) s += sh[i];
+        total = s;
+    }
+    __syncthreads();
+    const double r = total;
+    __syncthreads();
+    return r;
+}
+
+// Reads the solver's done flag once per block and broadcasts it, so that every
+// thread of a block takes the same early exit (the flag may be set by block 0
+// of the running kernel while other blocks start).
+__device__ inline bool blockDone(const CgState* st) {
+    __shared__ int flag;
+    if (threadIdx.x == 0) flag = st->done;
+    __syncthreads();
+    return flag != 0;
+}
+
+__device__ inline double reducePartials(const double* __restrict__ part) {
+    double v = 0;
+    for (int i = threadIdx.x; i < kReduceBlocks; i += kBlock) v += part[i];
+    return blockReduceSum(v);
+}
+
+// ---------------------------------------------------------------- geometry
+// Affine map of a P1 simplex on the [-1,1] reference element
+// (feelpoly/geomap.hpp:2258-2355): K[:,a] = (x_{a+1} - x_0)/2, J = |det K|
+// (:1972-1976).  Returns Kinv (K^{-1}) and J.
+template <int DIM>
+__device__ inline double affineMap(const double* __restrict__ coords, const int32_t* __restrict__ el,
+                                   double Kinv[DIM][DIM]) {
+    double X[DIM + 1][DIM];
+#pragma unroll
+    for (int v = 0; v <= DIM; ++v)
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) X[v][c] = coords[(size_t)el[v] * DIM + c];
+    double K[DIM][DIM];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) K[r][a] = 0.5 * (X[a + 1][r] - X[0][r]);
+    if constexpr (DIM == 2) {
+        const double det = K[0][0] * K[1][1] - K[0][1] * K[1][0];
+        const double id = 1.0 / det;
+        Kinv[0][0] = K[1][1] * id; Kinv[0][1] = -K[0][1] * id;
+        Kinv[1][0] = -K[1][0] * id; Kinv[1][1] = K[0][0] * id;
+        return fabs(det);
+    } else {
+        const double c00 = K[1][1] * K[2][2] - K[1][2] * K[2][1];
+        const double c01 = K[1][2] * K[2][0] - K[1][0] * K[2][2];
+        const double c02 = K[1][0] * K[2][1] - K[1][1] * K[2][0];
+        const double det = K[0][0] * c00 + K[0][1] * c01 + K[0][2] * c02;
+        const double id = 1.0 / det;
+        Kinv[0][0] = c00 * id;
+        Kinv[1][0] = c01 * id;
+        Kinv[2][0] = c02 * id;
+        Kinv[0][1] = (K[0][2] * K[2][1] - K[0][1] * K[2][2]) * id;
+        Kinv[1][1] = (K[0][0] * K[2][2] - K[0][2] * K[2][0]) * id;
+        Kinv[2][1] = (K[0][1] * K[2][0] - K[0][0] * K[2][1]) * id;
+        Kinv[0][2] = (K[0][1] * K[1][2] - K[0][2] * K[1][1]) * id;
+        Kinv[1][2] = (K[0][2] * K[1][0] - K[0][0] * K[1][2]) * id;
+        Kinv[2][2] = (K[0][0] * K[1][1] - K[0][1] * K[1][0]) * id;
+        return fabs(det);
+    }
+}
+
+// Per-element factors of the scalar forms.  With grad phi = B grad_ref phi,
+// B = K^{-T} (polynomialset_impl.hpp:274-302), the element matrix
+//   A_ij = sum_q w_q J (kd grad phi_i . grad phi_j + km phi_i phi_j)
+//        = sum_{a<=b} F_ab Rsym_ab(i,j) + F_m M(i,j)
+// with C = B^T B = K^{-1} K^{-T}, F_ab = kd J C_ab, F_m = km J and the
+// reference tensors Rsym, M computed once per (order, quadrature).
+// Layout: 3D [C00 C11 C22 C01 C02 C12 | m | pad] (stride 8), 2D [C00 C11 C01 | m] (stride 4).
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void kGeoScalar(int64_t ne, const double* __restrict__ coords,
+                                                     const int32_t* __restrict__ elems,
+                                                     const double* __restrict__ kd, double kds,
+                                                     const double* __restrict__ km, double kms,
+                                                     double* __restrict__ F) {
+    constexpr int STRIDE = DIM == 3 ? 8 : 4;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+        double Ki[DIM][DIM];
+        const double J = affineMap<DIM>(coords, elems + e * (DIM + 1), Ki);
+        const double a = kds * (kd ? kd[e] : 1.0) * J;
+        const double m = kms * (km ? km[e] : 1.0) * J;
+        double C[DIM][DIM];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+            for (int j = 0; j < DIM; ++j) {
+                double s = 0;
+#pragma unroll
+                for (int r = 0; r < DIM; ++r) s += Ki[i][r] * Ki[j][r];
+                C[i][j] = s;
+            }
+        double* f = F + e * STRIDE;
+        if constexpr (DIM == 3) {
+            f[0] = a * C[0][0]; f[1] = a * C[1][1]; f[2] = a * C[2][2];
+            f[3] = a * C[0][1]; f[4] = a * C[0][2]; f[5] = a * C[1][2];
+            f[6] = m; f[7] = 0.0;
+        } else {
+            f[0] = a * C[0][0]; f[1] = a * C[1][1]; f[2] = a * C[0][1]; f[3] = m;
+        }
+    }
+}
+
+// Owner-computes scatter: one thread per CSR slot p sums the element entries
+// that land in p (ascending element order), each entry evaluated from the
+// element factors F and the reference tensors staged in LDS.
+template <int NN, int NS, int STRIDE>
+__global__ __launch_bounds__(kBlock) void kGatherScalar(int64_t nnz, const uint32_t* __restrict__ cptr,
+                                                        const uint32_t* __restrict__ centry,
+                                                        const double* __restrict__ F,
+                                                        const double* __restrict__ rtab,
+                                                        double* __restrict__ v64, float* __restrict__ v32,
+                                                        int add) {
+    __shared__ double R[(NS + 1) * NN];
+    for (int i = threadIdx.x; i < (NS + 1) * NN; i += kBlock) R[i] = rtab[i];
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t b = cptr[p], e = cptr[p + 1];
+        double s = 0.0;
+        for (uint32_t k = b; k < e; ++k) {
+            const uint32_t c = centry[k];
+            const uint32_t el = c / NN;
+            const uint32_t ij = c - el * NN;
+            const double* f = F + (size_t)el * STRIDE;
+            double a = 0.0;
+#pragma unroll
+            for (int t = 0; t <= NS; ++t) a += f[t] * R[t * NN + ij];
+            s += a;
+        }
+        if (v64) {
+            const double v = add ? v64[p] + s : s;
+            v64[p] = v;
+            if (v32) v32[p] = (float)v;
+        } else if (v32) {
+            v32[p] = add ? (float)((double)v32[p] + s) : (float)s;
+        }
+    }
+}
+
+// Generic scatter of materialised element matrices (elem[e*NN + ij]).
+static __global__ __launch_bounds__(kBlock) void kGatherElemMat(int64_t nnz, const uint32_t* __restrict__ cptr,
+                                                         const uint32_t* __restrict__ centry,
+                                                         const double* __restrict__ elem,
+                                                         double* __restrict__ v64, float* __restrict__ v32,
+                                                         int add) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (uint32_t k = cptr[p]; k < cptr[p + 1]; ++k) s += elem[centry[k]];
+        if (v64) {
+            const double v = add ? v64[p] + s : s;
+            v64[p] = v;
+            if (v32) v32[p] = (float)v;
+        } else if (v32) {
+            v32[p] = add ? (float)((double)v32[p] + s) : (float)s;
+        }
+    }
+}
+
+// Element vectors of f*id(v) (linearformcontext.hpp:294-305):
+// b_e(c*nl+i) = sum_q w_q J f_c(x_q) phi_i(q), x_q = sum_v X_v lambda_v(q).
+// qtab layout: w[Q] | phi[nl*Q] | lambda[(DIM+1)*Q].
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void kElemRhs(int64_t ne, int nl, int ncomp, int Q,
+                                                   const double* __restrict__ coords,
+                                                   const int32_t* __restrict__ elems,
+                                                   const double* __restrict__ qtab, int kind, double p0,
+                                                   double p1, double p2, double* __restrict__ be) {
+    const double* w = qtab;
+    const double* phi = qtab + Q;
+    const double* lam = phi + (size_t)nl * Q;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t* el = elems + e * (DIM + 1);
+        double Ki[DIM][DIM];
+        const double J = affineMap<DIM>(coords, el, Ki);
+        double* out = be + e * (size_t)nl * ncomp;
+        for (int i = 0; i < nl * ncomp; ++i) out[i] = 0.0;
+        for (int q = 0; q < Q; ++q) {
+            double f[3] = {p0, p1, p2};
+            if (kind == FEEL_HIP_SRC_SINSIN) {
+                double x[DIM];
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) x[c] = 0.0;
+                for (int v = 0; v <= DIM; ++v)
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) x[c] += coords[(size_t)el[v] * DIM + c] * lam[v * Q + q];
+                f[0] = p0 * sin(M_PI * x[0]) * sin(M_PI * x[1]);
+            }
+            const double wJ = w[q] * J;
+            for (int c = 0; c < ncomp; ++c) {
+                const double fc = (kind == FEEL_HIP_SRC_VECTOR) ? f[c] : f[0];
+                for (int i = 0; i < nl; ++i) out[c * nl + i] += wJ * fc * phi[i * Q + q];
+            }
+        }
+    }
+}
+
+static __global__ __launch_bounds__(kBlock) void kGatherVec(int64_t n, const uint32_t* __restrict__ rptr,
+                                                     const uint32_t* __restrict__ rentry,
+                                                     const double* __restrict__ be, double* __restrict__ b,
+                                                     int add) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (uint32_t k = rptr[r]; k < rptr[r + 1]; ++k) s += be[rentry[k]];
+        b[r] = add ? b[r] + s : s;
+    }
+}
+
+// ---------------------------------------------------------------- Dirichlet
+// MatZeroRows / MatZeroRowsColumns semantics (matrixpetsc.cpp:1764-1850).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kDirichlet(int64_t n, const int32_t* __restrict__ ia,
+                                                     const int32_t* __restrict__ ja, T* __restrict__ val,
+                                                     double* __restrict__ rhs, const uint8_t* __restrict__ flag,
+                                                     const double* __restrict__ g, int symmetric, double diag,
+                                                     int touchRhs) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int b = ia[r], e = ia[r + 1];
+        if (flag[r]) {
+            for (int p = b; p < e; ++p) val[p] = (ja[p] == r) ? (T)diag : (T)0;
+            if (touchRhs && rhs) rhs[r] = diag * (g ? g[r] : 0.0);
+        } else if (symmetric) {
+            double corr = 0.0;
+            for (int p = b; p < e; ++p) {
+                const int c = ja[p];
+                if (flag[c]) {
+                    if (g) corr += (double)val[p] * g[c];
+                    val[p] = (T)0;
+                }
+            }
+            if (touchRhs && rhs && g) rhs[r] -= corr;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- CG kernels
+// y = A x with L lanes per row (grid-stride over row groups).  When part is
+// given, also writes the block partial of sum_r x_r y_r (= p.Ap).
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void kSpmv(int64_t n, const int32_t* __restrict__ ia,
+                                                const int32_t* __restrict__ ja, const T* __restrict__ val,
+                                                const T* __restrict__ x, T* __restrict__ y,
+                                                double* __restrict__ part, const CgState* __restrict__ st) {
+    if (st && blockDone(st)) return;
+    const int lane = threadIdx.x & (L - 1);
+    constexpr int RPB = kBlock / L;
+    double dot = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / L; r < n; r += (int64_t)gridDim.x * RPB) {
+        const int b = ia[r], e = ia[r + 1];
+        T s = 0;
+        for (int k = b + lane; k < e; k += L) s += val[k] * x[ja[k]];
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+        if (lane == 0) {
+            y[r] = s;
+            if (part) dot += (double)x[r] * (double)s;
+        }
+    }
+    if (part) {
+        const double t = blockReduceSum(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
+// p = z + (beta/beta_prev) p, preceded by the convergence test on the
+// preconditioned residual norm (KSPConvergedDefault with UIRNorm).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const T* __restrict__ z,
+                                                       T* __restrict__ p, const double* __restrict__ zrPart,
+                                                       const double* __restrict__ zzPart, CgState* st,
+                                                       double* betaSlots) {
+    if (blockDone(st)) return;
+    const double beta = reducePartials(zrPart);
+    const double dp = sqrt(reducePartials(zzPart));
+    const double ttol = it == 0 ? fmax(st->rtol * dp, st->atol) : st->ttol;
+    const double rnorm0 = it == 0 ? dp : st->rnorm0;
+    int reason = 0;
+    if (dp != dp) reason = -9;
+    else if (dp <= ttol) reason = dp < st->atol ? 3 : 2;
+    else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
+    else if (it >= st->maxit) reason = -3;
+    else if (beta == 0.0) reason = 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->dp = dp;
+        if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
+        if (reason) { st->done = 1; st->reason = reason; st->its = it; }
+        betaSlots[it & 1] = beta;
+    }
+    if (reason) return;
+    const double bb = it == 0 ? 0.0 : beta / betaSlots[(it - 1) & 1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = it == 0 ? z[i] : (T)((double)z[i] + bb * (double)p[i]);
+}
+
+// x += a p, r -= a w, z = D^{-1} r; partials of z.r and z.z.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __restrict__ x, T* __restrict__ r,
+                                                    T* __restrict__ z, const T* __restrict__ p,
+                                                    const T* __restrict__ w, const T* __restrict__ dinv,
+                                                    const double* __restrict__ dpiPart, double* __restrict__ zrPart,
+                                                    double* __restrict__ zzPart, CgState* st,
+                                                    const double* betaSlots, double* dpiSlots) {
+    if (blockDone(st)) return;
+    const double dpi = reducePartials(dpiPart);
+    const bool indefinite = dpi == 0.0 || (it > 0 && dpi * dpiSlots[(it - 1) & 1] <= 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dpiSlots[it & 1] = dpi;
+        if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
+    }
+    if (indefinite) return;
+    const double a = betaSlots[it & 1] / dpi;
+    double zr = 0.0, zz = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        x[i] = (T)((double)x[i] + a * (double)p[i]);
+        const T ri = (T)((double)r[i] - a * (double)w[i]);
+        r[i] = ri;
+        const T zi = dinv[i] * ri;
+        z[i] = zi;
+        zr += (double)zi * (double)ri;
+        zz += (double)zi * (double)zi;
+    }
+    zr = blockReduceSum(zr);
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) { zrPart[blockIdx.x] = zr; zzPart[blockIdx.x] = zz; }
+}
+
+// r = b - w (w = A x0), z = D^{-1} r; partials of z.r and z.z.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kCgInit(int64_t n, const T* __restrict__ b, const T* __restrict__ w,
+                                                  T* __restrict__ r, T* __restrict__ z, const T* __restrict__ dinv,
+                                                  double* __restrict__ zrPart, double* __restrict__ zzPart) {
+    double zr = 0.0, zz = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T ri = b[i] - w[i];
+        r[i] = ri;
+        const T zi = dinv[i] * ri;
+        z[i] = zi;
+        zr += (double)zi * (double)ri;
+        zz += (double)zi * (double)zi;
+    }
+    zr = blockReduceSum(zr);
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) { zrPart[blockIdx.x] = zr; zzPart[blockIdx.x] = zz; }
+}
+
+// r64 = b - w; r32 = s * r64 (for the FP32 inner solve); partial of ||D^{-1} r64||^2.
+static __global__ __launch_bounds__(kBlock) void kResidual64(int64_t n, const double* __restrict__ b,
+                                                      const double* __restrict__ w, const double* __restrict__ dinv,
+                                                      double* __restrict__ r, double* __restrict__ zzPart) {
+    double zz = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = b[i] - w[i];
+        r[i] = ri;
+        const double zi = dinv[i] * ri;
+        zz += zi * zi;
+    }
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) zzPart[blockIdx.x] = zz;
+}
+
+static __global__ __launch_bounds__(kBlock) void kScaleToF32(int64_t n, const double* __restrict__ a, double s,
+                                                      float* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)(s * a[i]);
+}
+
+static __global__ __launch_bounds__(kBlock) void kAxpyF32(int64_t n, double a, const float* __restrict__ d,
+                                                   double* __restrict__ x) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += a * (double)d[i];
+}
+
+// Jacobi: D^{-1} from the stored diagonal; PCJACOBI treats a zero diagonal as 1.
+static __global__ __launch_bounds__(kBlock) void kDiagInv(int64_t n, const int32_t* __restrict__ diagpos,
+                                                   const double* __restrict__ v64, const float* __restrict__ v32,
+                                                   double* __restrict__ d64, float* __restrict__ d32) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (d64) {
+            const double d = v64 ? v64[diagpos[i]] : (double)v32[diagpos[i]];
+            d64[i] = d != 0.0 ? 1.0 / d : 1.0;
+        }
+        if (d32) {
+            const double d = v32 ? (double)v32[diagpos[i]] : v64[diagpos[i]];
+            d32[i] = (float)(d != 0.0 ? 1.0 / d : 1.0);
+        }
+    }
+}
+
+static __global__ __launch_bounds__(kBlock) void kCastF32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+static __global__ __launch_bounds__(kBlock) void kDotPartial(int64_t n, const double* __restrict__ a,
+                                                      const double* __restrict__ b, double* __restrict__ part) {
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += a[i] * b[i];
+    s = blockReduceSum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// sequential ADD of host-supplied entries at known CSR slots (MatrixSparse::addMatrix slow path)
+static __global__ void kAddAt(int64_t m, const int64_t* __restrict__ pos, const double* __restrict__ vals,
+                       double* __restrict__ v64, float* __restrict__ v32) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int64_t k = 0; k < m; ++k) {
+        if (v64) {
+            v64[pos[k]] += vals[k];
+            if (v32) v32[pos[k]] = (float)v64[pos[k]];
+        } else if (v32) {
+            v32[pos[k]] = (float)((double)v32[pos[k]] + vals[k]);
+        }
+    }
+}
+
+}  // namespace feelpp_amd

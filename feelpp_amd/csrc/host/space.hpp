@@ -1,0 +1,73 @@
+// Host-side mesh, DofTable and sparsity graph for continuous Lagrange Pk on
+// simplices.  These reproduce Feel++'s numbering and pattern bit for bit
+// (first-touch dof numbering, COUPLED stencil, sorted CSR) with array-based
+// algorithms instead of std::map / std::set.
+#pragma once
+#include <cstdint>
+#include <iosfwd>
+#include <string>
+#include <vector>
+
+namespace feelpp_amd {
+
+struct SimplexMesh {
+    int dim = 0;
+    int64_t nv = 0, ne = 0;
+    std::vector<double> coords;    // nv x dim
+    std::vector<int32_t> elems;    // ne x (dim+1), vertex indices in point-id order
+};
+
+// Seeded synthetic meshes (SplitMix64).  Interior vertices move by
+// h*U(-jitter,jitter) per coordinate, drawn in vertex-id order (x fastest).
+SimplexMesh unitSquareMesh(int n, double jitter, uint64_t seed);
+SimplexMesh kuhnBoxMesh(int nx, int ny, int nz, double lx, double ly, double lz, double jitter,
+                        uint64_t seed);
+
+// Gmsh MSH 2.2 (ASCII) I/O with Feel++ import semantics (feelfilters/importergmsh.hpp):
+// element id = insertion order of elements of the mesh dimension (:2631,:2851),
+// point id = file node id (:2050; kept as an order-preserving relabelling).
+void writeMsh22(const SimplexMesh& m, std::ostream& os);
+SimplexMesh readMsh22(std::istream& is, int dim);
+
+// Per-element random fields
+std::vector<double> lognormalField(int64_t n, double mu, double sigma, uint64_t seed);
+std::vector<double> uniformField(int64_t n, double lo, double hi, bool logscale, uint64_t seed);
+
+struct DofTable {
+    int dim = 0, order = 0, ncomp = 1;
+    int nloc = 0;     // scalar local dofs
+    int nldof = 0;    // nloc * ncomp
+    int64_t ne = 0, ndof = 0;
+    std::vector<int32_t> elem2dof;   // ne x nldof, local index nloc*c + l
+};
+DofTable buildDofTable(const SimplexMesh& m, int order, int ncomp);
+
+struct GraphCSR {
+    int64_t nrows = 0, nnz = 0;
+    std::vector<int32_t> ia, ja;
+};
+GraphCSR buildGraph(const DofTable& dt, int nthreads = 0);
+
+// Boundary dofs of on(boundaryfaces(mesh)) (sel = 0) or of the faces whose
+// vertices all lie on x == 0 (sel = 1).  Returns a 0/1 flag per dof.
+std::vector<uint8_t> boundaryDofs(const SimplexMesh& m, const DofTable& dt, int sel);
+
+// Gather map for the owner-computes scatter: for CSR slot p, the element-matrix
+// entries (e * nldof^2 + i * nldof + j) that add into it, in ascending element
+// order (the order in which MatSetValues(ADD) would add them).
+struct GatherMap {
+    std::vector<int64_t> ptr;     // nnz + 1
+    std::vector<uint32_t> entry;  // sum of contributions
+};
+GatherMap buildGatherMap(const DofTable& dt, const GraphCSR& g, int nthreads = 0);
+
+// dof -> (element, local dof) incidence in element order: entry = e*nldof + l
+struct Incidence {
+    std::vector<int64_t> ptr;     // ndof + 1
+    std::vector<uint32_t> entry;
+};
+Incidence buildIncidence(const DofTable& dt);
+
+int defaultThreads();
+
+}  // namespace feelpp_amd

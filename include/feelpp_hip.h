@@ -1,0 +1,192 @@
+/*
+ * feelpp_hip.h -- C-ABI of the MI355X (gfx950) device layer for Feel++'s
+ * form assembly and Jacobi-CG solve, continuous Lagrange Pk on simplices.
+ *
+ * This is the boundary a Feel++ "hip" backend kind binds: every entry point
+ * below replaces one step of the reference path, cited as file:line relative
+ * to the Feel++ source tree (feelpp/feel/...).  Plain pointers and sizes only;
+ * host arrays are borrowed for the duration of a call; device memory is owned
+ * by the handles.  Every function returns 0 on success or a negative status;
+ * feel_hip_last_error() describes the last failure of the calling thread.
+ * Solver non-convergence is not an error: it is reported in
+ * feel_hip_solve_data, as SolveData is (feelalg/solverlinear.hpp:81).
+ *
+ * Threading: a context owns one HIP stream; calls on one context must not
+ * overlap.  Use one context per host thread.
+ */
+#ifndef FEELPP_HIP_H
+#define FEELPP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct feel_hip_ctx feel_hip_ctx;
+typedef struct feel_hip_space feel_hip_space;
+typedef struct feel_hip_graph feel_hip_graph;
+typedef struct feel_hip_mat feel_hip_mat;
+typedef struct feel_hip_vec feel_hip_vec;
+typedef struct feel_hip_rows feel_hip_rows;
+
+enum {
+    FEEL_HIP_OK = 0,
+    FEEL_HIP_EINVAL = -1,      /* bad argument (shape, order, null handle) */
+    FEEL_HIP_EUNSUPPORTED = -2,/* expression / space outside the supported subset */
+    FEEL_HIP_EDEVICE = -3,     /* HIP runtime error */
+    FEEL_HIP_ENOMEM = -4,      /* device allocation failed */
+    FEEL_HIP_EPATTERN = -5     /* an element entry falls outside the sparsity graph */
+};
+
+/* precision of stored matrix values / of the solve */
+enum { FEEL_HIP_FP64 = 1, FEEL_HIP_FP32 = 2, FEEL_HIP_FP64_FP32 = 3 };
+
+/* Expression subset of integrate(_range=elements(mesh), _expr=...)
+ * (feelvf/integrate.hpp:64-98; bilinearformcontext.hpp:455-628):
+ *   MASS        mass_scale * idv(m) * idt(u) * id(v)
+ *   DIFFUSION   diff_scale * inner(idv(k) * gradt(u), grad(v))   (feelpde/cg_laplacian.hpp:124-125)
+ *   ELASTICITY  inner(lambda tr(eps(u)) I + 2 mu eps(u), grad(v)), lambda/mu from P0 E, nu
+ *               (quickstart/qs_elasticity.cpp:41-63)
+ * MASS|DIFFUSION may be combined.  P0 coefficients are per-element device
+ * vectors (nullptr = 1). */
+enum { FEEL_HIP_EXPR_MASS = 1, FEEL_HIP_EXPR_DIFFUSION = 2, FEEL_HIP_EXPR_ELASTICITY = 4 };
+
+typedef struct {
+    int kind;                       /* FEEL_HIP_EXPR_* bitmask */
+    int quad_order;                 /* "im(d,order,simplex)", pointsetquadrature.hpp:85-159 */
+    int precision;                  /* which value arrays to write: FEEL_HIP_FP64 / FP32 / both */
+    double diff_scale;
+    const feel_hip_vec* diff_coef;  /* length nelements or null */
+    double mass_scale;
+    const feel_hip_vec* mass_coef;  /* length nelements or null */
+    const feel_hip_vec* young;      /* elasticity: E per element */
+    const feel_hip_vec* poisson;    /* elasticity: nu per element */
+} feel_hip_bilinear_desc;
+
+/* Source of integrate(_range=elements(mesh), _expr=f*id(v)) (linearformcontext.hpp:294-305) */
+enum { FEEL_HIP_SRC_CONST = 0, FEEL_HIP_SRC_SINSIN = 1, FEEL_HIP_SRC_VECTOR = 2 };
+typedef struct {
+    int kind;            /* CONST: f = p[0]; SINSIN: f = p[0] sin(pi x) sin(pi y); VECTOR: f = (p0,p1,p2) */
+    int quad_order;
+    double params[4];
+} feel_hip_linear_desc;
+
+/* Dirichlet elimination modes, on.type (feelpp/feel/options.cpp:338, feelalg/enums.hpp:59-62) */
+enum { FEEL_HIP_ON_ELIMINATION = 0, FEEL_HIP_ON_ELIMINATION_SYMMETRIC = 1 };
+
+typedef struct {
+    double rtol;        /* ksp-rtol: ||B r_k|| <= max(rtol ||B r_0||, atol) (UIRNorm, solverlinearpetsc.cpp:592,610) */
+    double atol;        /* ksp-atol */
+    double dtol;        /* ksp-dtol */
+    int maxit;          /* ksp-maxit */
+    int precision;      /* FEEL_HIP_FP64: plain FP64 Jacobi-CG; FEEL_HIP_FP32: FP32 Jacobi-CG inside
+                           FP64 iterative refinement (FP64 residual and stopping test) */
+    double inner_rtol;  /* FP32 inner-solve relative reduction per refinement step */
+    int inner_maxit;
+    int max_outer;
+    int check_every;    /* host polls the device convergence flag every this many iterations (0 = auto) */
+} feel_hip_solve_opts;
+
+typedef struct {
+    int converged;      /* SolveData::isConverged */
+    int its;            /* total Jacobi-CG iterations (A*p products), all refinement steps included */
+    double residual;    /* final ||B r|| (preconditioned norm), FP64 */
+    int reason;         /* KSPConvergedReason-like: 2 rtol, 3 atol, -3 maxit, -4 dtol, -8 indefinite, -9 nan */
+    int outer_its;      /* refinement steps (1 for FP64) */
+    double rnorm0;      /* initial ||B r_0|| */
+} feel_hip_solve_data;
+
+/* ---- lifetime ------------------------------------------------------------ */
+const char* feel_hip_last_error(void);
+int feel_hip_ctx_create(int device, feel_hip_ctx** out);
+int feel_hip_ctx_destroy(feel_hip_ctx* ctx);
+int feel_hip_ctx_synchronize(feel_hip_ctx* ctx);
+/* HIP stream (hipStream_t) the context launches on */
+void* feel_hip_ctx_stream(feel_hip_ctx* ctx);
+/* Device memory held by the context's handles, in bytes */
+int64_t feel_hip_ctx_device_bytes(feel_hip_ctx* ctx);
+
+/* Kernel timing with HIP events on the context stream: when enabled, every
+ * SpMV launch of a solve is bracketed by events.  name: "spmv_fp64",
+ * "spmv_fp32", "assemble_gather", "assemble_geo". */
+int feel_hip_ctx_set_timing(feel_hip_ctx* ctx, int enable);
+int feel_hip_ctx_timing(feel_hip_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
+int feel_hip_ctx_reset_timing(feel_hip_ctx* ctx);
+
+/* ---- mesh + function space ----------------------------------------------
+ * Replaces the upload of a FunctionSpace: mesh geometry (Mesh<Simplex<d>>,
+ * feeldiscr/meshimpl.hpp:163-240) and the DofTable local-to-global map
+ * (feeldiscr/doftable.hpp:732-750).  elem2dof is ne x (nloc*ncomp), local index
+ * nloc*c + l (doftable.hpp:1213-1300). */
+int feel_hip_space_create(feel_hip_ctx* ctx, int dim, int order, int ncomp, int64_t nverts,
+                          const double* coords, int64_t nelems, const int32_t* elems,
+                          const int32_t* elem2dof, int64_t ndof, feel_hip_space** out);
+int feel_hip_space_destroy(feel_hip_space* s);
+int64_t feel_hip_space_ndof(const feel_hip_space* s);
+int64_t feel_hip_space_nelements(const feel_hip_space* s);
+
+/* ---- sparsity graph ------------------------------------------------------
+ * Replaces MatrixPetsc::init(m, n, ..., graph) (feelalg/matrixpetsc.cpp:355-415):
+ * uploads Feel++'s own CSR graph (GraphCSR::close, feelalg/graphcsr.cpp:663-1021)
+ * and derives the owner-computes scatter map of the space's element entries. */
+int feel_hip_graph_create(feel_hip_space* space, int64_t nrows, const int32_t* ia, const int32_t* ja,
+                          feel_hip_graph** out);
+int feel_hip_graph_destroy(feel_hip_graph* g);
+int64_t feel_hip_graph_nnz(const feel_hip_graph* g);
+
+/* ---- matrices (MatrixSparse<double>, feelalg/matrixsparse.hpp:97) -------- */
+int feel_hip_mat_create(feel_hip_graph* g, int precision, feel_hip_mat** out);   /* Backend::newMatrix, backend.hpp:349-440 */
+int feel_hip_mat_destroy(feel_hip_mat* A);
+int feel_hip_mat_zero(feel_hip_mat* A);                                          /* zero(), matrixsparse.hpp:420 */
+/* addMatrix(rows, nrows, cols, ncols, data) ADD semantics, data row-major (matrixsparse.hpp:496-498) */
+int feel_hip_mat_add_matrix(feel_hip_mat* A, int nrows, const int32_t* rows, int ncols, const int32_t* cols,
+                            const double* data);
+int feel_hip_mat_get_values(feel_hip_mat* A, int precision, void* host_values);  /* nnz values in CSR order */
+int feel_hip_mat_set_values(feel_hip_mat* A, const double* host_values);        /* FP64 values (and FP32 copy) */
+int feel_hip_mat_mult(feel_hip_mat* A, const feel_hip_vec* x, feel_hip_vec* y);  /* multVector, matrixsparse.hpp:530 */
+
+/* ---- vectors (Vector<double>, feelalg/vector.hpp:143-576) ------------------ */
+int feel_hip_vec_create(feel_hip_ctx* ctx, int64_t n, feel_hip_vec** out);      /* Backend::newVector, backend.hpp:571-592 */
+int feel_hip_vec_destroy(feel_hip_vec* v);
+int feel_hip_vec_set(feel_hip_vec* v, const double* host);                      /* host -> device */
+int feel_hip_vec_get(const feel_hip_vec* v, double* host);                      /* device -> host */
+int feel_hip_vec_zero(feel_hip_vec* v);
+int feel_hip_vec_dot(const feel_hip_vec* a, const feel_hip_vec* b, double* out); /* dot(), vector.hpp:544 */
+int64_t feel_hip_vec_size(const feel_hip_vec* v);
+/* raw device pointer (double*) of the vector's storage */
+void* feel_hip_vec_device_ptr(feel_hip_vec* v);
+
+/* ---- assembly -------------------------------------------------------------
+ * form2 = integrate(elements(mesh), expr): BilinearForm::operator= / +=
+ * (feelvf/bilinearform.hpp:1437-1468) -> Integrator::assemble
+ * (feelvf/integrator.hpp:975-1211) -> element matrices + ADD scatter
+ * (bilinearformcontext.hpp:772-889, matrixpetsc.cpp:919-937).
+ * add = 0 replaces the values (operator=), add = 1 accumulates (operator+=). */
+int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* desc, int add);
+/* form1 = integrate(elements(mesh), f*id(v)) (feelvf/linearform.hpp:1214) */
+int feel_hip_assemble_linear(feel_hip_space* space, const feel_hip_linear_desc* desc, feel_hip_vec* b,
+                             int add);
+
+/* ---- Dirichlet ------------------------------------------------------------
+ * on(_range=..., _rhs=l, _element=u, _expr=g) -> MatrixSparse::zeroRows
+ * (feelvf/on.hpp:457-701, feelalg/matrixpetsc.cpp:1764-1850).  The row set is
+ * uploaded once and stays resident. */
+int feel_hip_rows_create(feel_hip_ctx* ctx, int64_t n, int64_t nrows, const int32_t* rows, feel_hip_rows** out);
+int feel_hip_rows_destroy(feel_hip_rows* r);
+/* values: g at every dof (length n) or null for g = 0 */
+int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const feel_hip_vec* values,
+                           feel_hip_vec* rhs, int mode, double diag);
+
+/* ---- solve ----------------------------------------------------------------
+ * BilinearFormBase::solve (feelvf/bilinearformbase.hpp:446-466) ->
+ * Backend::solve -> SolverLinearPetsc::solve (feelalg/solverlinearpetsc.cpp:575-640)
+ * with ksp-type=cg, pc-type=jacobi.  x holds the initial guess on entry. */
+void feel_hip_solve_opts_default(feel_hip_solve_opts* o);
+int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, const feel_hip_solve_opts* opts,
+                       feel_hip_solve_data* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEELPP_HIP_H */

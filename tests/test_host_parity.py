@@ -1,0 +1,108 @@
+"""Host side of the product vs the oracle (CPU): meshes, Feel++ DofTable
+numbering, Stencil CSR pattern and boundary-dof sets must be bit-exact;
+node sets, quadrature and reference tensors must agree to round-off."""
+import os
+
+import numpy as np
+import pytest
+
+import feelpp_amd as fa
+
+
+@pytest.mark.parametrize("n,jit,seed", [(4, 0.0, 0), (16, 0.2, 7), (128, 0.2, 1)])
+def test_square_mesh_bit_exact(orc, n, jit, seed):
+    c1, e1 = fa.mesh_unit_square(n, jit, seed)
+    c2, e2 = orc.mesh_square(n, jit, seed)
+    assert (e1 == e2).all() and np.array_equal(c1, c2)
+
+
+@pytest.mark.parametrize("dims", [(3, 3, 3, 1, 1, 1), (5, 2, 3, 1.0, 0.25, 0.25), (12, 12, 12, 1, 1, 1)])
+def test_kuhn_mesh_bit_exact(orc, dims):
+    nx, ny, nz, lx, ly, lz = dims
+    c1, e1 = fa.mesh_kuhn_box(nx, ny, nz, lx, ly, lz, 0.15, 42)
+    c2, e2 = orc.mesh_kuhn(nx, ny, nz, lx, ly, lz, 0.15, 42)
+    assert (e1 == e2).all() and np.array_equal(c1, c2)
+
+
+def test_random_fields_bit_exact(orc):
+    assert np.array_equal(fa.lognormal(1001, 0.0, 1.0, 9), orc.lognormal(1001, 0.0, 1.0, 9))
+    assert np.array_equal(fa.uniform(777, 1.0, 100.0, 3, log=True), orc.uniform(777, 1.0, 100.0, 3, log=True))
+
+
+CASES = [(2, 1, 1), (2, 2, 1), (2, 1, 2), (3, 1, 1), (3, 2, 1), (3, 3, 1), (3, 1, 3), (3, 2, 3), (3, 3, 3)]
+
+
+def mesh_for(orc, dim, n=5):
+    return orc.mesh_square(n, 0.2, 3) if dim == 2 else orc.mesh_kuhn(n, n - 1, n - 2, jitter=0.15, seed=3)
+
+
+@pytest.mark.parametrize("dim,order,ncomp", CASES)
+def test_doftable_graph_boundary_bit_exact(orc, dim, order, ncomp):
+    coords, elems = mesh_for(orc, dim)
+    e2d_o, nd_o = orc.doftable(dim, order, ncomp, elems)
+    e2d_p, nd_p = fa.doftable(dim, order, ncomp, elems, coords.shape[0])
+    assert nd_o == nd_p and np.array_equal(e2d_o, e2d_p)
+    ia_o, ja_o = orc.graph(e2d_o, nd_o)
+    ia_p, ja_p = fa.graph(e2d_p, nd_p)
+    assert np.array_equal(ia_o, ia_p) and np.array_equal(ja_o, ja_p)
+    for sel in (0, 1):
+        b_o = orc.boundary_dofs(dim, order, ncomp, coords, elems, e2d_o, nd_o, sel)
+        b_p = fa.boundary_dofs(dim, order, ncomp, coords, elems, e2d_p, nd_p, sel)
+        assert np.array_equal(b_o, b_p)
+
+
+def test_p2_kuhn_graph_large_bit_exact(orc):
+    # a 12^3 P2 space (10 368 tets, ~15k dofs): pattern from both implementations
+    coords, elems = orc.mesh_kuhn(12, jitter=0.15, seed=2)
+    e2d_o, nd_o = orc.doftable(3, 2, 1, elems)
+    e2d_p, nd_p = fa.doftable(3, 2, 1, elems, coords.shape[0])
+    assert np.array_equal(e2d_o, e2d_p)
+    ia_o, ja_o = orc.graph(e2d_o, nd_o)
+    ia_p, ja_p = fa.graph(e2d_p, nd_p)
+    assert np.array_equal(ia_o, ia_p) and np.array_equal(ja_o, ja_p)
+    n = 12
+    assert len(ja_p) == 230 * n ** 3 + 138 * n ** 2 + 24 * n + 1
+
+
+def test_c1_full_size_pattern_bit_exact(orc):
+    # the 128x128 P1 triangle space (32 768 elements, 16 641 dofs, 115 457 nnz)
+    coords, elems = orc.mesh_square(128, 0.2, 1)
+    e2d_o, nd_o = orc.doftable(2, 1, 1, elems)
+    e2d_p, nd_p = fa.doftable(2, 1, 1, elems, coords.shape[0])
+    ia_o, ja_o = orc.graph(e2d_o, nd_o)
+    ia_p, ja_p = fa.graph(e2d_p, nd_p)
+    assert nd_p == 16641 and len(ja_p) == 115457
+    assert np.array_equal(e2d_o, e2d_p) and np.array_equal(ia_o, ia_p) and np.array_equal(ja_o, ja_p)
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3)])
+def test_reference_element_agrees(orc, dim, order):
+    assert np.abs(fa.lagrange_nodes(dim, order) - orc.lagrange_nodes(dim, order)).max() < 1e-15
+    for q in range(0, 9):
+        p1, w1 = fa.quadrature(dim, q)
+        p2, w2 = orc.quadrature(dim, q)
+        assert np.array_equal(p1, p2) and np.array_equal(w1, w2)
+    # reference tensors vs the oracle's basis tabulated at the same quadrature
+    qo = 2 * order
+    R, M, m = fa.reference_tensors(dim, order, qo)
+    pts, w = orc.quadrature(dim, qo)
+    phi, dphi = orc.lagrange_eval(dim, order, pts)
+    Rr = np.einsum("q,iqa,jqb->abij", w, dphi, dphi).reshape(dim * dim, phi.shape[0], phi.shape[0])
+    assert np.abs(R - Rr).max() < 1e-13 * max(1, np.abs(Rr).max())
+    assert np.abs(M - np.einsum("q,iq,jq->ij", w, phi, phi)).max() < 1e-14
+    assert np.abs(m - phi @ w).max() < 1e-14
+
+
+def test_msh22_roundtrip(tmp_path):
+    coords, elems = fa.mesh_kuhn_box(3, 2, 2, jitter=0.1, seed=4)
+    p = os.path.join(str(tmp_path), "m.msh")
+    fa.write_msh22(p, coords, elems)
+    c2, e2 = fa.read_msh22(p, 3)
+    assert np.array_equal(elems, e2) and np.abs(coords - c2).max() < 1e-15
+
+
+def test_host_rejects_bad_input():
+    with pytest.raises(fa.FeelHipError):
+        fa.doftable(3, 2, 1, np.array([[0, 1, 2, 99]], np.int32), 4)
+    with pytest.raises(fa.FeelHipError):
+        fa.doftable(3, 9, 1, np.array([[0, 1, 2, 3]], np.int32), 4)
