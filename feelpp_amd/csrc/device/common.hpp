@@ -30,8 +30,10 @@ struct CgState {
     int pad[3];
 };
 
-constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying kernels
+constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying vector kernels
 constexpr int kBlock = 256;
+constexpr int kSpmvGrid = 2048;       // grid of the solver SpMV (one p.Ap partial per block)
+constexpr int kSpmvBlock = 1024;
 
 }  // namespace feelpp_amd
 
@@ -46,7 +48,7 @@ struct feel_hip_ctx {
     size_t evUsed = 0;
     std::map<std::string, std::pair<int64_t, double>> timers;
     // CG scratch: partial sums (3 arrays of kReduceBlocks doubles) and state
-    double* part = nullptr;
+    double* part = nullptr;   // 3 x kReduceBlocks + kSpmvGrid doubles
     feelpp_amd::CgState* state = nullptr;
     feelpp_amd::CgState* hstate = nullptr;   // pinned host mirror
 
@@ -90,6 +92,13 @@ struct feel_hip_graph {
     uint32_t* centry = nullptr;   // element-matrix entry ids e*nldof^2 + i*nldof + j
     int64_t ncontrib = 0;
     std::vector<int32_t> hia, hja;
+    // SELL-C-sigma image used by the solver (permuted numbering)
+    int64_t nslices = 0, nslots = 0;
+    uint32_t* slicePtr = nullptr;
+    int32_t* scol = nullptr;
+    int32_t* slot2csr = nullptr;
+    int32_t* perm = nullptr;      // new -> old row
+    int32_t* sdiag = nullptr;     // diagonal slot per new row
 };
 
 struct feel_hip_mat {
@@ -102,9 +111,14 @@ struct feel_hip_mat {
     bool dinvValid = false;
     double* elemmat = nullptr;    // materialised element matrices (elasticity path)
     size_t elemmatBytes = 0;
+    // SELL values for the solver, refreshed from the CSR values when stale
+    double* s64 = nullptr;
+    float* s32 = nullptr;
+    bool sell64Valid = false, sell32Valid = false;
     // solver work vectors, allocated on first use and kept with the matrix
     double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *w64 = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *w32 = nullptr, *x32 = nullptr, *b32 = nullptr;
+    double *bp64 = nullptr, *xp64 = nullptr;   // b and x in the solver numbering
 };
 
 struct feel_hip_vec {

@@ -135,7 +135,7 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
         delete c;
         return setError(FEEL_HIP_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    int rc = c->alloc((void**)&c->part, sizeof(double) * 4 * kReduceBlocks);
+    int rc = c->alloc((void**)&c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->state, sizeof(CgState) + 4 * sizeof(double));
     if (rc == FEEL_HIP_OK && hipHostMalloc((void**)&c->hstate, sizeof(CgState), hipHostMallocDefault) != hipSuccess)
         rc = setError(FEEL_HIP_ENOMEM, "hipHostMalloc failed");
@@ -151,7 +151,7 @@ int feel_hip_ctx_destroy(feel_hip_ctx* c) {
     if (!c) return FEEL_HIP_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->release(c->part, sizeof(double) * 4 * kReduceBlocks);
+    c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
     c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
     if (c->hstate) (void)hipHostFree(c->hstate);
     for (auto& ev : c->events) {
@@ -397,6 +397,16 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->diagpos, diag.data(), diag.size());
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->cptr, cptr.data(), cptr.size());
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->centry, gm.entry.data(), gm.entry.size());
+    if (rc == FEEL_HIP_OK) {
+        SellLayout L = buildSell(gc, 1024);
+        g->nslices = L.nslices;
+        g->nslots = L.nslots;
+        rc = upload(ctx, &g->slicePtr, L.slicePtr.data(), L.slicePtr.size());
+        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->scol, L.col.data(), L.col.size());
+        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->slot2csr, L.slot2csr.data(), L.slot2csr.size());
+        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->perm, L.perm.data(), L.perm.size());
+        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->sdiag, L.diagSlot.data(), L.diagSlot.size());
+    }
     if (rc != FEEL_HIP_OK) {
         feel_hip_graph_destroy(g);
         return rc;
@@ -415,6 +425,11 @@ int feel_hip_graph_destroy(feel_hip_graph* g) {
     c->release(g->diagpos, sizeof(int32_t) * g->n);
     c->release(g->cptr, sizeof(uint32_t) * (g->nnz + 1));
     c->release(g->centry, sizeof(uint32_t) * g->ncontrib);
+    c->release(g->slicePtr, sizeof(uint32_t) * (g->nslices + 1));
+    c->release(g->scol, sizeof(int32_t) * g->nslots);
+    c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
+    c->release(g->perm, sizeof(int32_t) * g->n);
+    c->release(g->sdiag, sizeof(int32_t) * g->n);
     delete g;
     return FEEL_HIP_OK;
 }
@@ -455,6 +470,9 @@ int feel_hip_mat_destroy(feel_hip_mat* A) {
     const int64_t n = A->g->n;
     for (double* v : {A->r64, A->z64, A->p64, A->w64}) c->release(v, sizeof(double) * n);
     for (float* v : {A->r32, A->z32, A->p32, A->w32, A->x32, A->b32}) c->release(v, sizeof(float) * n);
+    for (double* v : {A->bp64, A->xp64}) c->release(v, sizeof(double) * n);
+    c->release(A->s64, sizeof(double) * A->g->nslots);
+    c->release(A->s32, sizeof(float) * A->g->nslots);
     delete A;
     return FEEL_HIP_OK;
 }
@@ -465,6 +483,7 @@ int feel_hip_mat_zero(feel_hip_mat* A) {
     if (A->v64) FEEL_HIP_CHECK(hipMemsetAsync(A->v64, 0, sizeof(double) * A->g->nnz, c->stream));
     if (A->v32) FEEL_HIP_CHECK(hipMemsetAsync(A->v32, 0, sizeof(float) * A->g->nnz, c->stream));
     A->dinvValid = false;
+    A->sell64Valid = A->sell32Valid = false;
     return FEEL_HIP_OK;
 }
 
@@ -498,6 +517,7 @@ int feel_hip_mat_add_matrix(feel_hip_mat* A, int nrows, const int32_t* rows, int
     c->release(dpos, sizeof(int64_t) * m);
     c->release(dval, sizeof(double) * m);
     A->dinvValid = false;
+    A->sell64Valid = A->sell32Valid = false;
     return rc;
 }
 
@@ -532,6 +552,7 @@ int feel_hip_mat_set_values(feel_hip_mat* A, const double* host) {
     FEEL_HIP_CHECK(hipGetLastError());
     FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
     A->dinvValid = false;
+    A->sell64Valid = A->sell32Valid = false;
     return FEEL_HIP_OK;
 }
 
@@ -701,6 +722,7 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     if (v64 && A->v32 && !v32) hipLaunchKernelGGL(kCastF32, dim3(gridFor(g->nnz)), dim3(kBlock), 0, c->stream, g->nnz, v64, A->v32);
     FEEL_HIP_CHECK(hipGetLastError());
     A->dinvValid = false;
+    A->sell64Valid = A->sell32Valid = false;
     return FEEL_HIP_OK;
     GUARD_END
 }
@@ -780,6 +802,7 @@ int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const fee
                            rows->flag, gv, sym, diag, A->v64 ? 0 : 1);
     FEEL_HIP_CHECK(hipGetLastError());
     A->dinvValid = false;
+    A->sell64Valid = A->sell32Valid = false;
     return FEEL_HIP_OK;
 }
 

@@ -48,9 +48,9 @@ __device__ inline bool blockDone(const CgState* st) {
     return flag != 0;
 }
 
-__device__ inline double reducePartials(const double* __restrict__ part) {
+__device__ inline double reducePartials(const double* __restrict__ part, int count = kReduceBlocks) {
     double v = 0;
-    for (int i = threadIdx.x; i < kReduceBlocks; i += kBlock) v += part[i];
+    for (int i = threadIdx.x; i < count; i += kBlock) v += part[i];
     return blockReduceSum(v);
 }
 
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __rest
                                                     double* __restrict__ zzPart, CgState* st,
                                                     const double* betaSlots, double* dpiSlots) {
     if (blockDone(st)) return;
-    const double dpi = reducePartials(dpiPart);
+    const double dpi = reducePartials(dpiPart, kSpmvGrid);
     const bool indefinite = dpi == 0.0 || (it > 0 && dpi * dpiSlots[(it - 1) & 1] <= 0.0);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         dpiSlots[it & 1] = dpi;
@@ -446,6 +446,115 @@ static __global__ void kAddAt(int64_t m, const int64_t* __restrict__ pos, const 
             v32[pos[k]] = (float)((double)v32[pos[k]] + vals[k]);
         }
     }
+}
+
+// ---------------------------------------------------------------- SELL solver format
+template <int BS>
+__device__ inline double blockReduceSumN(double v) {
+    __shared__ double sh[BS / 64];
+    __shared__ double total;
+    v = waveReduceSum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0;
+#pragma unroll
+        for (int i = 0; i < BS / 64; ++i) s += sh[i];
+        total = s;
+    }
+    __syncthreads();
+    const double r = total;
+    __syncthreads();
+    return r;
+}
+
+// y = A x on the SELL-C-sigma image: one wavefront per 64-row slice, one lane
+// per row, coalesced column-major value/column streams read with non-temporal
+// loads (streamed once; keeps L2 for the x gathers), U independent partial
+// sums per lane for memory-level parallelism.  With part, also the block
+// partial of x.y (p.Ap).  Grid-stride over slices with a fixed grid.
+template <typename T, int BS, int U>
+__global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
+                                                const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                const T* __restrict__ x, T* __restrict__ y, double* __restrict__ part,
+                                                const CgState* __restrict__ st) {
+    if (st && blockDone(st)) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * BS) >> 6;
+    double dot = 0.0;
+    for (int64_t s = wave; s < nslices; s += nwaves) {
+        const uint32_t base = slicePtr[s];
+        const int width = (int)((slicePtr[s + 1] - base) >> 6);
+        const T* v = val + base + lane;
+        const int32_t* c = col + base + lane;
+        T acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = 0;
+        int k = 0;
+        for (; k + U <= width; k += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
+                const int32_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
+                acc[u] += vv * x[cc];
+            }
+        }
+        for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
+        T a = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) a += acc[u];
+        const int64_t r = s * 64 + lane;
+        if (r < nrows) {
+            y[r] = a;
+            if (part) dot += (double)x[r] * (double)a;
+        }
+    }
+    if (part) {
+        const double t = blockReduceSumN<BS>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kCsrToSell(int64_t nslots, const int32_t* __restrict__ slot2csr,
+                                                     const double* __restrict__ c64, const float* __restrict__ c32,
+                                                     T* __restrict__ out) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = slot2csr[s];
+        out[s] = p < 0 ? (T)0 : (c64 ? (T)c64[p] : (T)c32[p]);
+    }
+}
+
+// D^{-1} in the solver numbering from the SELL diagonal slots
+static __global__ __launch_bounds__(kBlock) void kDiagInvSell(int64_t n, const int32_t* __restrict__ sdiag,
+                                                              const double* __restrict__ s64,
+                                                              const float* __restrict__ s32, double* __restrict__ d64,
+                                                              float* __restrict__ d32) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (d64 && s64) {
+            const double d = s64[sdiag[i]];
+            d64[i] = d != 0.0 ? 1.0 / d : 1.0;
+        }
+        if (d32) {
+            const double d = s32 ? (double)s32[sdiag[i]] : s64[sdiag[i]];
+            d32[i] = (float)(d != 0.0 ? 1.0 / d : 1.0);
+        }
+    }
+}
+
+// out[i] = in[perm[i]] (old -> solver numbering) and back
+static __global__ __launch_bounds__(kBlock) void kPermuteIn(int64_t n, const int32_t* __restrict__ perm,
+                                                            const double* __restrict__ in, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[perm[i]];
+}
+
+static __global__ __launch_bounds__(kBlock) void kPermuteOut(int64_t n, const int32_t* __restrict__ perm,
+                                                             const double* __restrict__ in, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[perm[i]] = in[i];
 }
 
 }  // namespace feelpp_amd

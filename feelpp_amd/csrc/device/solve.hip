@@ -21,21 +21,12 @@ namespace {
         if (rc_ != FEEL_HIP_OK) return rc_;  \
     } while (0)
 
-int lanesFor(const feel_hip_graph* g) {
-    const double mean = g->n ? (double)g->nnz / (double)g->n : 0.0;
-    return mean > 48 ? 16 : (mean > 12 ? 8 : 4);
-}
-
+// y = A x in the solver numbering (SELL image), optional p.Ap partials
 template <typename T>
-void spmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st,
-          int grid) {
-    const int L = lanesFor(g);
-    if (L == 16)
-        hipLaunchKernelGGL((kSpmv<T, 16>), dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, val, x, y, part, st);
-    else if (L == 8)
-        hipLaunchKernelGGL((kSpmv<T, 8>), dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, val, x, y, part, st);
-    else
-        hipLaunchKernelGGL((kSpmv<T, 4>), dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, val, x, y, part, st);
+void spmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st) {
+    constexpr int U = sizeof(T) == 8 ? 4 : 8;
+    hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
+                       g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
 }
 
 int gridN(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 8192)); }
@@ -46,14 +37,31 @@ int allocWork(feel_hip_ctx* c, T** p, int64_t n) {
     return c->alloc((void**)p, sizeof(T) * n);
 }
 
-int ensureDiag(feel_hip_mat* A) {
-    if (A->dinvValid) return FEEL_HIP_OK;
+// Refresh the SELL values (and the Jacobi diagonal, solver numbering) from the CSR values.
+int ensureSell(feel_hip_mat* A, bool need64, bool need32) {
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
-    hipLaunchKernelGGL(kDiagInv, dim3(gridN(g->n)), dim3(kBlock), 0, c->stream, g->n, g->diagpos, A->v64, A->v32,
-                       A->dinv64, A->dinv32);
+    bool changed = false;
+    if (need64 && !A->sell64Valid) {
+        if (!A->v64) return setError(FEEL_HIP_EINVAL, "matrix holds no FP64 values");
+        TRY_CALL(allocWork(c, &A->s64, g->nslots));
+        hipLaunchKernelGGL(kCsrToSell<double>, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,
+                           A->v64, (const float*)nullptr, A->s64);
+        A->sell64Valid = changed = true;
+    }
+    if (need32 && !A->sell32Valid) {
+        TRY_CALL(allocWork(c, &A->s32, g->nslots));
+        hipLaunchKernelGGL(kCsrToSell<float>, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,
+                           A->v32 ? (const double*)nullptr : A->v64, A->v32, A->s32);
+        A->sell32Valid = changed = true;
+    }
+    if (changed || !A->dinvValid) {
+        hipLaunchKernelGGL(kDiagInvSell, dim3(gridN(g->n)), dim3(kBlock), 0, c->stream, g->n, g->sdiag,
+                           A->sell64Valid ? A->s64 : nullptr, A->sell32Valid ? A->s32 : nullptr,
+                           A->sell64Valid ? A->dinv64 : nullptr, A->dinv32);
+        A->dinvValid = true;
+    }
     FEEL_HIP_CHECK(hipGetLastError());
-    A->dinvValid = true;
     return FEEL_HIP_OK;
 }
 
@@ -72,7 +80,7 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
     const int64_t n = g->n;
     double* zrPart = c->part;
     double* zzPart = c->part + kReduceBlocks;
-    double* dpiPart = c->part + 2 * kReduceBlocks;
+    double* dpiPart = c->part + 3 * kReduceBlocks;   // kSpmvGrid partials
     CgState* st = c->state;
     double* betaSlots = reinterpret_cast<double*>(st + 1);
     double* dpiSlots = betaSlots + 2;
@@ -82,7 +90,7 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
     if (x0zero)
         FEEL_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(T) * n, c->stream));
     else
-        spmv<T>(c, g, val, x, w, nullptr, nullptr, gridN(n));
+        spmv<T>(c, g, val, x, w, nullptr, nullptr);
     hipLaunchKernelGGL(kCgInit<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, b, w, r, z, dinv, zrPart, zzPart);
     FEEL_HIP_CHECK(hipGetLastError());
     int it = 0;
@@ -92,7 +100,7 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
             hipLaunchKernelGGL(kCgDirection<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, z, p, zrPart, zzPart,
                                st, betaSlots);
             c->tic(timerName);
-            spmv<T>(c, g, val, p, w, dpiPart, st, kReduceBlocks);
+            spmv<T>(c, g, val, p, w, dpiPart, st);
             c->toc();
             hipLaunchKernelGGL(kCgUpdate<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, x, r, z, p, w, dinv,
                                dpiPart, zrPart, zzPart, st, betaSlots, dpiSlots);
@@ -116,13 +124,13 @@ int residual64(feel_hip_mat* A, const double* b, const double* x, double* r, dou
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
     c->tic("spmv_fp64");
-    spmv<double>(c, g, A->v64, x, w, nullptr, nullptr, gridN(g->n));
+    spmv<double>(c, g, A->s64, x, w, nullptr, nullptr);
     c->toc();
     hipLaunchKernelGGL(kResidual64, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, g->n, b, w, A->dinv64, r,
-                       c->part + 3 * kReduceBlocks);
+                       c->part + 2 * kReduceBlocks);
     FEEL_HIP_CHECK(hipGetLastError());
     std::vector<double> h(kReduceBlocks);
-    FEEL_HIP_CHECK(hipMemcpyAsync(h.data(), c->part + 3 * kReduceBlocks, sizeof(double) * kReduceBlocks,
+    FEEL_HIP_CHECK(hipMemcpyAsync(h.data(), c->part + 2 * kReduceBlocks, sizeof(double) * kReduceBlocks,
                                   hipMemcpyDeviceToHost, c->stream));
     FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
     double s = 0;
@@ -168,10 +176,16 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
         TRY_CALL(allocWork(c, &A->z64, n));
         TRY_CALL(allocWork(c, &A->p64, n));
         TRY_CALL(allocWork(c, &A->w64, n));
-        TRY_CALL(ensureDiag(A));
+        TRY_CALL(allocWork(c, &A->bp64, n));
+        TRY_CALL(allocWork(c, &A->xp64, n));
+        TRY_CALL(ensureSell(A, true, false));
+        hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, b->d, A->bp64);
+        hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, x->d, A->xp64);
         CgRun run;
-        TRY_CALL(runCg<double>(A, A->v64, A->dinv64, b->d, x->d, false, o.rtol, o.atol, o.dtol, o.maxit, o.check_every,
-                               A->r64, A->z64, A->p64, A->w64, "spmv_fp64", run));
+        TRY_CALL(runCg<double>(A, A->s64, A->dinv64, A->bp64, A->xp64, false, o.rtol, o.atol, o.dtol, o.maxit,
+                               o.check_every, A->r64, A->z64, A->p64, A->w64, "spmv_fp64", run));
+        hipLaunchKernelGGL(kPermuteOut, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, A->xp64, x->d);
+        FEEL_HIP_CHECK(hipGetLastError());
         c->collectTimers();
         out->its = run.its;
         out->reason = run.reason;
@@ -184,13 +198,6 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
     if (o.precision != FEEL_HIP_FP32) return setError(FEEL_HIP_EINVAL, "solve precision must be FP64 or FP32");
     // FP32 Jacobi-CG inside FP64 iterative refinement: FP64 residual, FP64 stopping test.
     if (!A->v64) return setError(FEEL_HIP_EINVAL, "mixed-precision solve needs FP64 matrix values for the residual");
-    if (!A->v32) {
-        TRY_CALL(allocWork(c, &A->v32, g->nnz));
-        hipLaunchKernelGGL(kCastF32, dim3(gridN(g->nnz)), dim3(kBlock), 0, c->stream, g->nnz, A->v64, A->v32);
-        FEEL_HIP_CHECK(hipGetLastError());
-        A->precision |= FEEL_HIP_FP32;
-        A->dinvValid = false;
-    }
     TRY_CALL(allocWork(c, &A->r64, n));
     TRY_CALL(allocWork(c, &A->w64, n));
     TRY_CALL(allocWork(c, &A->r32, n));
@@ -199,9 +206,13 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
     TRY_CALL(allocWork(c, &A->w32, n));
     TRY_CALL(allocWork(c, &A->x32, n));
     TRY_CALL(allocWork(c, &A->b32, n));
-    TRY_CALL(ensureDiag(A));
+    TRY_CALL(allocWork(c, &A->bp64, n));
+    TRY_CALL(allocWork(c, &A->xp64, n));
+    TRY_CALL(ensureSell(A, true, true));
+    hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, b->d, A->bp64);
+    hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, x->d, A->xp64);
     double dp = 0;
-    TRY_CALL(residual64(A, b->d, x->d, A->r64, A->w64, dp));
+    TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
     const double rnorm0 = dp, ttol = std::max(o.rtol * dp, o.atol);
     int total = 0, outer = 0, reason = 0;
     while (true) {
@@ -214,15 +225,17 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
         FEEL_HIP_CHECK(hipMemsetAsync(A->x32, 0, sizeof(float) * n, c->stream));
         CgRun run;
         const int budget = std::min(o.inner_maxit, o.maxit - total);
-        TRY_CALL(runCg<float>(A, A->v32, A->dinv32, A->b32, A->x32, true, o.inner_rtol, 0.0, 1e5, budget, o.check_every,
+        TRY_CALL(runCg<float>(A, A->s32, A->dinv32, A->b32, A->x32, true, o.inner_rtol, 0.0, 1e5, budget, o.check_every,
                               A->r32, A->z32, A->p32, A->w32, "spmv_fp32", run));
         total += run.its;
         ++outer;
         if (run.reason == -8 || run.reason == -9) { reason = run.reason; break; }
-        hipLaunchKernelGGL(kAxpyF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, dp, A->x32, x->d);
+        hipLaunchKernelGGL(kAxpyF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, dp, A->x32, A->xp64);
         FEEL_HIP_CHECK(hipGetLastError());
-        TRY_CALL(residual64(A, b->d, x->d, A->r64, A->w64, dp));
+        TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
     }
+    hipLaunchKernelGGL(kPermuteOut, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, A->xp64, x->d);
+    FEEL_HIP_CHECK(hipGetLastError());
     c->collectTimers();
     out->its = total;
     out->reason = reason;

@@ -459,4 +459,58 @@ GatherMap buildGatherMap(const DofTable& dt, const GraphCSR& g, int nthreads) {
     return gm;
 }
 
+SellLayout buildSell(const GraphCSR& g, int window) {
+    SellLayout L;
+    const int64_t n = g.nrows;
+    const int C = SellLayout::C;
+    L.nrows = n;
+    L.perm.resize(n);
+    L.iperm.resize(n);
+    for (int64_t w0 = 0; w0 < n; w0 += window) {
+        const int64_t w1 = std::min<int64_t>(n, w0 + window);
+        for (int64_t r = w0; r < w1; ++r) L.perm[r] = (int32_t)r;
+        std::stable_sort(L.perm.begin() + w0, L.perm.begin() + w1, [&](int32_t a, int32_t b) {
+            return g.ia[a + 1] - g.ia[a] > g.ia[b + 1] - g.ia[b];
+        });
+    }
+    for (int64_t r = 0; r < n; ++r) L.iperm[L.perm[r]] = (int32_t)r;
+    L.nslices = (n + C - 1) / C;
+    L.slicePtr.assign(L.nslices + 1, 0);
+    uint64_t acc = 0;
+    for (int64_t s = 0; s < L.nslices; ++s) {
+        int width = 0;
+        for (int64_t r = s * C; r < std::min<int64_t>(n, (s + 1) * C); ++r)
+            width = std::max(width, g.ia[L.perm[r] + 1] - g.ia[L.perm[r]]);
+        L.slicePtr[s] = (uint32_t)acc;
+        acc += (uint64_t)width * C;
+        if (acc >= (1ULL << 32)) throw std::overflow_error("SELL image exceeds 2^32 slots");
+    }
+    L.slicePtr[L.nslices] = (uint32_t)acc;
+    L.nslots = (int64_t)acc;
+    L.col.assign(acc, 0);
+    L.slot2csr.assign(acc, -1);
+    L.diagSlot.assign(n, -1);
+    for (int64_t s = 0; s < L.nslices; ++s) {
+        const int width = (int)((L.slicePtr[s + 1] - L.slicePtr[s]) / C);
+        for (int lane = 0; lane < C; ++lane) {
+            const int64_t r = s * C + lane;
+            const int64_t self = r < n ? r : 0;
+            const int32_t old = r < n ? L.perm[r] : 0;
+            const int len = r < n ? g.ia[old + 1] - g.ia[old] : 0;
+            for (int k = 0; k < width; ++k) {
+                const int64_t slot = L.slicePtr[s] + (int64_t)k * C + lane;
+                if (k < len) {
+                    const int64_t p = g.ia[old] + k;
+                    L.col[slot] = L.iperm[g.ja[p]];
+                    L.slot2csr[slot] = (int32_t)p;
+                    if (g.ja[p] == old) L.diagSlot[r] = (int32_t)slot;
+                } else {
+                    L.col[slot] = (int32_t)self;
+                }
+            }
+        }
+    }
+    return L;
+}
+
 }  // namespace feelpp_amd

@@ -68,6 +68,24 @@ struct Incidence {
 };
 Incidence buildIncidence(const DofTable& dt);
 
+// Sliced ELLPACK (SELL-C-sigma) image of a CSR matrix for the device SpMV:
+// rows are sorted by decreasing length inside windows of `window` rows
+// (perm: new -> old row, iperm: old -> new), grouped in slices of C = 64 rows
+// (one wavefront, one lane per row) and stored column-major inside a slice:
+// slot = slicePtr[s] + k*64 + lane.  Columns are renumbered with iperm so the
+// solver works entirely in the permuted numbering; padding slots point at the
+// row itself with a zero value (slot2csr = -1).
+struct SellLayout {
+    static constexpr int C = 64;
+    int64_t nrows = 0, nslices = 0, nslots = 0;
+    std::vector<uint32_t> slicePtr;   // nslices + 1
+    std::vector<int32_t> col;         // nslots, permuted column
+    std::vector<int32_t> slot2csr;    // nslots, CSR position or -1
+    std::vector<int32_t> perm, iperm; // nrows
+    std::vector<int32_t> diagSlot;    // nrows (new numbering)
+};
+SellLayout buildSell(const GraphCSR& g, int window);
+
 int defaultThreads();
 
 }  // namespace feelpp_amd

@@ -84,11 +84,12 @@ def test_status_semantics(ctx, orc):
     x.zero()
     st = P["A"].solve(z, x, rtol=1e-12)
     assert st["its"] == 0 and st["converged"]
-    # warm start from the solution converges at once
+    # x holds the initial guess (UIRNorm: the test is relative to ||B r_0|| of that guess)
     x.zero()
     st1 = P["A"].solve(P["bb"], x, rtol=1e-12, maxit=1000)
     st2 = P["A"].solve(P["bb"], x, rtol=1e-3, maxit=1000)
-    assert st1["converged"] and st2["its"] <= 2
+    assert st1["converged"] and st2["converged"]
+    assert st2["rnorm0"] < 1e-10 * st1["rnorm0"]
 
 
 def test_solve_deterministic(ctx, orc):
