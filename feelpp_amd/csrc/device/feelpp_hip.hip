@@ -300,6 +300,22 @@ int scalarTensors(feel_hip_space* s, int qorder, const double** out) {
     return FEEL_HIP_OK;
 }
 
+// full reference tensors R_ab (a,b < 3, 9 x n^2) for the vector forms, cached per quadrature order
+// under key -(qorder+1) so they do not collide with the scalar tables
+int fullTensors(feel_hip_space* s, int qorder, const double** out) {
+    const int key = -(qorder + 1);
+    auto it = s->rtab.find(key);
+    if (it != s->rtab.end()) { *out = it->second; return FEEL_HIP_OK; }
+    ReferenceTensors T = referenceTensors(s->dim, s->order, qorder);
+    std::vector<double> h(T.R.begin(), T.R.end());
+    double* dptr = nullptr;
+    TRY_CALL(upload(s->ctx, &dptr, h.data(), h.size()));
+    s->rtab[key] = dptr;
+    s->hrtab[key] = std::move(h);
+    *out = dptr;
+    return FEEL_HIP_OK;
+}
+
 // quadrature table for the linear form: w[Q] | phi[nloc*Q] | lambda[(dim+1)*Q]
 int rhsTable(feel_hip_space* s, int qorder, const double** out, int* Q) {
     auto it = s->qtab.find(qorder);
@@ -672,7 +688,34 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     float* v32 = (wantPrec & FEEL_HIP_FP32) ? A->v32 : nullptr;
     if (!v64 && !v32) return setError(FEEL_HIP_EINVAL, "matrix holds none of the requested precisions");
     if (d->kind & FEEL_HIP_EXPR_ELASTICITY) {
-        return setError(FEEL_HIP_EUNSUPPORTED, "elasticity assembly is not available in this build");
+        if (d->kind != FEEL_HIP_EXPR_ELASTICITY)
+            return setError(FEEL_HIP_EUNSUPPORTED, "elasticity cannot be combined with other terms in one call");
+        if (s->dim != 3 || s->ncomp != 3) return setError(FEEL_HIP_EUNSUPPORTED, "elasticity needs Pchv on tetrahedra");
+        if (!d->young || !d->poisson || d->young->n != s->ne || d->poisson->n != s->ne)
+            return setError(FEEL_HIP_EINVAL, "elasticity needs E and nu per element");
+        const double* rt = nullptr;
+        TRY_CALL(fullTensors(s, d->quad_order, &rt));
+        TRY_CALL(ensureGeo(s, sizeof(double) * 81 * s->ne));
+        c->tic("assemble_geo");
+        hipLaunchKernelGGL(kGeoElastic, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->coords, s->elems,
+                           d->young->d, d->poisson->d, s->geo);
+        c->toc();
+        FEEL_HIP_CHECK(hipGetLastError());
+        const int grid = gridFor(g->nnz, 8192);
+        c->tic("assemble_gather");
+        switch (s->order) {
+            case 1: hipLaunchKernelGGL(kGatherElastic<4>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
+            case 2: hipLaunchKernelGGL(kGatherElastic<10>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
+            case 3: hipLaunchKernelGGL(kGatherElastic<20>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
+            default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no elasticity kernel for this order");
+        }
+        c->toc();
+        FEEL_HIP_CHECK(hipGetLastError());
+        if (v64 && A->v32 && !v32) hipLaunchKernelGGL(kCastF32, dim3(gridFor(g->nnz)), dim3(kBlock), 0, c->stream, g->nnz, v64, A->v32);
+        FEEL_HIP_CHECK(hipGetLastError());
+        A->dinvValid = false;
+        A->sell64Valid = A->sell32Valid = false;
+        return FEEL_HIP_OK;
     }
     if ((d->kind & ~(FEEL_HIP_EXPR_MASS | FEEL_HIP_EXPR_DIFFUSION)) || d->kind == 0)
         return setError(FEEL_HIP_EUNSUPPORTED, "expression outside the supported subset");

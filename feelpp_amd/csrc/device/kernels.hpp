@@ -172,6 +172,83 @@ __global__ __launch_bounds__(kBlock) void kGatherScalar(int64_t nnz, const uint3
     }
 }
 
+// Per-element factors of isotropic linear elasticity inner(sigma(u), grad v),
+// sigma = lambda tr(eps) I + 2 mu eps (quickstart/qs_elasticity.cpp:41-63).
+// With B = K^{-T}, C = B^T B and R_ab(i,j) = sum_q w_q d_a phi_i d_b phi_j on the
+// reference element, the (c,i),(d,j) entry is sum_ab Ke[c][d][a][b] R_ab(i,j) with
+//   Ke[c][d][a][b] = J (lambda B_ca B_db + mu B_da B_cb + mu delta_cd C_ab).
+// Layout: 81 doubles per element, index ((c*3+d)*3+a)*3+b.
+static __global__ __launch_bounds__(kBlock) void kGeoElastic(int64_t ne, const double* __restrict__ coords,
+                                                             const int32_t* __restrict__ elems,
+                                                             const double* __restrict__ young,
+                                                             const double* __restrict__ poisson,
+                                                             double* __restrict__ F) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+        double Ki[3][3];
+        const double J = affineMap<3>(coords, elems + e * 4, Ki);
+        const double E = young[e], nu = poisson[e];
+        const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+        const double mu = E / (2.0 * (1.0 + nu));
+        double B[3][3], C[3][3];
+#pragma unroll
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) B[x][a] = Ki[a][x];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) C[a][b] = B[0][a] * B[0][b] + B[1][a] * B[1][b] + B[2][a] * B[2][b];
+        double* f = F + e * 81;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        f[((c * 3 + d) * 3 + a) * 3 + b] =
+                            J * (lam * B[c][a] * B[d][b] + mu * B[d][a] * B[c][b] + (c == d ? mu * C[a][b] : 0.0));
+    }
+}
+
+// Owner-computes scatter for the vector elasticity form: local entry
+// ij = (c*n + i) * 3n + (d*n + j) of element e; R (9 x n^2) staged in LDS.
+template <int N>
+__global__ __launch_bounds__(kBlock) void kGatherElastic(int64_t nnz, const uint32_t* __restrict__ cptr,
+                                                         const uint32_t* __restrict__ centry,
+                                                         const double* __restrict__ F,
+                                                         const double* __restrict__ rtab,
+                                                         double* __restrict__ v64, float* __restrict__ v32, int add) {
+    constexpr int NN = N * N, N3 = 3 * N, NN3 = N3 * N3;
+    __shared__ double R[9 * NN];
+    for (int i = threadIdx.x; i < 9 * NN; i += kBlock) R[i] = rtab[i];
+    __syncthreads();
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (uint32_t k = cptr[p]; k < cptr[p + 1]; ++k) {
+            const uint32_t c = centry[k];
+            const uint32_t el = c / NN3;
+            const uint32_t loc = c - el * NN3;
+            const uint32_t rl = loc / N3, cl = loc - rl * N3;
+            const uint32_t cc = rl / N, i = rl - cc * N, dd = cl / N, j = cl - dd * N;
+            const double* f = F + (size_t)el * 81 + (cc * 3 + dd) * 9;
+            const uint32_t ij = i * N + j;
+            double a = 0.0;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) a += f[t] * R[t * NN + ij];
+            s += a;
+        }
+        if (v64) {
+            const double v = add ? v64[p] + s : s;
+            v64[p] = v;
+            if (v32) v32[p] = (float)v;
+        } else if (v32) {
+            v32[p] = add ? (float)((double)v32[p] + s) : (float)s;
+        }
+    }
+}
+
 // Generic scatter of materialised element matrices (elem[e*NN + ij]).
 static __global__ __launch_bounds__(kBlock) void kGatherElemMat(int64_t nnz, const uint32_t* __restrict__ cptr,
                                                          const uint32_t* __restrict__ centry,

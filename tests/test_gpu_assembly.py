@@ -145,3 +145,33 @@ def test_full_size_p2_properties(ctx):
     x.set(c)
     A.mult(x, y)
     assert abs(c @ Aa - a @ y.get()) < 1e-10 * abs(c @ Aa) + 1e-12    # symmetric
+
+
+@pytest.mark.parametrize("order,qo", [(1, 2), (2, 4), (3, 6)])
+def test_elasticity_fp64_vs_oracle(ctx, orc, order, qo):
+    # 1 x 0.25 x 0.25 beam, E log-uniform in [1,100], nu uniform in [0.2,0.35]
+    coords, elems = orc.mesh_kuhn(4, 2, 2, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=13)
+    e2d, ndof = orc.doftable(3, order, 3, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    E = orc.uniform(len(elems), 1.0, 100.0, 5, log=True)
+    nu = orc.uniform(len(elems), 0.2, 0.35, 6)
+    ref = orc.assemble_elasticity(order, qo, coords, elems, e2d, E, nu, ia, ja)
+    sp = fa.Space(ctx, coords, elems, order, 3, e2d, ndof)
+    g = fa.Graph(sp, ia, ja)
+    A = fa.Matrix(g, fa.FP64_FP32)
+    A.assemble(fa.ELASTICITY, qo, young=ctx.vector(len(elems), E), poisson=ctx.vector(len(elems), nu))
+    assert rel_fro(A.values(), ref) < TOL64
+    assert rel_fro(A.values(fa.FP32).astype(np.float64), ref) < 1e-6
+
+
+def test_unsupported_requests_fail_loudly(ctx, orc):
+    coords, elems, e2d, ndof, ia, ja, sp, g = setup(ctx, orc, 3, 1, 2)
+    A = fa.Matrix(g)
+    with pytest.raises(fa.FeelHipError):
+        A.assemble(fa.ELASTICITY, 2)                       # scalar space
+    with pytest.raises(fa.FeelHipError):
+        A.assemble(8, 2)                                   # unknown expression kind
+    with pytest.raises(fa.FeelHipError):
+        A.assemble(fa.DIFFUSION, 99)                       # no such quadrature
+    with pytest.raises(fa.FeelHipError):
+        fa.Space(ctx, coords, elems, 5, 1, e2d, ndof)      # P5 not provided

@@ -98,3 +98,36 @@ def test_solve_deterministic(ctx, orc):
     s1 = P["A"].solve(P["bb"], x1, rtol=1e-10)
     s2 = P["A"].solve(P["bb"], x2, rtol=1e-10)
     assert s1 == s2 and np.array_equal(x1.get(), x2.get())
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_clamped_beam_elasticity(ctx, orc, order):
+    # left face clamped, body force (0,0,-1); FP64 and mixed solves vs the oracle
+    coords, elems = orc.mesh_kuhn(8, 2, 2, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=17)
+    e2d, ndof = orc.doftable(3, order, 3, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    E = orc.uniform(len(elems), 1.0, 100.0, 1, log=True)
+    nu = orc.uniform(len(elems), 0.2, 0.35, 2)
+    qo = 2 * order
+    val = orc.assemble_elasticity(order, qo, coords, elems, e2d, E, nu, ia, ja)
+    b = orc.assemble_rhs(3, order, 3, qo, coords, elems, e2d, ndof, orc.FK_VECTOR, [0.0, 0.0, -1.0])
+    isb = orc.boundary_dofs(3, order, 3, coords, elems, e2d, ndof, sel=1)
+    orc.dirichlet(1, ia, ja, val, b, isb)
+    xo, its_o, _, reason = orc.pcg(ia, ja, val, b, rtol=1e-13, maxit=50000)
+    assert reason > 0
+    space = fa.Space(ctx, coords, elems, order, 3, e2d, ndof)
+    g = fa.Graph(space, ia, ja)
+    A = fa.Matrix(g, fa.FP64_FP32)
+    A.assemble(fa.ELASTICITY, qo, young=ctx.vector(len(elems), E), poisson=ctx.vector(len(elems), nu))
+    bb = ctx.vector(ndof)
+    fa.assemble_linear(space, bb, fa.SRC_VECTOR, qo, (0.0, 0.0, -1.0))
+    assert rel_fro(bb.get(), orc.assemble_rhs(3, order, 3, qo, coords, elems, e2d, ndof, orc.FK_VECTOR,
+                                              [0.0, 0.0, -1.0])) < 1e-12
+    rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
+    A.zero_rows(rows, None, bb, mode=fa.ELIMINATION_SYMMETRIC)
+    x = ctx.vector(ndof)
+    st = A.solve(bb, x, rtol=1e-13, maxit=50000)
+    assert st["converged"] and rel_fro(x.get(), xo) < 1e-10
+    x.zero()
+    st = A.solve(bb, x, rtol=1e-10, maxit=50000, precision=fa.FP32)
+    assert st["converged"] and rel_fro(x.get(), xo) < 1e-8
