@@ -52,6 +52,18 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # quickstart driver executables, linked against the library (rpath to feelpp_amd/)
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    apps = sorted(f for f in os.listdir(os.path.join(HERE, "apps")) if f.endswith(".cpp"))
+    for app in apps:
+        exe = os.path.join(bindir, "feelpp_" + app[:-4])
+        cmd = ["g++", os.path.join(HERE, "apps", app), "-o", exe + ".tmp"] + COMMON + CXX_FLAGS + \
+              [LIB, "-Wl,-rpath,$ORIGIN/..", "-lpthread"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("app build failed: " + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+        os.replace(exe + ".tmp", exe)
     if verbose:
         print("built", LIB)
     return LIB

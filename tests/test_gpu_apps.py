@@ -1,0 +1,81 @@
+"""The Feel++-shaped C++ surface (form2/form1/integrate/on/solve) end to end:
+the quickstart drivers run on the GPU and their solutions are compared with
+the oracle's FP64 assembly and tightly converged Jacobi-CG."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rel_fro
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "feelpp_amd", "bin")
+
+
+def run(app, args, tmp_path):
+    out = os.path.join(str(tmp_path), "run")
+    r = subprocess.run([os.path.join(BIN, app)] + [str(a) for a in args] + ["--out", out], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr + r.stdout
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    return info, np.fromfile(out + ".sol", dtype=np.float64)
+
+
+def oracle_poisson(orc, dim, n, order, qo, rhs_q, kappa_seed, source, mass, jitter, seed):
+    coords, elems = orc.mesh_square(n, jitter, seed) if dim == 2 else orc.mesh_kuhn(n, jitter=jitter, seed=seed)
+    e2d, ndof = orc.doftable(dim, order, 1, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    k = orc.lognormal(len(elems), 0.0, 1.0, kappa_seed) if kappa_seed is not None else None
+    val = orc.assemble_scalar(dim, order, qo, coords, elems, e2d, ia, ja, diff=1.0, kdiff=k, mass=mass)
+    b = orc.assemble_rhs(dim, order, 1, rhs_q, coords, elems, e2d, ndof, source, [1.0])
+    isb = orc.boundary_dofs(dim, order, 1, coords, elems, e2d, ndof)
+    orc.dirichlet(1, ia, ja, val, b, isb)
+    x, its, _, reason = orc.pcg(ia, ja, val, b, rtol=1e-14, maxit=20000)
+    assert reason > 0
+    return x
+
+
+def test_qs_laplacian_p2_lognormal(orc, tmp_path):
+    info, x = run("feelpp_qs_laplacian", ["--dim", 3, "--n", 6, "--order", 2, "--seed", 5, "--jitter", 0.15,
+                                          "--rtol", 1e-12], tmp_path)
+    assert info["form"] == "diffusion" and info["converged"] == 1
+    xo = oracle_poisson(orc, 3, 6, 2, 2, 2, 6, orc.FK_CONST, 0.0, 0.15, 5)
+    assert rel_fro(x, xo) < 1e-10
+
+
+def test_qs_laplacian_c1_like(orc, tmp_path):
+    info, x = run("feelpp_qs_laplacian", ["--dim", 2, "--n", 32, "--order", 1, "--seed", 1, "--jitter", 0.2,
+                                          "--kappa", "one", "--source", "sinsin", "--mass", 1.0, "--rtol", 1e-12],
+                  tmp_path)
+    assert info["form"] == "mass+diffusion"
+    xo = oracle_poisson(orc, 2, 32, 1, 2, 2, None, orc.FK_SINSIN, 1.0, 0.2, 1)
+    assert rel_fro(x, xo) < 1e-10
+
+
+def test_qs_laplacian_mixed_precision(orc, tmp_path):
+    info, x = run("feelpp_qs_laplacian", ["--dim", 3, "--n", 6, "--order", 2, "--seed", 5, "--jitter", 0.15,
+                                          "--rtol", 1e-11, "--precision", "fp32"], tmp_path)
+    assert info["converged"] == 1 and info["outer"] >= 2
+    xo = oracle_poisson(orc, 3, 6, 2, 2, 2, 6, orc.FK_CONST, 0.0, 0.15, 5)
+    assert rel_fro(x, xo) < 1e-8
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_qs_elasticity_beam(orc, tmp_path, order):
+    info, x = run("feelpp_qs_elasticity", ["--nx", 8, "--ny", 2, "--nz", 2, "--order", order, "--seed", 7,
+                                           "--rtol", 1e-12], tmp_path)
+    assert info["form"] == "elasticity" and info["converged"] == 1
+    coords, elems = orc.mesh_kuhn(8, 2, 2, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=7)
+    e2d, ndof = orc.doftable(3, order, 3, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    E = orc.uniform(len(elems), 1.0, 100.0, 8, log=True)
+    nu = orc.uniform(len(elems), 0.2, 0.35, 9)
+    val = orc.assemble_elasticity(order, 2 * order, coords, elems, e2d, E, nu, ia, ja)
+    b = orc.assemble_rhs(3, order, 3, 2 * order, coords, elems, e2d, ndof, orc.FK_VECTOR, [0.0, 0.0, -1.0])
+    isb = orc.boundary_dofs(3, order, 3, coords, elems, e2d, ndof, sel=1)
+    orc.dirichlet(1, ia, ja, val, b, isb)
+    xo, _, _, reason = orc.pcg(ia, ja, val, b, rtol=1e-14, maxit=100000)
+    assert rel_fro(x, xo) < 1e-10
