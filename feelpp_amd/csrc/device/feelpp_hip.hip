@@ -707,6 +707,7 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
             case 1: hipLaunchKernelGGL(kGatherElastic<4>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
             case 2: hipLaunchKernelGGL(kGatherElastic<10>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
             case 3: hipLaunchKernelGGL(kGatherElastic<20>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
+            case 4: hipLaunchKernelGGL((kGatherElastic<35, false>), dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
             default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no elasticity kernel for this order");
         }
         c->toc();
@@ -757,6 +758,10 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
         case 31: GATHER(16, 6, 8); break;
         case 32: GATHER(100, 6, 8); break;
         case 33: GATHER(400, 6, 8); break;
+        case 34:
+            hipLaunchKernelGGL((kGatherScalar<1225, 6, 8, false>), dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr,
+                               g->centry, s->geo, rt, v64, v32, add);
+            break;
         default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no scatter kernel for this space");
     }
 #undef GATHER

@@ -139,16 +139,20 @@ __global__ __launch_bounds__(kBlock) void kGeoScalar(int64_t ne, const double* _
 // Owner-computes scatter: one thread per CSR slot p sums the element entries
 // that land in p (ascending element order), each entry evaluated from the
 // element factors F and the reference tensors staged in LDS.
-template <int NN, int NS, int STRIDE>
+// Tables larger than the LDS budget (P4: 7 x 1225 doubles) are read through L1/L2.
+template <int NN, int NS, int STRIDE, bool USE_LDS = true>
 __global__ __launch_bounds__(kBlock) void kGatherScalar(int64_t nnz, const uint32_t* __restrict__ cptr,
                                                         const uint32_t* __restrict__ centry,
                                                         const double* __restrict__ F,
                                                         const double* __restrict__ rtab,
                                                         double* __restrict__ v64, float* __restrict__ v32,
                                                         int add) {
-    __shared__ double R[(NS + 1) * NN];
-    for (int i = threadIdx.x; i < (NS + 1) * NN; i += kBlock) R[i] = rtab[i];
-    __syncthreads();
+    __shared__ double Rs[USE_LDS ? (NS + 1) * NN : 1];
+    if constexpr (USE_LDS) {
+        for (int i = threadIdx.x; i < (NS + 1) * NN; i += kBlock) Rs[i] = rtab[i];
+        __syncthreads();
+    }
+    const double* R = USE_LDS ? Rs : rtab;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t b = cptr[p], e = cptr[p + 1];
         double s = 0.0;
@@ -214,16 +218,19 @@ static __global__ __launch_bounds__(kBlock) void kGeoElastic(int64_t ne, const d
 
 // Owner-computes scatter for the vector elasticity form: local entry
 // ij = (c*n + i) * 3n + (d*n + j) of element e; R (9 x n^2) staged in LDS.
-template <int N>
+template <int N, bool USE_LDS = true>
 __global__ __launch_bounds__(kBlock) void kGatherElastic(int64_t nnz, const uint32_t* __restrict__ cptr,
                                                          const uint32_t* __restrict__ centry,
                                                          const double* __restrict__ F,
                                                          const double* __restrict__ rtab,
                                                          double* __restrict__ v64, float* __restrict__ v32, int add) {
     constexpr int NN = N * N, N3 = 3 * N, NN3 = N3 * N3;
-    __shared__ double R[9 * NN];
-    for (int i = threadIdx.x; i < 9 * NN; i += kBlock) R[i] = rtab[i];
-    __syncthreads();
+    __shared__ double Rs[USE_LDS ? 9 * NN : 1];
+    if constexpr (USE_LDS) {
+        for (int i = threadIdx.x; i < 9 * NN; i += kBlock) Rs[i] = rtab[i];
+        __syncthreads();
+    }
+    const double* R = USE_LDS ? Rs : rtab;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
         double s = 0.0;
         for (uint32_t k = cptr[p]; k < cptr[p + 1]; ++k) {

@@ -24,7 +24,7 @@ const int kTetEdge[6][2] = {{1, 2}, {2, 0}, {0, 1}, {0, 3}, {1, 3}, {2, 3}};
 const int kTetFace[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
 const int kTetFaceEdges[4][3] = {{5, 4, 0}, {5, 3, 1}, {4, 3, 2}, {0, 1, 2}};
 
-int maxSupportedOrder(int dim) { return dim == 2 ? 2 : 3; }
+int maxSupportedOrder(int dim) { return dim == 2 ? 2 : 4; }
 
 QuadratureRule quadrature(int dim, int order) {
     for (const auto& t : im_simplex_tables) {
@@ -47,8 +47,40 @@ static std::vector<double> gllInterior(int k) {
     switch (k) {
         case 2: return {0.0};
         case 3: return {-1.0 / std::sqrt(5.0), 1.0 / std::sqrt(5.0)};
+        case 4: return {-std::sqrt(3.0 / 7.0), 0.0, std::sqrt(3.0 / 7.0)};
         default: return {};
     }
+}
+
+// Warp & blend (Warburton) face node of P4 tetrahedra as Feel++ builds it
+// (feelpoly/warpblend.hpp:366-500 with alpha_2d[4] = 0.1001): the equispaced
+// face-interior point with barycentrics (1/2,1/4,1/4) is displaced along its
+// median by the blended 1D warp; returns the weight at the near vertex.
+static double p4FaceNearWeight() {
+    const int N = 4;
+    double req[N + 1], rgl[N + 1];
+    const auto gl = gllInterior(N);
+    for (int i = 0; i <= N; ++i) req[i] = -1.0 + 2.0 * i / N;
+    rgl[0] = -1; rgl[N] = 1;
+    for (int i = 1; i < N; ++i) rgl[i] = gl[i - 1];
+    // warp(r) = sum_i (rgl_i - req_i) l_i(r) / (1 - r^2), l_i the equispaced Lagrange basis
+    auto warp = [&](double r) {
+        double w = 0;
+        for (int i = 0; i <= N; ++i) {
+            double li = 1;
+            for (int j = 0; j <= N; ++j)
+                if (j != i) li *= (r - req[j]) / (req[i] - req[j]);
+            w += (rgl[i] - req[i]) * li;
+        }
+        return w / (1 - r * r);
+    };
+    const double alpha = 0.1001, L1 = 0.5, L2 = 0.25, L3 = 0.25;
+    // only the two warps with a component along the median through vertex 1 survive
+    const double w2 = 4 * L3 * L1 * warp(L1 - L3) * (1 + alpha * alpha * L2 * L2);
+    const double w3 = 4 * L1 * L2 * warp(L2 - L1) * (1 + alpha * alpha * L3 * L3);
+    const double dy = std::sqrt(3.0) / 2 * (w2 - w3);
+    const double y = (2 * L1 - L2 - L3) / std::sqrt(3.0) + dy;   // side-2 equilateral triangle
+    return (std::sqrt(3.0) * y + 1) / 3;
 }
 
 std::vector<double> lagrangeNodes(int dim, int k) {
@@ -70,6 +102,17 @@ std::vector<double> lagrangeNodes(int dim, int k) {
         for (int f = 0; f < 4; ++f)
             for (int c = 0; c < 3; ++c)
                 out.push_back((V3[kTetFace[f][0]][c] + V3[kTetFace[f][1]][c] + V3[kTetFace[f][2]][c]) / 3.0);
+    if (dim == 3 && k == 4) {
+        // face l-th point near the face's l-th vertex (equispaced interior lattice order,
+        // feelpoly/equispaced.hpp:520-545), then the interior centroid
+        const double tb = p4FaceNearWeight(), ts = 0.5 * (1.0 - tb);
+        for (int f = 0; f < 4; ++f)
+            for (int l = 0; l < 3; ++l)
+                for (int c = 0; c < 3; ++c)
+                    out.push_back(tb * V3[kTetFace[f][l]][c] + ts * V3[kTetFace[f][(l + 1) % 3]][c] +
+                                  ts * V3[kTetFace[f][(l + 2) % 3]][c]);
+        for (int c = 0; c < 3; ++c) out.push_back(-0.5);
+    }
     return out;
 }
 

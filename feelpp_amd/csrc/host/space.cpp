@@ -306,17 +306,23 @@ DofTable buildDofTable(const SimplexMesh& m, int order, int ncomp) {
             }
         if (ndf > 0)
             for (int f = 0; f < 4; ++f) {
-                int32_t v[3] = {el[kTetFace[f][0]], el[kTetFace[f][1]], el[kTetFace[f][2]]};
+                const int32_t fv[3] = {el[kTetFace[f][0]], el[kTetFace[f][1]], el[kTetFace[f][2]]};
+                int32_t v[3] = {fv[0], fv[1], fv[2]};
                 std::sort(v, v + 3);
                 if (m.nv >= (1LL << 21)) throw std::invalid_argument("face keys need < 2^21 vertices");
                 const uint64_t key = ((uint64_t)v[0] << 42) | ((uint64_t)v[1] << 21) | (uint64_t)v[2];
                 bool ins;
                 const int32_t id = faces.findOrInsert(key, (int32_t)(fbase.size() / ndf), ins);
                 if (ins) fbase.resize(fbase.size() + ndf, -1);
-                if (ndf > 1) throw std::invalid_argument("face dof permutations (P4) not supported");
-                int32_t& s = fbase[(size_t)id * ndf];
-                if (s < 0) s = fresh();
-                put(s);
+                // several dofs per face (P4): slot by the mesh vertex the dof sits next to, which
+                // is what a conforming face permutation (vector_permutation,
+                // doffromelement.hpp:372-437) amounts to; the first element fixes the order
+                for (int l = 0; l < ndf; ++l) {
+                    const int rank = ndf == 1 ? 0 : (int)(std::find(v, v + 3, fv[l]) - v);
+                    int32_t& s = fbase[(size_t)id * ndf + rank];
+                    if (s < 0) s = fresh();
+                    put(s);
+                }
             }
         for (int l = 0; l < ndv; ++l) put(fresh());
         if (next >= (1LL << 31)) throw std::overflow_error("more than 2^31 dofs");

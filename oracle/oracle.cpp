@@ -89,9 +89,45 @@ std::vector<double> gll_interior(int k) {
     return {};
 }
 
+// Warp & blend face warp of PointSetWarpBlend (feelpoly/warpblend.hpp:366-500)
+// evaluated for the face-interior points of P4 (equispaced face barycentrics
+// (1/2,1/4,1/4)): warpFactor (:383-407) interpolates the GLL - equispaced
+// displacement divided by 1-r^2; the 2D deformation (:409-442) adds
+// blend_i * warp_i * (1 + (alpha_2d[4] L_i)^2) along the three edge directions,
+// alpha_2d[4] = 0.1001 (:334).  By symmetry the point stays on the face median;
+// returns its barycentric weight at the near vertex.
+double p4_face_weight() {
+    const int N = 4;
+    const double xeq[5] = {-1, -0.5, 0, 0.5, 1};
+    const double gll[5] = {-1, -std::sqrt(3.0 / 7.0), 0, std::sqrt(3.0 / 7.0), 1};
+    auto warp = [&](double r) {
+        double w = 0;
+        for (int i = 0; i <= N; ++i) {
+            double d = gll[i] - xeq[i];
+            for (int j = 1; j < N; ++j)
+                if (j != i) d *= (r - xeq[j]) / (xeq[i] - xeq[j]);
+            if (i != 0) d *= -1.0 / (xeq[i] - xeq[0]);
+            if (i != N) d *= 1.0 / (xeq[i] - xeq[N]);
+            w += d;
+        }
+        return w;
+    };
+    const double alpha = 0.1001;
+    const double L[3] = {0.5, 0.25, 0.25};
+    double dy = 0;
+    for (int i = 0; i < 3; ++i) {
+        const double blend = 4 * L[(i + 1) % 3] * L[(i + 2) % 3];
+        const double wi = blend * warp(L[(i + 2) % 3] - L[(i + 1) % 3]) * (1 + alpha * alpha * L[i] * L[i]);
+        dy += std::sin(2.0 * i * M_PI / 3.0) * wi;
+    }
+    // equilateral triangle of side 2, L[0] at (0, 2/sqrt3): y = (3 L0 - 1)/sqrt3
+    const double y = (3 * L[0] - 1) / std::sqrt(3.0) + dy;
+    return (std::sqrt(3.0) * y + 1) / 3;
+}
+
 bool lagrange_nodes(int dim, int k, std::vector<double>& nodes) {
     nodes.clear();
-    if (k < 1 || k > 3) return false;
+    if (k < 1 || k > 4) return false;
     if (dim == 2 && k > 2) return false;
     std::vector<std::vector<double>> V;
     if (dim == 2) V = {{-1, -1}, {1, -1}, {-1, 1}};
@@ -110,6 +146,20 @@ bool lagrange_nodes(int dim, int k, std::vector<double>& nodes) {
         for (int f = 0; f < 4; ++f)
             for (int c = 0; c < 3; ++c)
                 nodes.push_back((V[TET_F2P[f][0]][c] + V[TET_F2P[f][1]][c] + V[TET_F2P[f][2]][c]) / 3.0);
+    }
+    if (dim == 3 && k == 4) {
+        // three points per face; the l-th sits near the face's l-th vertex (equispaced
+        // interior triangle lattice order, equispaced.hpp:520-545, mapped by
+        // pt_to_face_tetrahedron :671-690)
+        const double tb = p4_face_weight(), ts = (1 - tb) / 2;
+        for (int f = 0; f < 4; ++f)
+            for (int l = 0; l < 3; ++l)
+                for (int c = 0; c < 3; ++c) {
+                    double x = 0;
+                    for (int m = 0; m < 3; ++m) x += (m == l ? tb : ts) * V[TET_F2P[f][m]][c];
+                    nodes.push_back(x);
+                }
+        for (int c = 0; c < 3; ++c) nodes.push_back(-0.5);   // interior: centroid
     }
     return true;
 }
@@ -455,7 +505,7 @@ int orc_lagrange_eval(int dim, int order, int npts, const double* pts, double* p
 // vertex pair.  Faces with one dof (P3) need no permutation.
 // ---------------------------------------------------------------------------
 long orc_doftable(int dim, int order, int ncomp, int ne, const int* elems, int* elem2dof) {
-    if (order < 1 || order > 3 || (dim == 2 && order > 2)) return -1;
+    if (order < 1 || order > 4 || (dim == 2 && order > 2)) return -1;
     const int nv = nvert(dim), nle = nedges(dim), nlf = nfaces3(dim);
     const int nde = dof_per_edge(order), ndf = dim == 3 ? dof_per_face(order) : 0;
     const int nl = nloc_scalar(dim, order);
@@ -482,10 +532,20 @@ long orc_doftable(int dim, int order, int ncomp, int ne, const int* elems, int* 
                 insert({1, lo, hi, sub});
             }
         }
+        // Face dofs: with one dof per face (P3) the key is the face.  With three (P4)
+        // the element that creates the face numbers them in its local order and the
+        // neighbour maps its local dofs through vector_permutation
+        // (doffromelement.hpp:372-437, doftable.hpp:1506-1516); a conforming
+        // permutation sends each dof to the one at the same physical point, i.e. near
+        // the same mesh vertex, so the key here is (face, nearest vertex).
         for (int f = 0; f < nlf; ++f) {
             int v[3] = {el[TET_F2P[f][0]], el[TET_F2P[f][1]], el[TET_F2P[f][2]]};
-            std::sort(v, v + 3);
-            for (int l = 0; l < ndf; ++l) insert({2, v[0], v[1], (long)v[2] * 16 + l});
+            int s3[3] = {v[0], v[1], v[2]};
+            std::sort(s3, s3 + 3);
+            for (int l = 0; l < ndf; ++l) {
+                const long near = ndf == 1 ? 0 : v[l];
+                insert({2, s3[0], (long)s3[1] * (1L << 31) + s3[2], near});
+            }
         }
         if (dim == 2 && order >= 3) return -1;
         for (int l = 0; l < (dim == 3 ? dof_per_vol(order) : 0); ++l) insert({3, e, l, 0});

@@ -31,7 +31,8 @@ def setup(ctx, orc, dim, order, n, jitter=0.15, seed=5, ncomp=1):
     return coords, elems, e2d, ndof, ia, ja, sp, g
 
 
-@pytest.mark.parametrize("dim,order,n,qo", [(2, 1, 16, 2), (2, 2, 8, 4), (3, 1, 5, 2), (3, 2, 5, 4), (3, 3, 3, 6)])
+@pytest.mark.parametrize("dim,order,n,qo", [(2, 1, 16, 2), (2, 2, 8, 4), (3, 1, 5, 2), (3, 2, 5, 4), (3, 3, 3, 6),
+                                            (3, 4, 2, 8)])
 def test_diffusion_mass_fp64_vs_oracle(ctx, orc, dim, order, n, qo):
     coords, elems, e2d, ndof, ia, ja, sp, g = setup(ctx, orc, dim, order, n)
     k = orc.lognormal(len(elems), 0.0, 1.0, 11)
@@ -147,7 +148,7 @@ def test_full_size_p2_properties(ctx):
     assert abs(c @ Aa - a @ y.get()) < 1e-10 * abs(c @ Aa) + 1e-12    # symmetric
 
 
-@pytest.mark.parametrize("order,qo", [(1, 2), (2, 4), (3, 6)])
+@pytest.mark.parametrize("order,qo", [(1, 2), (2, 4), (3, 6), (4, 8)])
 def test_elasticity_fp64_vs_oracle(ctx, orc, order, qo):
     # 1 x 0.25 x 0.25 beam, E log-uniform in [1,100], nu uniform in [0.2,0.35]
     coords, elems = orc.mesh_kuhn(4, 2, 2, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=13)

@@ -29,7 +29,8 @@ def test_random_fields_bit_exact(orc):
     assert np.array_equal(fa.uniform(777, 1.0, 100.0, 3, log=True), orc.uniform(777, 1.0, 100.0, 3, log=True))
 
 
-CASES = [(2, 1, 1), (2, 2, 1), (2, 1, 2), (3, 1, 1), (3, 2, 1), (3, 3, 1), (3, 1, 3), (3, 2, 3), (3, 3, 3)]
+CASES = [(2, 1, 1), (2, 2, 1), (2, 1, 2), (3, 1, 1), (3, 2, 1), (3, 3, 1), (3, 4, 1), (3, 1, 3), (3, 2, 3), (3, 3, 3),
+         (3, 4, 3)]
 
 
 def mesh_for(orc, dim, n=5):
@@ -75,7 +76,7 @@ def test_c1_full_size_pattern_bit_exact(orc):
     assert np.array_equal(e2d_o, e2d_p) and np.array_equal(ia_o, ia_p) and np.array_equal(ja_o, ja_p)
 
 
-@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3)])
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_reference_element_agrees(orc, dim, order):
     assert np.abs(fa.lagrange_nodes(dim, order) - orc.lagrange_nodes(dim, order)).max() < 1e-15
     for q in range(0, 9):

@@ -68,7 +68,7 @@ def test_stiffness_row_sums_vanish_and_symmetric(orc, dim, order, qorder):
     assert np.abs(A - A.T).max() < 1e-14 * np.abs(A).max()
 
 
-@pytest.mark.parametrize("order,qorder", [(2, 4), (3, 6)])
+@pytest.mark.parametrize("order,qorder", [(2, 4), (3, 6), (4, 6)])
 def test_stiffness_energy_of_quadratic(orc, order, qorder):
     # u = x^2 - y z is interpolated exactly by P2/P3: u^T K u = int |grad u|^2
     coords, elems, e2d, ndof, ia, ja = space(orc, 3, order, n=3, jitter=0.15)
@@ -81,7 +81,7 @@ def test_stiffness_energy_of_quadratic(orc, order, qorder):
 
 def test_elasticity_rigid_modes_and_linear_energy(orc):
     coords, elems = orc.mesh_kuhn(2, 1, 1, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=3)
-    for order, qorder in [(1, 2), (2, 4), (3, 6)]:
+    for order, qorder in [(1, 2), (2, 4), (3, 6), (4, 8)]:
         e2d, ndof = orc.doftable(3, order, 3, elems)
         ia, ja = orc.graph(e2d, ndof)
         E = np.full(len(elems), 10.0)
@@ -124,7 +124,7 @@ def test_rhs_sinsin_p1_converges(orc):
     assert errs[1] < errs[0] / 3 and errs[2] < errs[1] / 3
 
 
-@pytest.mark.parametrize("dim,order,qorder", [(2, 1, 2), (2, 2, 2), (3, 1, 0), (3, 2, 2)])
+@pytest.mark.parametrize("dim,order,qorder", [(2, 1, 2), (2, 2, 2), (3, 1, 0), (3, 2, 2), (3, 4, 6)])
 def test_dirichlet_patch_test(orc, dim, order, qorder):
     # pyfeelpp tests/test_forms.py:44-53: Laplace with u = x on the boundary gives u = x
     coords, elems, e2d, ndof, ia, ja = space(orc, dim, order, n=4, jitter=0.2)

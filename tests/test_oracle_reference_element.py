@@ -65,14 +65,14 @@ def test_nodes_match_reference_fixture(orc, fixture, dim, order, tol):
     assert np.abs(nodes - ref).max() < tol
 
 
-@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3)])
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_basis_nodal_property(orc, dim, order):
     nodes = orc.lagrange_nodes(dim, order)
     phi, dphi = orc.lagrange_eval(dim, order, nodes)
     assert np.abs(phi - np.eye(len(nodes))).max() < 1e-13
 
 
-@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3)])
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_basis_partition_of_unity(orc, dim, order):
     rng = np.random.default_rng(1)
     lam = rng.dirichlet(np.ones(dim + 1), size=20)
@@ -104,3 +104,37 @@ def test_basis_reproduces_polynomials(orc):
     p = lambda x: x[:, 0] ** 3 - 2 * x[:, 0] * x[:, 1] * x[:, 2] + x[:, 2] ** 2 - 0.5
     phi, _ = orc.lagrange_eval(3, 3, pts)
     assert np.abs(p(nodes) @ phi - p(pts)).max() < 1e-12
+
+
+def test_basis_p4_reproduces_quartics(orc):
+    nodes = orc.lagrange_nodes(3, 4)
+    rng = np.random.default_rng(5)
+    pts = 2 * rng.dirichlet(np.ones(4), size=15)[:, 1:] - 1
+    p = lambda x: x[:, 0] ** 4 - 3 * x[:, 1] ** 2 * x[:, 2] * x[:, 0] + x[:, 2] ** 3 - x[:, 1]
+    phi, _ = orc.lagrange_eval(3, 4, pts)
+    assert np.abs(p(nodes) @ phi - p(pts)).max() < 1e-11
+
+
+def test_p4_node_set_structure(orc):
+    """P4 warp-blend nodes: GLL points on edges (the code path the P3 fixture pins),
+    three points per face on the face medians, weight t at the near vertex,
+    interior centroid.  t follows from warpblend.hpp's warp factor, which for N=4
+    is wf(r) = (8/3)(sqrt(3/7) - 1/2) r; the closed form below restates it."""
+    nodes = orc.lagrange_nodes(3, 4)
+    V = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], float)
+    E = [(1, 2), (2, 0), (0, 1), (0, 3), (1, 3), (2, 3)]
+    F = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    g = np.sqrt(3 / 7)
+    for e, (a, b) in enumerate(E):
+        for m, s in enumerate([-g, 0.0, g]):
+            assert np.allclose(nodes[4 + 3 * e + m], V[a] * (1 - s) / 2 + V[b] * (1 + s) / 2, atol=1e-15)
+    wf = lambda r: 8.0 / 3.0 * (g - 0.5) * r
+    alpha = 0.1001
+    dy = np.sqrt(3) / 2 * 2 * (0.5 * wf(0.25) * (1 + alpha ** 2 / 16))
+    t = (np.sqrt(3) * (0.5 / np.sqrt(3) + dy) + 1) / 3
+    for f, vs in enumerate(F):
+        for l in range(3):
+            w = np.full(3, (1 - t) / 2)
+            w[l] = t
+            assert np.allclose(nodes[22 + 3 * f + l], w @ V[list(vs)], atol=1e-14)
+    assert np.allclose(nodes[34], [-0.5, -0.5, -0.5])

@@ -47,7 +47,7 @@ def test_eight_elements_vector_interleave(orc):
     assert (e2d2[:, :3] == 2 * e2d1).all() and (e2d2[:, 3:] == 2 * e2d1 + 1).all()
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
 def test_kuhn_numbering_is_conforming(orc, order):
     coords, elems = orc.mesh_kuhn(3, 2, 2, jitter=0.15, seed=5)
     e2d, ndof = orc.doftable(3, order, 1, elems)
@@ -61,7 +61,8 @@ def test_kuhn_numbering_is_conforming(orc, order):
              [(t[1], t[2]), (t[2], t[0]), (t[0], t[1]), (t[0], t[3]), (t[1], t[3]), (t[2], t[3])]}
     faces = {tuple(sorted(f)) for t in elems for f in
              [(t[1], t[2], t[3]), (t[0], t[2], t[3]), (t[0], t[1], t[3]), (t[0], t[1], t[2])]}
-    expect = nv + (order - 1) * len(edges) + (order - 1) * (order - 2) // 2 * len(faces)
+    expect = nv + (order - 1) * len(edges) + (order - 1) * (order - 2) // 2 * len(faces) + \
+        (order - 1) * (order - 2) * (order - 3) // 6 * len(elems)
     assert ndof == expect
 
 
@@ -82,7 +83,7 @@ def dense_pattern(e2d, ndof):
     return M
 
 
-@pytest.mark.parametrize("dim,order,ncomp", [(2, 1, 1), (2, 2, 1), (3, 1, 1), (3, 2, 1), (3, 3, 1), (3, 1, 3)])
+@pytest.mark.parametrize("dim,order,ncomp", [(2, 1, 1), (2, 2, 1), (3, 1, 1), (3, 2, 1), (3, 3, 1), (3, 4, 1), (3, 1, 3)])
 def test_graph_matches_dense_neighbours(orc, dim, order, ncomp):
     if dim == 2:
         coords, elems = orc.mesh_square(4, jitter=0.2, seed=3)
