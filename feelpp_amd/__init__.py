@@ -53,6 +53,12 @@ class SolveData(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("inner_rtol", C.c_double), ("maxit", C.c_int),
+                ("inner_maxit", C.c_int), ("max_outer", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64),
+                ("use_graph", C.c_int)]
+
+
 # exported symbols of the two C-ABIs: name -> (restype, argtypes)
 _SIGS = {
     # host
@@ -114,6 +120,15 @@ _SIGS = {
     "feel_hip_mat_zero_rows": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double]),
     "feel_hip_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "feel_hip_solve_pcg": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveData)]),
+    "feel_hip_batch_opts_default": (None, [C.POINTER(BatchOpts)]),
+    "feel_hip_batch_create": (C.c_int, [_vp, C.POINTER(BatchOpts), C.POINTER(_vp)]),
+    "feel_hip_batch_destroy": (C.c_int, [_vp]),
+    "feel_hip_batch_add_family": (C.c_int, [_vp, C.c_int, C.POINTER(_vp), _vp, _vp, C.POINTER(C.c_int)]),
+    "feel_hip_batch_submit": (C.c_int, [_vp, C.c_int, _f64p, C.POINTER(C.c_int64)]),
+    "feel_hip_batch_step": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "feel_hip_batch_poll": (C.c_int, [_vp, C.POINTER(C.c_int64), C.POINTER(SolveData)]),
+    "feel_hip_batch_solution": (C.c_int, [_vp, C.c_int64, _vp]),
+    "feel_hip_batch_device_bytes": (C.c_int64, [_vp]),
 }
 
 
@@ -422,3 +437,50 @@ def assemble_linear(space, b, kind, quad_order, params=(1.0,), add=False):
     p = (C.c_double * 4)(*(list(params) + [0.0] * (4 - len(params))))
     d = LinearDesc(kind, quad_order, p)
     _check(lib().feel_hip_assemble_linear(space.h, C.byref(d), b.h, int(add)))
+
+
+class Batch(_Handle):
+    """Batched / continuously batched solves of affine families A(mu) = sum_t mu_t A_t."""
+    _destroy = "feel_hip_batch_destroy"
+
+    def __init__(self, ctx, rtol=1e-8, inner_rtol=1e-4, precision=FP32, hbm_budget=0, use_graph=True,
+                 maxit=100000, max_outer=50):
+        o = BatchOpts()
+        lib().feel_hip_batch_opts_default(C.byref(o))
+        o.rtol, o.inner_rtol, o.precision, o.hbm_budget, o.use_graph = rtol, inner_rtol, precision, hbm_budget, int(
+            use_graph)
+        o.maxit, o.max_outer = maxit, max_outer
+        h = _vp()
+        _check(lib().feel_hip_batch_create(ctx.h, C.byref(o), C.byref(h)))
+        super().__init__(h, keep=(ctx,))
+        self.families = []
+
+    def add_family(self, terms, rhs, rows=None):
+        arr = (_vp * len(terms))(*[t.h for t in terms])
+        fam = C.c_int()
+        _check(lib().feel_hip_batch_add_family(self.h, len(terms), arr, rhs.h, rows.h if rows else None, C.byref(fam)))
+        self.families.append((terms, rhs, rows))
+        return fam.value
+
+    def submit(self, family, mu):
+        rid = C.c_int64()
+        _check(lib().feel_hip_batch_submit(self.h, family, np.ascontiguousarray(mu, dtype=np.float64), C.byref(rid)))
+        return rid.value
+
+    def step(self, iterations=32):
+        a, d = C.c_int(), C.c_int()
+        _check(lib().feel_hip_batch_step(self.h, iterations, C.byref(a), C.byref(d)))
+        return a.value, d.value
+
+    def poll(self):
+        rid, sd = C.c_int64(), SolveData()
+        got = _check(lib().feel_hip_batch_poll(self.h, C.byref(rid), C.byref(sd)))
+        return (rid.value, sd.as_dict()) if got else None
+
+    def solution(self, request, n):
+        x = np.zeros(n)
+        _check(lib().feel_hip_batch_solution(self.h, request, x.ctypes.data_as(_vp)))
+        return x
+
+    def device_bytes(self):
+        return lib().feel_hip_batch_device_bytes(self.h)

@@ -1,0 +1,856 @@
+// Batched and continuously batched assemble-and-solve of affine-parameter
+// Poisson families: A(mu) = sum_t mu_t A_t over resident term matrices
+// (the affine decomposition Feel++'s reduced-basis layer assembles once,
+// mor/mor/feel/feelmor/modelcrbbase.hpp:1244), homogeneous Dirichlet rows,
+// Jacobi-CG with FP32 inner solves inside FP64 iterative refinement (or FP64
+// throughout).  Every in-flight instance advances in the same kernel launches:
+// launches run over a work list of (instance, slice range) items, each
+// instance keeps its own CG state and its own iteration origin, so instances
+// admitted at different times share launches (continuous batching).
+#include <algorithm>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace feelpp_amd;
+
+namespace {
+
+#define TRY_CALL(expr)                       \
+    do {                                     \
+        int rc_ = (expr);                    \
+        if (rc_ != FEEL_HIP_OK) return rc_;  \
+    } while (0)
+
+constexpr int kItemSlices = 64;   // slices (64 rows each) per work item
+constexpr int kMaxTerms = 8;
+
+struct BInst {
+    int64_t n, nslices;
+    const uint32_t* slicePtr;
+    const int32_t* col;
+    const int32_t* sdiag;
+    const uint8_t* mask;       // per slot: 0 keep, 1 zero, 2 unit diagonal (Dirichlet)
+    const double* b64;         // family rhs, solver numbering, Dirichlet rows zero
+    const double* terms[kMaxTerms];
+    int nterms;
+    double mu[kMaxTerms];
+    double *val64, *dinv64, *x64, *r64, *w64;
+    void *valT, *dinvT, *xT, *rT, *zT, *pT, *wT, *bT;
+    CgState* st;
+    double* slots;             // beta[2], dpi[2]
+    double* part;              // per-instance partials: z.r | z.z | p.Ap, nitems each
+    int nitems;
+    int start;                 // global iteration at which the current inner solve began
+    int ibeg, iend;            // item range of this instance in the work list
+    double scale;              // inner rhs = scale * r64
+    double xscale;             // x64 += xscale * xT
+};
+struct BItem {
+    int inst, s0, s1;
+};
+
+template <typename T>
+__device__ inline T* tp(void* p) { return reinterpret_cast<T*>(p); }
+
+// per-instance sum of item partials, identical in every block of the instance
+__device__ inline double instReduce(const double* __restrict__ part, int n) {
+    double v = 0;
+    for (int i = (int)threadIdx.x; i < n; i += kBlock) v += part[i];
+    return blockReduceSum(v);
+}
+
+__device__ inline bool instDone(const CgState* st) { return blockDone(st); }
+
+// rows of an item
+__device__ inline void itemRows(const BInst& I, const BItem& it, int64_t& r0, int64_t& r1) {
+    r0 = (int64_t)it.s0 * 64;
+    r1 = std::min<int64_t>((int64_t)it.s1 * 64, I.n);
+}
+
+// A(mu) on the SELL slots of the item, FP64 and T copies
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBCombine(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem it = items[blockIdx.x];
+    const BInst& I = insts[it.inst];
+    const uint32_t a = I.slicePtr[it.s0], b = I.slicePtr[it.s1];
+    for (uint32_t s = a + threadIdx.x; s < b; s += kBlock) {
+        const uint8_t m = I.mask[s];
+        double v = 0.0;
+        if (m == 0)
+            for (int t = 0; t < I.nterms; ++t) v += I.mu[t] * I.terms[t][s];
+        else if (m == 2)
+            v = 1.0;
+        I.val64[s] = v;
+        tp<T>(I.valT)[s] = (T)v;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBDiag(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem it = items[blockIdx.x];
+    const BInst& I = insts[it.inst];
+    int64_t r0, r1;
+    itemRows(I, it, r0, r1);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const double d = I.val64[I.sdiag[r]];
+        const double inv = d != 0.0 ? 1.0 / d : 1.0;
+        I.dinv64[r] = inv;
+        const double dT = (double)tp<T>(I.valT)[I.sdiag[r]];
+        tp<T>(I.dinvT)[r] = (T)(dT != 0.0 ? 1.0 / dT : 1.0);
+    }
+}
+
+// x64 += xscale * xT (skipped when xscale == 0)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBAxpy(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem it = items[blockIdx.x];
+    const BInst& I = insts[it.inst];
+    if (I.xscale == 0.0) return;
+    int64_t r0, r1;
+    itemRows(I, it, r0, r1);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) I.x64[r] += I.xscale * (double)tp<T>(I.xT)[r];
+}
+
+// w64 = A64 x64 on the item's slices (one wavefront per slice)
+__global__ __launch_bounds__(kBlock) void kBSpmv64(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem it = items[blockIdx.x];
+    const BInst& I = insts[it.inst];
+    const int lane = threadIdx.x & 63;
+    for (int s = it.s0 + (int)(threadIdx.x >> 6); s < it.s1; s += kBlock / 64) {
+        const uint32_t base = I.slicePtr[s];
+        const int width = (int)((I.slicePtr[s + 1] - base) >> 6);
+        double acc = 0.0;
+        for (int k = 0; k < width; ++k) {
+            const size_t slot = base + (size_t)k * 64 + lane;
+            acc += I.val64[slot] * I.x64[I.col[slot]];
+        }
+        const int64_t r = (int64_t)s * 64 + lane;
+        if (r < I.n) I.w64[r] = acc;
+    }
+}
+
+// r64 = b - w64; partial ||D^{-1} r64||^2 per item
+__global__ __launch_bounds__(kBlock) void kBResidual(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem it = items[blockIdx.x];
+    const BInst& I = insts[it.inst];
+    double* zzPart = I.part + I.nitems;
+    int64_t r0, r1;
+    itemRows(I, it, r0, r1);
+    double zz = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const double ri = I.b64[r] - I.w64[r];
+        I.r64[r] = ri;
+        const double zi = I.dinv64[r] * ri;
+        zz += zi * zi;
+    }
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) zzPart[blockIdx.x - I.ibeg] = zz;
+}
+
+// one block per instance of the work list: out[i] = sum of its z.z partials
+__global__ __launch_bounds__(kBlock) void kBInstSum(const BInst* __restrict__ insts, double* __restrict__ out) {
+    const BInst& I = insts[blockIdx.x];
+    const double v = instReduce(I.part + I.nitems, I.nitems);
+    if (threadIdx.x == 0) out[blockIdx.x] = v;
+}
+
+// Dirichlet mask on the SELL slots (solver numbering): boundary row -> unit diagonal,
+// boundary column -> zero (MatZeroRowsColumns), padding -> zero
+__global__ __launch_bounds__(kBlock) void kBMask(int64_t n, int64_t nslices, const uint32_t* __restrict__ slicePtr,
+                                                const int32_t* __restrict__ col, const int32_t* __restrict__ slot2csr,
+                                                const int32_t* __restrict__ perm, const uint8_t* __restrict__ flag,
+                                                uint8_t* __restrict__ mask) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+    for (int64_t s = wave; s < nslices; s += nwaves) {
+        const uint32_t base = slicePtr[s];
+        const int width = (int)((slicePtr[s + 1] - base) >> 6);
+        const int64_t r = s * 64 + lane;
+        for (int k = 0; k < width; ++k) {
+            const size_t slot = base + (size_t)k * 64 + lane;
+            uint8_t m = 1;
+            if (r < n && slot2csr[slot] >= 0) {
+                const int32_t c = col[slot];
+                if (flag[perm[r]]) m = (c == r) ? 2 : 1;
+                else m = flag[perm[c]] ? 1 : 0;
+            }
+            mask[slot] = m;
+        }
+    }
+}
+
+// permuted rhs with zero Dirichlet rows
+__global__ __launch_bounds__(kBlock) void kBRhs(int64_t n, const int32_t* __restrict__ perm, const uint8_t* __restrict__ flag,
+                                               const double* __restrict__ b, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = flag[perm[i]] ? 0.0 : b[perm[i]];
+}
+
+// inner solve set-up: b = scale * r64, x = 0, r = b, z = D^{-1} r; partials z.r, z.z
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBInnerInit(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem it = items[blockIdx.x];
+    const BInst& I = insts[it.inst];
+    int64_t r0, r1;
+    itemRows(I, it, r0, r1);
+    double zr = 0.0, zz = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        const T bi = (T)(I.scale * I.r64[r]);
+        tp<T>(I.bT)[r] = bi;
+        tp<T>(I.xT)[r] = (T)0;
+        tp<T>(I.rT)[r] = bi;
+        const T zi = tp<T>(I.dinvT)[r] * bi;
+        tp<T>(I.zT)[r] = zi;
+        zr += (double)zi * (double)bi;
+        zz += (double)zi * (double)zi;
+    }
+    zr = blockReduceSum(zr);
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) { I.part[blockIdx.x - I.ibeg] = zr; I.part[I.nitems + blockIdx.x - I.ibeg] = zz; }
+}
+
+// CG direction with the convergence test (as kCgDirection, per instance)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ items, const BInst* __restrict__ insts,
+                                                     const int* __restrict__ gcount) {
+    const int bid = blockIdx.x;
+    const BItem item = items[bid];
+    const BInst& I = insts[item.inst];
+    CgState* st = I.st;
+    if (instDone(st)) return;
+    const int it = *gcount - I.start;
+    const double beta = instReduce(I.part, I.nitems);
+    const double dp = sqrt(instReduce(I.part + I.nitems, I.nitems));
+    const double ttol = it == 0 ? fmax(st->rtol * dp, st->atol) : st->ttol;
+    const double rnorm0 = it == 0 ? dp : st->rnorm0;
+    int reason = 0;
+    if (dp != dp) reason = -9;
+    else if (dp <= ttol) reason = dp < st->atol ? 3 : 2;
+    else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
+    else if (it >= st->maxit) reason = -3;
+    else if (beta == 0.0) reason = 3;
+    if (bid == I.ibeg && threadIdx.x == 0) {
+        st->dp = dp;
+        if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
+        if (reason) { st->done = 1; st->reason = reason; st->its = it; }
+        I.slots[it & 1] = beta;
+    }
+    if (reason) return;
+    const double bb = it == 0 ? 0.0 : beta / I.slots[(it - 1) & 1];
+    int64_t r0, r1;
+    itemRows(I, item, r0, r1);
+    T* p = tp<T>(I.pT);
+    const T* z = tp<T>(I.zT);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) p[r] = it == 0 ? z[r] : (T)((double)z[r] + bb * (double)p[r]);
+}
+
+// w = A p on the item's slices, partial p.Ap
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    const BItem item = items[blockIdx.x];
+    const BInst& I = insts[item.inst];
+    if (instDone(I.st)) return;
+    const int lane = threadIdx.x & 63;
+    const T* val = tp<T>(I.valT);
+    const T* x = tp<T>(I.pT);
+    T* y = tp<T>(I.wT);
+    double dot = 0.0;
+    for (int s = item.s0 + (int)(threadIdx.x >> 6); s < item.s1; s += kBlock / 64) {
+        const uint32_t base = I.slicePtr[s];
+        const int width = (int)((I.slicePtr[s + 1] - base) >> 6);
+        T a0 = 0, a1 = 0;
+        int k = 0;
+        for (; k + 1 < width; k += 2) {
+            const size_t s0 = base + (size_t)k * 64 + lane, s1 = s0 + 64;
+            a0 += __builtin_nontemporal_load(val + s0) * x[__builtin_nontemporal_load(I.col + s0)];
+            a1 += __builtin_nontemporal_load(val + s1) * x[__builtin_nontemporal_load(I.col + s1)];
+        }
+        if (k < width) {
+            const size_t s0 = base + (size_t)k * 64 + lane;
+            a0 += val[s0] * x[I.col[s0]];
+        }
+        const T acc = a0 + a1;
+        const int64_t r = (int64_t)s * 64 + lane;
+        if (r < I.n) {
+            y[r] = acc;
+            dot += (double)x[r] * (double)acc;
+        }
+    }
+    dot = blockReduceSum(dot);
+    if (threadIdx.x == 0) I.part[2 * I.nitems + blockIdx.x - I.ibeg] = dot;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ items, const BInst* __restrict__ insts,
+                                                  const int* __restrict__ gcount) {
+    const int bid = blockIdx.x;
+    const BItem item = items[bid];
+    const BInst& I = insts[item.inst];
+    CgState* st = I.st;
+    if (instDone(st)) return;
+    const int it = *gcount - I.start;
+    const double dpi = instReduce(I.part + 2 * I.nitems, I.nitems);
+    const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
+    if (bid == I.ibeg && threadIdx.x == 0) {
+        I.slots[2 + (it & 1)] = dpi;
+        if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
+    }
+    if (indefinite) return;
+    const double a = I.slots[it & 1] / dpi;
+    int64_t r0, r1;
+    itemRows(I, item, r0, r1);
+    T *x = tp<T>(I.xT), *r = tp<T>(I.rT), *z = tp<T>(I.zT);
+    const T *p = tp<T>(I.pT), *w = tp<T>(I.wT), *dinv = tp<T>(I.dinvT);
+    double zr = 0.0, zz = 0.0;
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        x[i] = (T)((double)x[i] + a * (double)p[i]);
+        const T ri = (T)((double)r[i] - a * (double)w[i]);
+        r[i] = ri;
+        const T zi = dinv[i] * ri;
+        z[i] = zi;
+        zr += (double)zi * (double)ri;
+        zz += (double)zi * (double)zi;
+    }
+    zr = blockReduceSum(zr);
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) { I.part[bid - I.ibeg] = zr; I.part[I.nitems + bid - I.ibeg] = zz; }
+}
+
+__global__ void kBBump(int* gcount) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *gcount += 1;
+}
+
+__global__ __launch_bounds__(kBlock) void kBPermuteOut(int64_t n, const int32_t* __restrict__ perm,
+                                                      const double* __restrict__ in, double* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[perm[i]] = in[i];
+}
+
+}  // namespace
+
+// ============================================================== host engine
+struct BatchFamily {
+    feel_hip_graph* g = nullptr;
+    int nterms = 0;
+    std::vector<double*> terms;   // SELL FP64 values per term
+    uint8_t* mask = nullptr;
+    double* b64 = nullptr;        // permuted rhs with zero Dirichlet rows
+    int64_t n = 0, nslices = 0, nslots = 0;
+    int nitems = 0;
+    size_t instBytes = 0;
+    std::vector<int> freeBufs;    // reusable instance buffer sets
+};
+
+struct InstBuffers {
+    double *val64 = nullptr, *dinv64 = nullptr, *x64 = nullptr, *r64 = nullptr, *w64 = nullptr, *out = nullptr;
+    double* part = nullptr;
+    void *valT = nullptr, *dinvT = nullptr, *xT = nullptr, *rT = nullptr, *zT = nullptr, *pT = nullptr, *wT = nullptr,
+         *bT = nullptr;
+};
+
+enum class Phase { Pending, Resid, Inner, Done, Released };
+
+struct Request {
+    int64_t id = 0;
+    int family = 0;
+    std::vector<double> mu;
+    Phase phase = Phase::Pending;
+    int buf = -1, state = -1;
+    int start = 0;
+    double dp = 0, rnorm0 = 0, ttol = 0, xscale = 0;
+    int its = 0, outer = 0, reason = 0;
+    bool unpermuted = false;   // solution copied to Feel++ numbering
+};
+
+struct feel_hip_batch {
+    feel_hip_ctx* ctx = nullptr;
+    feel_hip_batch_opts opts{};
+    int64_t used = 0;
+    std::vector<BatchFamily> fams;
+    std::vector<InstBuffers> bufs;
+    std::map<int64_t, Request> reqs;
+    std::deque<int64_t> pending;
+    std::deque<int64_t> finished;   // done, not yet polled
+    int64_t nextId = 0;
+    CgState* states = nullptr;
+    double* slots = nullptr;
+    int nStates = 0;
+    std::vector<int> freeStates;
+    BItem* dItems = nullptr;
+    BInst* dInsts = nullptr;
+    double* dSums = nullptr;
+    int capItems = 0, capInsts = 0;
+    int* gcount = nullptr;
+    int hgcount = 0;
+    hipGraphExec_t exec = nullptr;
+    int execItems = -1, execChunk = -1;
+    uint64_t execVersion = ~0ULL, version = 0;
+};
+
+namespace {
+
+template <typename T>
+int alloc(feel_hip_ctx* c, T** p, size_t count) {
+    return c->alloc((void**)p, sizeof(T) * count);
+}
+
+size_t tsize(const feel_hip_batch* B) { return B->opts.precision == FEEL_HIP_FP32 ? 4 : 8; }
+
+size_t instanceBytes(const feel_hip_batch* B, const BatchFamily& f) {
+    const size_t t = tsize(B);
+    return f.nslots * (8 + t) + f.n * (8 * 5 + t * 7) + sizeof(double) * 3 * f.nitems;
+}
+
+int allocInstance(feel_hip_batch* B, int fam, int* out) {
+    BatchFamily& f = B->fams[fam];
+    if (!f.freeBufs.empty()) {
+        *out = f.freeBufs.back();
+        f.freeBufs.pop_back();
+        return FEEL_HIP_OK;
+    }
+    feel_hip_ctx* c = B->ctx;
+    const size_t t = tsize(B);
+    InstBuffers ib;
+    TRY_CALL(alloc(c, &ib.val64, f.nslots));
+    TRY_CALL(c->alloc(&ib.valT, t * f.nslots));
+    for (double** p : {&ib.dinv64, &ib.x64, &ib.r64, &ib.w64, &ib.out}) TRY_CALL(alloc(c, p, f.n));
+    TRY_CALL(alloc(c, &ib.part, 3 * (size_t)f.nitems));
+    for (void** p : {&ib.dinvT, &ib.xT, &ib.rT, &ib.zT, &ib.pT, &ib.wT, &ib.bT}) TRY_CALL(c->alloc(p, t * f.n));
+    B->bufs.push_back(ib);
+    B->used += (int64_t)instanceBytes(B, f);
+    *out = (int)B->bufs.size() - 1;
+    return FEEL_HIP_OK;
+}
+
+int ensureStates(feel_hip_batch* B, int need) {
+    if (need <= B->nStates) return FEEL_HIP_OK;
+    const int cap = std::max(need, 2 * B->nStates + 16);
+    CgState* ns = nullptr;
+    double* nsl = nullptr;
+    TRY_CALL(alloc(B->ctx, &ns, cap));
+    TRY_CALL(alloc(B->ctx, &nsl, 4 * (size_t)cap));
+    if (B->nStates) {
+        FEEL_HIP_CHECK(hipMemcpyAsync(ns, B->states, sizeof(CgState) * B->nStates, hipMemcpyDeviceToDevice, B->ctx->stream));
+        FEEL_HIP_CHECK(hipMemcpyAsync(nsl, B->slots, sizeof(double) * 4 * B->nStates, hipMemcpyDeviceToDevice, B->ctx->stream));
+        FEEL_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+        B->ctx->release(B->states, sizeof(CgState) * B->nStates);
+        B->ctx->release(B->slots, sizeof(double) * 4 * B->nStates);
+    }
+    for (int s = cap - 1; s >= B->nStates; --s) B->freeStates.push_back(s);
+    B->states = ns;
+    B->slots = nsl;
+    B->nStates = cap;
+    ++B->version;
+    return FEEL_HIP_OK;
+}
+
+// Work list (and per-instance descriptors) for a set of requests.
+int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
+    std::vector<BItem> items;
+    std::vector<BInst> insts;
+    for (Request* r : rs) {
+        const BatchFamily& f = B->fams[r->family];
+        const InstBuffers& ib = B->bufs[r->buf];
+        BInst I{};
+        I.n = f.n;
+        I.nslices = f.nslices;
+        I.slicePtr = f.g->slicePtr;
+        I.col = f.g->scol;
+        I.sdiag = f.g->sdiag;
+        I.mask = f.mask;
+        I.b64 = f.b64;
+        I.nterms = f.nterms;
+        for (int t = 0; t < f.nterms; ++t) { I.terms[t] = f.terms[t]; I.mu[t] = r->mu[t]; }
+        I.val64 = ib.val64; I.dinv64 = ib.dinv64; I.x64 = ib.x64; I.r64 = ib.r64; I.w64 = ib.w64;
+        I.valT = ib.valT; I.dinvT = ib.dinvT; I.xT = ib.xT; I.rT = ib.rT; I.zT = ib.zT; I.pT = ib.pT;
+        I.wT = ib.wT; I.bT = ib.bT;
+        I.st = B->states + r->state;
+        I.slots = B->slots + 4 * (size_t)r->state;
+        I.part = ib.part;
+        I.nitems = f.nitems;
+        I.start = r->start;
+        I.scale = r->dp > 0 ? 1.0 / r->dp : 0.0;
+        I.xscale = r->xscale;
+        I.ibeg = (int)items.size();
+        for (int64_t s = 0; s < f.nslices; s += kItemSlices)
+            items.push_back({(int)insts.size(), (int)s, (int)std::min<int64_t>(f.nslices, s + kItemSlices)});
+        I.iend = (int)items.size();
+        insts.push_back(I);
+    }
+    feel_hip_ctx* c = B->ctx;
+    if ((int)items.size() > B->capItems) {
+        c->release(B->dItems, sizeof(BItem) * B->capItems);
+        B->capItems = (int)items.size() * 2;
+        TRY_CALL(alloc(c, &B->dItems, B->capItems));
+        ++B->version;
+    }
+    if ((int)insts.size() > B->capInsts) {
+        c->release(B->dInsts, sizeof(BInst) * B->capInsts);
+        c->release(B->dSums, sizeof(double) * B->capInsts);
+        B->capInsts = (int)insts.size() * 2;
+        TRY_CALL(alloc(c, &B->dInsts, B->capInsts));
+        TRY_CALL(alloc(c, &B->dSums, B->capInsts));
+        ++B->version;
+    }
+    if (!items.empty()) {
+        FEEL_HIP_CHECK(hipMemcpyAsync(B->dItems, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice, c->stream));
+        FEEL_HIP_CHECK(hipMemcpyAsync(B->dInsts, insts.data(), sizeof(BInst) * insts.size(), hipMemcpyHostToDevice, c->stream));
+        FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));   // host vectors go out of scope
+    }
+    *nItems = (int)items.size();
+    return FEEL_HIP_OK;
+}
+
+// per-instance ||D^{-1} r64|| after kBResidual on the same work list
+int residualNorms(feel_hip_batch* B, const std::vector<Request*>& rs) {
+    feel_hip_ctx* c = B->ctx;
+    hipLaunchKernelGGL(kBInstSum, dim3((int)rs.size()), dim3(kBlock), 0, c->stream, B->dInsts, B->dSums);
+    FEEL_HIP_CHECK(hipGetLastError());
+    std::vector<double> h(rs.size());
+    FEEL_HIP_CHECK(hipMemcpyAsync(h.data(), B->dSums, sizeof(double) * rs.size(), hipMemcpyDeviceToHost, c->stream));
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < rs.size(); ++i) rs[i]->dp = std::sqrt(h[i]);
+    return FEEL_HIP_OK;
+}
+
+template <typename T>
+int launchIterations(feel_hip_batch* B, int nItems, int chunk) {
+    feel_hip_ctx* c = B->ctx;
+    auto body = [&]() {
+        for (int k = 0; k < chunk; ++k) {
+            hipLaunchKernelGGL(kBDirection<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts, B->gcount);
+            hipLaunchKernelGGL(kBSpmv<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+            hipLaunchKernelGGL(kBUpdate<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts, B->gcount);
+            hipLaunchKernelGGL(kBBump, dim3(1), dim3(64), 0, c->stream, B->gcount);
+        }
+    };
+    if (!B->opts.use_graph) {
+        body();
+        FEEL_HIP_CHECK(hipGetLastError());
+        return FEEL_HIP_OK;
+    }
+    // the captured chunk reads the work list, descriptors and counter from device
+    // memory, so it stays valid while those buffers keep their addresses
+    if (!B->exec || B->execItems != nItems || B->execChunk != chunk || B->execVersion != B->version) {
+        if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
+        hipGraph_t graph;
+        FEEL_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        body();
+        FEEL_HIP_CHECK(hipStreamEndCapture(c->stream, &graph));
+        FEEL_HIP_CHECK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
+        (void)hipGraphDestroy(graph);
+        B->execItems = nItems;
+        B->execChunk = chunk;
+        B->execVersion = B->version;
+    }
+    FEEL_HIP_CHECK(hipGraphLaunch(B->exec, c->stream));
+    return FEEL_HIP_OK;
+}
+
+void finish(feel_hip_batch* B, Request& r, int reason) {
+    r.reason = reason;
+    r.phase = Phase::Done;
+    B->finished.push_back(r.id);
+}
+
+template <typename T>
+int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
+    feel_hip_ctx* c = B->ctx;
+    int nItems = 0;
+    // 1. admission by HBM footprint, FIFO
+    std::vector<Request*> admitted;
+    while (!B->pending.empty()) {
+        Request& r = B->reqs[B->pending.front()];
+        BatchFamily& f = B->fams[r.family];
+        const bool reuse = !f.freeBufs.empty();
+        if (!reuse && B->opts.hbm_budget > 0 && B->used + (int64_t)instanceBytes(B, f) > B->opts.hbm_budget) {
+            if (B->used == 0) return setError(FEEL_HIP_ENOMEM, "one instance exceeds the batch HBM budget");
+            break;
+        }
+        TRY_CALL(ensureStates(B, B->nStates - (int)B->freeStates.size() + 1));
+        TRY_CALL(allocInstance(B, r.family, &r.buf));
+        r.state = B->freeStates.back();
+        B->freeStates.pop_back();
+        admitted.push_back(&r);
+        B->pending.pop_front();
+    }
+    // 2. assembly of admitted instances: A(mu), D^{-1}; x = 0 so r = b
+    if (!admitted.empty()) {
+        for (Request* r : admitted) { r->dp = 0; r->xscale = 0; }
+        TRY_CALL(buildWork(B, admitted, &nItems));
+        hipLaunchKernelGGL(kBCombine<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+        hipLaunchKernelGGL(kBDiag<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+        for (Request* r : admitted) {
+            const InstBuffers& ib = B->bufs[r->buf];
+            const int64_t n = B->fams[r->family].n;
+            FEEL_HIP_CHECK(hipMemsetAsync(ib.x64, 0, sizeof(double) * n, c->stream));
+            FEEL_HIP_CHECK(hipMemsetAsync(ib.w64, 0, sizeof(double) * n, c->stream));
+        }
+        hipLaunchKernelGGL(kBResidual, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+        FEEL_HIP_CHECK(hipGetLastError());
+        TRY_CALL(residualNorms(B, admitted));
+        for (Request* r : admitted) {
+            r->rnorm0 = r->dp;
+            r->ttol = std::max(B->opts.rtol * r->rnorm0, B->opts.atol);
+            r->phase = Phase::Resid;
+        }
+    }
+    // 3. FP64 residual holders: converged, or start an inner solve
+    std::vector<Request*> toInner;
+    for (auto& kv : B->reqs) {
+        Request& r = kv.second;
+        if (r.phase != Phase::Resid) continue;
+        if (r.dp != r.dp) finish(B, r, -9);
+        else if (r.dp <= r.ttol) finish(B, r, r.dp < B->opts.atol ? 3 : 2);
+        else if (r.outer >= B->opts.max_outer || r.its >= B->opts.maxit) finish(B, r, -3);
+        else toInner.push_back(&r);
+    }
+    for (auto& kv : B->reqs) {
+        Request& r = kv.second;
+        if (r.phase == Phase::Done && r.buf >= 0 && !r.unpermuted) {
+            r.unpermuted = true;
+            const BatchFamily& f = B->fams[r.family];
+            const InstBuffers& ib = B->bufs[r.buf];
+            hipLaunchKernelGGL(kBPermuteOut, dim3((int)std::min<int64_t>(4096, (f.n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               c->stream, f.n, f.g->perm, ib.x64, ib.out);
+        }
+    }
+    std::vector<CgState> fresh(toInner.size());   // kept alive until buildWork synchronises
+    if (!toInner.empty()) {
+        for (size_t i = 0; i < toInner.size(); ++i) {
+            Request* r = toInner[i];
+            CgState& h = fresh[i];
+            h = CgState{};
+            h.rtol = B->opts.inner_rtol;
+            h.atol = 0;
+            h.dtol = 1e5;
+            h.maxit = std::min(B->opts.inner_maxit, B->opts.maxit - r->its);
+            FEEL_HIP_CHECK(hipMemcpyAsync(B->states + r->state, &h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+            r->start = B->hgcount;
+            r->phase = Phase::Inner;
+        }
+        TRY_CALL(buildWork(B, toInner, &nItems));
+        hipLaunchKernelGGL(kBInnerInit<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+        FEEL_HIP_CHECK(hipGetLastError());
+    }
+    // 4. one chunk of iterations for every instance in an inner solve
+    std::vector<Request*> inner;
+    for (auto& kv : B->reqs)
+        if (kv.second.phase == Phase::Inner) inner.push_back(&kv.second);
+    if (!inner.empty() && iterations > 0) {
+        TRY_CALL(buildWork(B, inner, &nItems));
+        TRY_CALL(launchIterations<T>(B, nItems, iterations));
+        B->hgcount += iterations;
+        std::vector<CgState> hs(B->nStates);
+        FEEL_HIP_CHECK(hipMemcpyAsync(hs.data(), B->states, sizeof(CgState) * B->nStates, hipMemcpyDeviceToHost, c->stream));
+        FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+        std::vector<Request*> ended;
+        for (Request* r : inner) {
+            const CgState& st = hs[r->state];
+            if (!st.done) continue;
+            r->its += st.its;
+            r->outer += 1;
+            if (st.reason == -8 || st.reason == -9) { finish(B, *r, st.reason); continue; }
+            r->xscale = r->dp;   // x64 += dp * x_inner (the inner rhs was r64 / dp)
+            ended.push_back(r);
+        }
+        // 5. FP64 residual of the instances whose inner solve ended
+        if (!ended.empty()) {
+            TRY_CALL(buildWork(B, ended, &nItems));
+            hipLaunchKernelGGL(kBAxpy<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+            hipLaunchKernelGGL(kBSpmv64, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+            hipLaunchKernelGGL(kBResidual, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+            FEEL_HIP_CHECK(hipGetLastError());
+            TRY_CALL(residualNorms(B, ended));
+            for (Request* r : ended) { r->xscale = 0; r->phase = Phase::Resid; }
+        }
+    }
+    int act = 0, done = 0;
+    for (auto& kv : B->reqs) {
+        if (kv.second.phase == Phase::Done) ++done;
+        else if (kv.second.phase == Phase::Resid || kv.second.phase == Phase::Inner) ++act;
+    }
+    if (nActive) *nActive = act + (int)B->pending.size();
+    if (nDone) *nDone = done;
+    return FEEL_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void feel_hip_batch_opts_default(feel_hip_batch_opts* o) {
+    if (!o) return;
+    o->rtol = 1e-8;
+    o->atol = 1e-50;
+    o->inner_rtol = 1e-4;
+    o->maxit = 100000;
+    o->inner_maxit = 100000;
+    o->max_outer = 50;
+    o->precision = FEEL_HIP_FP32;
+    o->hbm_budget = 0;
+    o->use_graph = 1;
+}
+
+int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out) {
+    if (!ctx || !out) return setError(FEEL_HIP_EINVAL, "null argument");
+    auto* B = new feel_hip_batch();
+    B->ctx = ctx;
+    if (opts) B->opts = *opts;
+    else feel_hip_batch_opts_default(&B->opts);
+    if (B->opts.precision != FEEL_HIP_FP32 && B->opts.precision != FEEL_HIP_FP64) {
+        delete B;
+        return setError(FEEL_HIP_EINVAL, "batch precision must be FP32 (refined) or FP64");
+    }
+    int rc = alloc(ctx, &B->gcount, 1);
+    if (rc == FEEL_HIP_OK && hipMemsetAsync(B->gcount, 0, sizeof(int), ctx->stream) != hipSuccess)
+        rc = setError(FEEL_HIP_EDEVICE, "hipMemset failed");
+    if (rc != FEEL_HIP_OK) { delete B; return rc; }
+    *out = B;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_batch_destroy(feel_hip_batch* B) {
+    if (!B) return FEEL_HIP_OK;
+    feel_hip_ctx* c = B->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    for (InstBuffers& ib : B->bufs) {
+        for (double* p : {ib.val64, ib.dinv64, ib.x64, ib.r64, ib.w64, ib.out, ib.part}) if (p) (void)hipFree(p);
+        for (void* p : {ib.valT, ib.dinvT, ib.xT, ib.rT, ib.zT, ib.pT, ib.wT, ib.bT}) if (p) (void)hipFree(p);
+    }
+    c->bytes -= B->used;   // instance buffers were counted through ctx->alloc
+    for (auto& f : B->fams) {
+        for (double* p : f.terms) c->release(p, sizeof(double) * f.nslots);
+        c->release(f.mask, f.nslots);
+        c->release(f.b64, sizeof(double) * f.n);
+    }
+    c->release(B->states, sizeof(CgState) * B->nStates);
+    c->release(B->slots, sizeof(double) * 4 * B->nStates);
+    c->release(B->dItems, sizeof(BItem) * B->capItems);
+    c->release(B->dInsts, sizeof(BInst) * B->capInsts);
+    c->release(B->dSums, sizeof(double) * B->capInsts);
+    c->release(B->gcount, sizeof(int));
+    delete B;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const* terms, const feel_hip_vec* rhs,
+                              const feel_hip_rows* dirichlet, int* family) {
+    if (!B || !terms || !rhs || !family || nterms < 1) return setError(FEEL_HIP_EINVAL, "bad argument");
+    if (nterms > kMaxTerms) return setError(FEEL_HIP_EUNSUPPORTED, "at most 8 affine terms");
+    feel_hip_graph* g = terms[0]->g;
+    for (int t = 0; t < nterms; ++t) {
+        if (!terms[t] || terms[t]->g != g) return setError(FEEL_HIP_EINVAL, "all terms must share one graph");
+        if (!terms[t]->v64) return setError(FEEL_HIP_EINVAL, "terms need FP64 values");
+    }
+    if (rhs->n != g->n) return setError(FEEL_HIP_EINVAL, "rhs size mismatch");
+    if (dirichlet && dirichlet->n != g->n) return setError(FEEL_HIP_EINVAL, "Dirichlet rows built for another size");
+    feel_hip_ctx* c = B->ctx;
+    BatchFamily f;
+    f.g = g;
+    f.nterms = nterms;
+    f.n = g->n;
+    f.nslices = g->nslices;
+    f.nslots = g->nslots;
+    f.nitems = (int)((f.nslices + kItemSlices - 1) / kItemSlices);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((f.nslots + kBlock - 1) / kBlock, 8192));
+    for (int t = 0; t < nterms; ++t) {
+        double* v = nullptr;
+        TRY_CALL(alloc(c, &v, f.nslots));
+        hipLaunchKernelGGL(kCsrToSell<double>, dim3(grid), dim3(kBlock), 0, c->stream, f.nslots, g->slot2csr, terms[t]->v64,
+                           (const float*)nullptr, v);
+        f.terms.push_back(v);
+    }
+    uint8_t* zeroFlags = nullptr;
+    const uint8_t* flags = dirichlet ? dirichlet->flag : nullptr;
+    if (!flags) {
+        TRY_CALL(alloc(c, &zeroFlags, f.n));
+        FEEL_HIP_CHECK(hipMemsetAsync(zeroFlags, 0, f.n, c->stream));
+        flags = zeroFlags;
+    }
+    TRY_CALL(alloc(c, &f.mask, f.nslots));
+    hipLaunchKernelGGL(kBMask, dim3(grid), dim3(kBlock), 0, c->stream, f.n, f.nslices, g->slicePtr, g->scol, g->slot2csr,
+                       g->perm, flags, f.mask);
+    TRY_CALL(alloc(c, &f.b64, f.n));
+    hipLaunchKernelGGL(kBRhs, dim3(grid), dim3(kBlock), 0, c->stream, f.n, g->perm, flags, rhs->d, f.b64);
+    FEEL_HIP_CHECK(hipGetLastError());
+    FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+    if (zeroFlags) c->release(zeroFlags, f.n);
+    B->fams.push_back(std::move(f));
+    *family = (int)B->fams.size() - 1;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_batch_submit(feel_hip_batch* B, int family, const double* mu, int64_t* request) {
+    if (!B || !mu || !request) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (family < 0 || family >= (int)B->fams.size()) return setError(FEEL_HIP_EINVAL, "unknown family");
+    Request r;
+    r.id = B->nextId++;
+    r.family = family;
+    r.mu.assign(mu, mu + B->fams[family].nterms);
+    B->reqs[r.id] = r;
+    B->pending.push_back(r.id);
+    *request = r.id;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_batch_step(feel_hip_batch* B, int iterations, int* n_active, int* n_done) {
+    if (!B || iterations < 0) return setError(FEEL_HIP_EINVAL, "bad argument");
+    try {
+        return B->opts.precision == FEEL_HIP_FP32 ? stepImpl<float>(B, iterations, n_active, n_done)
+                                                  : stepImpl<double>(B, iterations, n_active, n_done);
+    } catch (const std::exception& e) {
+        return setError(FEEL_HIP_EINVAL, e.what());
+    }
+}
+
+int feel_hip_batch_poll(feel_hip_batch* B, int64_t* request, feel_hip_solve_data* sd) {
+    if (!B || !request) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (B->finished.empty()) return 0;
+    const int64_t id = B->finished.front();
+    B->finished.pop_front();
+    const Request& r = B->reqs[id];
+    *request = id;
+    if (sd) {
+        sd->converged = r.reason > 0;
+        sd->its = r.its;
+        sd->residual = r.dp;
+        sd->reason = r.reason;
+        sd->outer_its = r.outer;
+        sd->rnorm0 = r.rnorm0;
+    }
+    return 1;
+}
+
+int feel_hip_batch_solution(feel_hip_batch* B, int64_t request, double* host_x) {
+    if (!B) return setError(FEEL_HIP_EINVAL, "null batch");
+    auto it = B->reqs.find(request);
+    if (it == B->reqs.end() || it->second.phase != Phase::Done) return setError(FEEL_HIP_EINVAL, "request not finished");
+    Request& r = it->second;
+    const BatchFamily& f = B->fams[r.family];
+    if (host_x) {
+        FEEL_HIP_CHECK(hipMemcpyAsync(host_x, B->bufs[r.buf].out, sizeof(double) * f.n, hipMemcpyDeviceToHost, B->ctx->stream));
+    }
+    FEEL_HIP_CHECK(hipStreamSynchronize(B->ctx->stream));
+    // release the instance's buffers and CG state for later admissions
+    B->fams[r.family].freeBufs.push_back(r.buf);
+    B->freeStates.push_back(r.state);
+    r.buf = -1;
+    r.state = -1;
+    r.phase = Phase::Released;
+    B->reqs.erase(it);
+    return FEEL_HIP_OK;
+}
+
+int64_t feel_hip_batch_device_bytes(const feel_hip_batch* B) { return B ? B->used : -1; }
+
+}  // extern "C"

@@ -186,6 +186,41 @@ void feel_hip_solve_opts_default(feel_hip_solve_opts* o);
 int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, const feel_hip_solve_opts* opts,
                        feel_hip_solve_data* out);
 
+/* ---- batched and continuously batched solves ------------------------------
+ * Affine-parameter families A(mu) = sum_t mu_t A_t (the affine decomposition of
+ * Feel++'s reduced-basis layer, mor/mor/feel/feelmor/modelcrbbase.hpp:1244, used
+ * here for many full-order solves): the term matrices, rhs and homogeneous
+ * Dirichlet rows stay resident; each submitted instance is assembled (A(mu), D^-1)
+ * and solved with Jacobi-CG (FP32 inner solves inside FP64 refinement, or FP64),
+ * all in-flight instances sharing kernel launches.  Instances are admitted in
+ * submission order while the HBM budget allows (0 = unlimited).  No counterpart
+ * in the reference, which solves one system at a time (SolverLinearPetsc::solve). */
+typedef struct feel_hip_batch feel_hip_batch;
+typedef struct {
+    double rtol, atol;        /* outer FP64 test ||D^-1 r|| <= max(rtol ||D^-1 b||, atol) */
+    double inner_rtol;        /* FP32 inner reduction per refinement step */
+    int maxit, inner_maxit, max_outer;
+    int precision;            /* FEEL_HIP_FP32 (refined) or FEEL_HIP_FP64 */
+    int64_t hbm_budget;       /* bytes of instance buffers allowed at once, 0 = unlimited */
+    int use_graph;            /* capture each iteration chunk in a hipGraph */
+} feel_hip_batch_opts;
+void feel_hip_batch_opts_default(feel_hip_batch_opts* o);
+int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out);
+int feel_hip_batch_destroy(feel_hip_batch* b);
+/* terms: nterms assembled FP64 matrices on one graph; rhs: the family's load vector;
+ * dirichlet: rows eliminated symmetrically with a unit diagonal (g = 0), or null */
+int feel_hip_batch_add_family(feel_hip_batch* b, int nterms, feel_hip_mat* const* terms, const feel_hip_vec* rhs,
+                              const feel_hip_rows* dirichlet, int* family);
+int feel_hip_batch_submit(feel_hip_batch* b, int family, const double* mu, int64_t* request);
+/* admits pending instances, then runs `iterations` CG iterations of every in-flight
+ * instance and advances their refinement; n_active counts pending + in-flight */
+int feel_hip_batch_step(feel_hip_batch* b, int iterations, int* n_active, int* n_done);
+/* pops one finished request (returns 1), or returns 0 when none is waiting */
+int feel_hip_batch_poll(feel_hip_batch* b, int64_t* request, feel_hip_solve_data* sd);
+/* copies the solution (Feel++ dof numbering) and releases the request's buffers */
+int feel_hip_batch_solution(feel_hip_batch* b, int64_t request, double* host_x);
+int64_t feel_hip_batch_device_bytes(const feel_hip_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

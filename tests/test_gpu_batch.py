@@ -1,0 +1,122 @@
+"""Batched and continuously batched solves of affine families (GPU, C-ABI).
+
+Each instance's solution must match the plain FP64 device solve of the same
+A(mu) (itself pinned to the oracle by tests/test_gpu_solve.py) within 1e-8,
+and be bitwise identical whether solved alone or in a mix of instances."""
+import numpy as np
+import pytest
+
+import feelpp_amd as fa
+from helpers import rel_fro
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    return fa.Context(0)
+
+
+def family(ctx, n, order, seed=3):
+    coords, elems = fa.mesh_kuhn_box(n, jitter=0.15, seed=seed)
+    e2d, ndof = fa.doftable(3, order, 1, elems, coords.shape[0])
+    ia, ja = fa.graph(e2d, ndof)
+    sp = fa.Space(ctx, coords, elems, order, 1, e2d, ndof)
+    g = fa.Graph(sp, ia, ja)
+    cen = coords[elems].mean(axis=1)
+    sub = (cen[:, 0] > 0.5).astype(int) + 2 * (cen[:, 1] > 0.5).astype(int)   # 4 axis-aligned subdomains
+    terms = []
+    for t in range(4):
+        A = fa.Matrix(g, fa.FP64)
+        A.assemble(fa.DIFFUSION, 2 * order - 2 if order > 1 else 2, kdiff=ctx.vector(len(elems), (sub == t) * 1.0))
+        terms.append(A)
+    b = ctx.vector(ndof)
+    fa.assemble_linear(sp, b, fa.SRC_CONST, order, (1.0,))
+    isb = fa.boundary_dofs(3, order, 1, coords, elems, e2d, ndof)
+    rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
+    return dict(sp=sp, g=g, terms=terms, b=b, rows=rows, sub=sub, ndof=ndof, order=order, ne=len(elems))
+
+
+def reference(ctx, F, mu):
+    A = fa.Matrix(F["g"], fa.FP64)
+    order = F["order"]
+    A.assemble(fa.DIFFUSION, 2 * order - 2 if order > 1 else 2, kdiff=ctx.vector(F["ne"], np.asarray(mu)[F["sub"]]))
+    b = ctx.vector(F["ndof"])
+    fa.assemble_linear(F["sp"], b, fa.SRC_CONST, order, (1.0,))
+    A.zero_rows(F["rows"], None, b, mode=fa.ELIMINATION_SYMMETRIC)
+    x = ctx.vector(F["ndof"])
+    st = A.solve(b, x, rtol=1e-13, maxit=100000)
+    assert st["converged"]
+    return x.get()
+
+
+def run_all(B, reqs, chunk=16, limit=10000):
+    out = {}
+    for _ in range(limit):
+        act, _ = B.step(chunk)
+        while True:
+            got = B.poll()
+            if not got:
+                break
+            out[got[0]] = got[1]
+        if act == 0 and len(out) == len(reqs):
+            break
+    return out
+
+
+@pytest.mark.parametrize("precision", [fa.FP32, fa.FP64])
+def test_batch_matches_single_solves(ctx, precision):
+    fams = [family(ctx, 6, 1), family(ctx, 4, 2), family(ctx, 3, 3)]
+    B = fa.Batch(ctx, rtol=1e-10, precision=precision)
+    ids = [B.add_family(F["terms"], F["b"], F["rows"]) for F in fams]
+    rng = np.random.default_rng(0)
+    reqs = {}
+    for k in range(9):
+        fi = k % 3
+        mu = np.exp(rng.uniform(np.log(0.1), np.log(10), 4))
+        reqs[B.submit(ids[fi], mu)] = (fi, mu)
+    done = run_all(B, reqs)
+    assert set(done) == set(reqs)
+    for rid, (fi, mu) in reqs.items():
+        assert done[rid]["converged"], done[rid]
+        x = B.solution(rid, fams[fi]["ndof"])
+        assert rel_fro(x, reference(ctx, fams[fi], mu)) < 1e-8
+
+
+def test_alone_equals_in_the_mix(ctx):
+    F1, F2 = family(ctx, 5, 2), family(ctx, 6, 1)
+    mu = np.array([0.3, 2.0, 7.5, 0.9])
+    B1 = fa.Batch(ctx, rtol=1e-10)
+    f1 = B1.add_family(F1["terms"], F1["b"], F1["rows"])
+    r = B1.submit(f1, mu)
+    run_all(B1, {r: 0})
+    alone = B1.solution(r, F1["ndof"])
+    B2 = fa.Batch(ctx, rtol=1e-10)
+    g1 = B2.add_family(F1["terms"], F1["b"], F1["rows"])
+    g2 = B2.add_family(F2["terms"], F2["b"], F2["rows"])
+    rng = np.random.default_rng(1)
+    reqs = {B2.submit(g2, np.exp(rng.uniform(-2, 2, 4))): 0 for _ in range(3)}
+    target = B2.submit(g1, mu)
+    reqs[target] = 0
+    reqs.update({B2.submit(g1, np.exp(rng.uniform(-2, 2, 4))): 0 for _ in range(2)})
+    run_all(B2, reqs, chunk=7)
+    mixed = B2.solution(target, F1["ndof"])
+    assert np.array_equal(alone, mixed)
+
+
+def test_admission_by_memory_budget(ctx):
+    F = family(ctx, 5, 2)
+    probe = fa.Batch(ctx, rtol=1e-9)
+    fp = probe.add_family(F["terms"], F["b"], F["rows"])
+    probe.submit(fp, np.ones(4))
+    probe.step(0)
+    one = probe.device_bytes()
+    B = fa.Batch(ctx, rtol=1e-9, hbm_budget=int(2.5 * one))
+    f = B.add_family(F["terms"], F["b"], F["rows"])
+    reqs = {B.submit(f, np.full(4, 1.0 + k)): k for k in range(6)}
+    B.step(0)
+    assert B.device_bytes() <= 2.5 * one          # only two admitted at first
+    done = run_all(B, reqs)
+    assert set(done) == set(reqs) and all(d["converged"] for d in done.values())
+    for rid in reqs:
+        B.solution(rid, F["ndof"])
