@@ -50,7 +50,9 @@ def reference(ctx, F, mu):
     return x.get()
 
 
-def run_all(B, reqs, chunk=16, limit=10000):
+def run_all(B, ndofs, chunk=16, limit=20000):
+    """Step until every request finished; fetch each solution when it completes
+    (which releases its buffers for waiting requests).  ndofs: request -> size."""
     out = {}
     for _ in range(limit):
         act, _ = B.step(chunk)
@@ -58,8 +60,9 @@ def run_all(B, reqs, chunk=16, limit=10000):
             got = B.poll()
             if not got:
                 break
-            out[got[0]] = got[1]
-        if act == 0 and len(out) == len(reqs):
+            rid, sd = got
+            out[rid] = (sd, B.solution(rid, ndofs[rid]))
+        if act == 0 and len(out) == len(ndofs):
             break
     return out
 
@@ -75,11 +78,11 @@ def test_batch_matches_single_solves(ctx, precision):
         fi = k % 3
         mu = np.exp(rng.uniform(np.log(0.1), np.log(10), 4))
         reqs[B.submit(ids[fi], mu)] = (fi, mu)
-    done = run_all(B, reqs)
+    done = run_all(B, {r: fams[fi]["ndof"] for r, (fi, _) in reqs.items()})
     assert set(done) == set(reqs)
     for rid, (fi, mu) in reqs.items():
-        assert done[rid]["converged"], done[rid]
-        x = B.solution(rid, fams[fi]["ndof"])
+        sd, x = done[rid]
+        assert sd["converged"], sd
         assert rel_fro(x, reference(ctx, fams[fi], mu)) < 1e-8
 
 
@@ -89,18 +92,16 @@ def test_alone_equals_in_the_mix(ctx):
     B1 = fa.Batch(ctx, rtol=1e-10)
     f1 = B1.add_family(F1["terms"], F1["b"], F1["rows"])
     r = B1.submit(f1, mu)
-    run_all(B1, {r: 0})
-    alone = B1.solution(r, F1["ndof"])
+    alone = run_all(B1, {r: F1["ndof"]})[r][1]
     B2 = fa.Batch(ctx, rtol=1e-10)
     g1 = B2.add_family(F1["terms"], F1["b"], F1["rows"])
     g2 = B2.add_family(F2["terms"], F2["b"], F2["rows"])
     rng = np.random.default_rng(1)
-    reqs = {B2.submit(g2, np.exp(rng.uniform(-2, 2, 4))): 0 for _ in range(3)}
+    reqs = {B2.submit(g2, np.exp(rng.uniform(-2, 2, 4))): F2["ndof"] for _ in range(3)}
     target = B2.submit(g1, mu)
-    reqs[target] = 0
-    reqs.update({B2.submit(g1, np.exp(rng.uniform(-2, 2, 4))): 0 for _ in range(2)})
-    run_all(B2, reqs, chunk=7)
-    mixed = B2.solution(target, F1["ndof"])
+    reqs[target] = F1["ndof"]
+    reqs.update({B2.submit(g1, np.exp(rng.uniform(-2, 2, 4))): F1["ndof"] for _ in range(2)})
+    mixed = run_all(B2, reqs, chunk=7)[target][1]
     assert np.array_equal(alone, mixed)
 
 
@@ -113,10 +114,9 @@ def test_admission_by_memory_budget(ctx):
     one = probe.device_bytes()
     B = fa.Batch(ctx, rtol=1e-9, hbm_budget=int(2.5 * one))
     f = B.add_family(F["terms"], F["b"], F["rows"])
-    reqs = {B.submit(f, np.full(4, 1.0 + k)): k for k in range(6)}
+    reqs = {B.submit(f, np.full(4, 1.0 + k)): F["ndof"] for k in range(6)}
     B.step(0)
     assert B.device_bytes() <= 2.5 * one          # only two admitted at first
     done = run_all(B, reqs)
-    assert set(done) == set(reqs) and all(d["converged"] for d in done.values())
-    for rid in reqs:
-        B.solution(rid, F["ndof"])
+    assert set(done) == set(reqs) and all(d[0]["converged"] for d in done.values())
+    assert B.device_bytes() <= 2.5 * one
