@@ -131,6 +131,133 @@ def cpu_baseline(orc, host, order, qorder, its_needed, budget_s=25.0):
     }
 
 
+def affine_families(fa, ctx, orders, sizes):
+    from feelpp_amd.families import Family
+    fams = {}
+    for k in orders:
+        for n in sizes:
+            fams[(k, n)] = Family(ctx, k, n, seed=100 * k + n)
+            F = fams[(k, n)]
+            log(f"family P{k} n={n}: {F.ndof} dofs, {F.nnz} nnz, setup {F.setup_s:.1f}s")
+    return fams
+
+
+def run_batch(fa, B, fams, ids, draws, chunk):
+    """Submit all draws at once, step until done; returns per-request latency (s) and wall time."""
+    t0 = time.perf_counter()
+    sub = {}
+    for key, mu in draws:
+        sub[B.submit(ids[key], mu)] = (key, t0)
+    lat, stats = {}, {}
+    while len(lat) < len(sub):
+        B.step(chunk)
+        while True:
+            got = B.poll()
+            if not got:
+                break
+            rid, sd = got
+            B.solution(rid, fams[sub[rid][0]].ndof)
+            lat[rid] = time.perf_counter() - sub[rid][1]
+            stats[rid] = sd
+    return lat, stats, time.perf_counter() - t0
+
+
+def bench_batched(fa, args):
+    """512 affine Poisson problems (orders P1-P4, Kuhn n in {8,12,16,24,32}) in one batch."""
+    ctx = fa.Context(0)
+    orders = [int(o) for o in args.orders.split(",")]
+    sizes = [int(n) for n in args.sizes.split(",")]
+    fams = affine_families(fa, ctx, orders, sizes)
+    rng = np.random.default_rng(args.seed)
+    keys = list(fams)
+    draws = [(keys[rng.integers(len(keys))], np.exp(rng.uniform(np.log(0.1), np.log(10.0), 4)))
+             for _ in range(args.problems)]
+    B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol, precision=fa.FP32, use_graph=True)
+    ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
+    run_batch(fa, B, fams, ids, draws[:min(8, len(draws))], args.chunk)      # warm-up
+    lat, stats, wall = run_batch(fa, B, fams, ids, draws, args.chunk)
+    its = [s["its"] for s in stats.values()]
+    ok = all(s["converged"] for s in stats.values())
+    # plain FP64 GPU form on a sample: assemble A(mu) + Dirichlet + FP64 Jacobi-CG, one at a time
+    sample = draws[:args.plain_sample]
+    t1 = time.perf_counter()
+    for key, mu in sample:
+        F = fams[key]
+        A = fa.Matrix(F.graph, fa.FP64)
+        A.assemble(fa.DIFFUSION, F.qorder, kdiff=ctx.vector(F.ne, mu[F.sub]))
+        b = ctx.vector(F.ndof)
+        fa.assemble_linear(F.space, b, fa.SRC_CONST, F.order, (1.0,))
+        A.zero_rows(F.rows, None, b, mode=fa.ELIMINATION_SYMMETRIC)
+        x = ctx.vector(F.ndof)
+        A.solve(b, x, rtol=args.rtol, maxit=100000)
+    plain_s = (time.perf_counter() - t1) / max(len(sample), 1) * len(draws)
+    L = np.array(list(lat.values()))
+    line = {
+        "metric": "batched affine Poisson assemble+solve/s (FP32 CG in FP64 refinement)",
+        "value": round(len(draws) / wall, 3), "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+        "config": {"workload": f"{len(draws)} affine Poisson problems, P{orders} on Kuhn n in {sizes}, "
+                               f"4-term subdomain coefficient mu log-uniform [0.1,10], rtol {args.rtol:g}"},
+        "wall_s": round(wall, 3), "converged": ok, "iterations_mean": float(np.mean(its)),
+        "latency_ms": {"p50": round(float(np.percentile(L, 50)) * 1e3, 2),
+                       "p99": round(float(np.percentile(L, 99)) * 1e3, 2)},
+        "plain_fp64_sequential_solves_per_s": round(len(draws) / plain_s, 3),
+        "plain_fp64_sample": len(sample),
+        "speedup_vs_plain_fp64": round(plain_s / wall, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(fa, args):
+    """Poisson-arrival stream of affine Poisson requests (C4's size/order mix) at several rates."""
+    ctx = fa.Context(0)
+    orders = [int(o) for o in args.orders.split(",")]
+    sizes = [int(n) for n in args.sizes.split(",")]
+    fams = affine_families(fa, ctx, orders, sizes)
+    keys = list(fams)
+    out = []
+    for rate in [float(r) for r in args.rates.split(",")]:
+        rng = np.random.default_rng(int(rate) + args.seed)
+        nreq = int(min(args.requests, max(20, rate * args.seconds)))
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+        draws = [(keys[rng.integers(len(keys))], np.exp(rng.uniform(np.log(0.1), np.log(10.0), 4)))
+                 for _ in range(nreq)]
+        B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol, precision=fa.FP32, use_graph=True,
+                     hbm_budget=int(args.hbm_gib * 2**30))
+        ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
+        run_batch(fa, B, fams, ids, draws[:4], args.chunk)   # warm-up (graph capture, allocations)
+        t0 = time.perf_counter()
+        nxt, sub, lat = 0, {}, []
+        last = t0
+        while len(lat) < nreq:
+            now = time.perf_counter() - t0
+            while nxt < nreq and arrivals[nxt] <= now:
+                key, mu = draws[nxt]
+                sub[B.submit(ids[key], mu)] = (key, arrivals[nxt])
+                nxt += 1
+            act, _ = B.step(args.chunk)
+            while True:
+                got = B.poll()
+                if not got:
+                    break
+                rid, sd = got
+                B.solution(rid, fams[sub[rid][0]].ndof)
+                done_t = time.perf_counter() - t0
+                lat.append(done_t - sub[rid][1])
+                last = done_t
+            if act == 0 and nxt < nreq:
+                time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.01)))
+        L = np.array(lat)
+        rec = {"rate_per_s": rate, "requests": nreq, "p50_ms": round(float(np.percentile(L, 50)) * 1e3, 2),
+               "p99_ms": round(float(np.percentile(L, 99)) * 1e3, 2),
+               "sustained_solves_per_s": round(nreq / (last - arrivals[0]), 2)}
+        log(f"serving {rec}")
+        out.append(rec)
+    print(json.dumps({"metric": "serving p50/p99 latency and sustained solves/s (one MI355X)", "n_gpus": 1,
+                      "config": {"workload": f"Poisson arrivals, affine Poisson P{orders} Kuhn n in {sizes}, "
+                                             f"rtol {args.rtol:g}, chunk {args.chunk} CG iterations"},
+                      "rates": out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,12 +269,31 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-4)
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving"])
+    ap.add_argument("--orders", default="1,2,3,4")
+    ap.add_argument("--sizes", default="8,12,16,24,32")
+    ap.add_argument("--problems", type=int, default=512)
+    ap.add_argument("--plain-sample", type=int, default=32)
+    ap.add_argument("--chunk", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--requests", type=int, default=200)
+    ap.add_argument("--seconds", type=float, default=4.0)
+    ap.add_argument("--hbm-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import feelpp_amd as fa
+    if args.config == "batched":
+        if args.rtol == 1e-10:
+            args.rtol = 1e-9
+        return bench_batched(fa, args)
+    if args.config == "serving":
+        if args.rtol == 1e-10:
+            args.rtol = 1e-9
+        return bench_serving(fa, args)
     order, qorder = args.order, 2 * args.order
     host = build_problem(fa, args.n, order, 0.15, 2024)
     ctx = fa.Context(0)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import feelpp_amd as fa
+from feelpp_amd.families import Family
 from helpers import rel_fro
 
 pytestmark = pytest.mark.gpu
@@ -18,29 +19,15 @@ def ctx():
 
 
 def family(ctx, n, order, seed=3):
-    coords, elems = fa.mesh_kuhn_box(n, jitter=0.15, seed=seed)
-    e2d, ndof = fa.doftable(3, order, 1, elems, coords.shape[0])
-    ia, ja = fa.graph(e2d, ndof)
-    sp = fa.Space(ctx, coords, elems, order, 1, e2d, ndof)
-    g = fa.Graph(sp, ia, ja)
-    cen = coords[elems].mean(axis=1)
-    sub = (cen[:, 0] > 0.5).astype(int) + 2 * (cen[:, 1] > 0.5).astype(int)   # 4 axis-aligned subdomains
-    terms = []
-    for t in range(4):
-        A = fa.Matrix(g, fa.FP64)
-        A.assemble(fa.DIFFUSION, 2 * order - 2 if order > 1 else 2, kdiff=ctx.vector(len(elems), (sub == t) * 1.0))
-        terms.append(A)
-    b = ctx.vector(ndof)
-    fa.assemble_linear(sp, b, fa.SRC_CONST, order, (1.0,))
-    isb = fa.boundary_dofs(3, order, 1, coords, elems, e2d, ndof)
-    rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
-    return dict(sp=sp, g=g, terms=terms, b=b, rows=rows, sub=sub, ndof=ndof, order=order, ne=len(elems))
+    F = Family(ctx, order, n, seed=seed)
+    return dict(obj=F, sp=F.space, g=F.graph, terms=F.terms, b=F.rhs, rows=F.rows, sub=F.sub, ndof=F.ndof,
+                order=order, ne=F.ne, qorder=F.qorder)
 
 
 def reference(ctx, F, mu):
     A = fa.Matrix(F["g"], fa.FP64)
     order = F["order"]
-    A.assemble(fa.DIFFUSION, 2 * order - 2 if order > 1 else 2, kdiff=ctx.vector(F["ne"], np.asarray(mu)[F["sub"]]))
+    A.assemble(fa.DIFFUSION, F["qorder"], kdiff=ctx.vector(F["ne"], np.asarray(mu)[F["sub"]]))
     b = ctx.vector(F["ndof"])
     fa.assemble_linear(F["sp"], b, fa.SRC_CONST, order, (1.0,))
     A.zero_rows(F["rows"], None, b, mode=fa.ELIMINATION_SYMMETRIC)
