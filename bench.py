@@ -149,7 +149,11 @@ def run_batch(fa, B, fams, ids, draws, chunk):
     for key, mu in draws:
         sub[B.submit(ids[key], mu)] = (key, t0)
     lat, stats = {}, {}
+    last_log = t0
     while len(lat) < len(sub):
+        if time.perf_counter() - last_log > 10:
+            last_log = time.perf_counter()
+            log(f"batch: {len(lat)}/{len(sub)} finished after {last_log - t0:.0f}s")
         B.step(chunk)
         while True:
             got = B.poll()
