@@ -99,6 +99,11 @@ struct feel_hip_graph {
     int32_t* slot2csr = nullptr;
     int32_t* perm = nullptr;      // new -> old row
     int32_t* sdiag = nullptr;     // diagonal slot per new row
+    bool compact = false;         // packed 16/32-bit column stream available
+    int32_t* colBase = nullptr;   // nslices, -1 = slice stores int32 columns
+    int64_t* ccptr = nullptr;     // nslices, slice start in packed (uint16 units)
+    uint16_t* packed = nullptr;   // npacked
+    int64_t npacked = 0;
 };
 
 struct feel_hip_mat {

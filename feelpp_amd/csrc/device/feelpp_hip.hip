@@ -422,6 +422,13 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
         if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->slot2csr, L.slot2csr.data(), L.slot2csr.size());
         if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->perm, L.perm.data(), L.perm.size());
         if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->sdiag, L.diagSlot.data(), L.diagSlot.size());
+        if (rc == FEEL_HIP_OK && L.compact) {
+            g->compact = true;
+            g->npacked = (int64_t)L.packed.size();
+            rc = upload(ctx, &g->colBase, L.colBase.data(), L.colBase.size());
+            if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->ccptr, L.cptr.data(), L.cptr.size());
+            if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->packed, L.packed.data(), L.packed.size());
+        }
     }
     if (rc != FEEL_HIP_OK) {
         feel_hip_graph_destroy(g);
@@ -446,6 +453,11 @@ int feel_hip_graph_destroy(feel_hip_graph* g) {
     c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
     c->release(g->perm, sizeof(int32_t) * g->n);
     c->release(g->sdiag, sizeof(int32_t) * g->n);
+    if (g->compact) {
+        c->release(g->colBase, sizeof(int32_t) * g->nslices);
+        c->release(g->ccptr, sizeof(int64_t) * g->nslices);
+        c->release(g->packed, sizeof(uint16_t) * g->npacked);
+    }
     delete g;
     return FEEL_HIP_OK;
 }

@@ -558,11 +558,14 @@ __device__ inline double blockReduceSumN(double v) {
 // loads (streamed once; keeps L2 for the x gathers), U independent partial
 // sums per lane for memory-level parallelism.  With part, also the block
 // partial of x.y (p.Ap).  Grid-stride over slices with a fixed grid.
-template <typename T, int BS, int U>
+// COMPACT: columns stored as 16-bit offsets from a per-slice base (colBase),
+// 2 bytes per slot instead of 4.
+template <typename T, int BS, int U, bool COMPACT>
 __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
-                                                const int32_t* __restrict__ col, const T* __restrict__ val,
-                                                const T* __restrict__ x, T* __restrict__ y, double* __restrict__ part,
-                                                const CgState* __restrict__ st) {
+                                                const int32_t* __restrict__ col, const int32_t* __restrict__ colBase,
+                                                const int64_t* __restrict__ cptr, const uint16_t* __restrict__ packed,
+                                                const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
+                                                double* __restrict__ part, const CgState* __restrict__ st) {
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
@@ -572,20 +575,37 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
         const uint32_t base = slicePtr[s];
         const int width = (int)((slicePtr[s + 1] - base) >> 6);
         const T* v = val + base + lane;
-        const int32_t* c = col + base + lane;
         T acc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) acc[u] = 0;
-        int k = 0;
-        for (; k + U <= width; k += U) {
+        // one wave per slice: the narrow/wide choice is uniform across the wave
+        const int32_t cb = COMPACT ? colBase[s] : -1;
+        if (cb >= 0) {
+            const uint16_t* c = packed + cptr[s] + lane;
+            const T* xs = x + cb;
+            int k = 0;
+            for (; k + U <= width; k += U) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
-                const int32_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
-                acc[u] += vv * x[cc];
+                for (int u = 0; u < U; ++u) {
+                    const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
+                    const uint16_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
+                    acc[u] += vv * xs[cc];
+                }
             }
+            for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * xs[c[(size_t)k * 64]];
+        } else {
+            const int32_t* c = COMPACT ? reinterpret_cast<const int32_t*>(packed + cptr[s]) + lane : col + base + lane;
+            int k = 0;
+            for (; k + U <= width; k += U) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
+                    const int32_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
+                    acc[u] += vv * x[cc];
+                }
+            }
+            for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
         }
-        for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
         T a = 0;
 #pragma unroll
         for (int u = 0; u < U; ++u) a += acc[u];

@@ -25,8 +25,12 @@ namespace {
 template <typename T>
 void spmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st) {
     constexpr int U = sizeof(T) == 8 ? 4 : 8;
-    hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
-                       g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
+    if (g->compact)
+        hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U, true>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
+                           g->nslices, g->slicePtr, g->scol, g->colBase, g->ccptr, g->packed, val, x, y, part, st);
+    else
+        hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U, false>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
+                           g->nslices, g->slicePtr, g->scol, g->colBase, g->ccptr, g->packed, val, x, y, part, st);
 }
 
 int gridN(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 8192)); }

@@ -1,8 +1,11 @@
 #include "space.hpp"
 
+#include <cstring>
+
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <istream>
 #include <map>
@@ -465,16 +468,70 @@ GatherMap buildGatherMap(const DofTable& dt, const GraphCSR& g, int nthreads) {
     return gm;
 }
 
-SellLayout buildSell(const GraphCSR& g, int window) {
+std::vector<int32_t> reverseCuthillMcKee(const GraphCSR& g) {
+    const int64_t n = g.nrows;
+    std::vector<int32_t> order;
+    order.reserve(n);
+    std::vector<uint8_t> seen(n, 0);
+    auto deg = [&](int32_t v) { return g.ia[v + 1] - g.ia[v]; };
+    std::vector<int32_t> nb;
+    // BFS from a pseudo-peripheral node of each connected component
+    auto bfsLast = [&](int32_t s, std::vector<int32_t>& lvl) {
+        std::vector<int32_t> q{s};
+        lvl.assign(1, s);
+        std::vector<uint8_t>& mark = seen;
+        std::vector<int32_t> touched{s};
+        mark[s] = 2;
+        size_t h = 0;
+        int32_t last = s;
+        while (h < q.size()) {
+            const int32_t v = q[h++];
+            last = v;
+            for (int32_t p = g.ia[v]; p < g.ia[v + 1]; ++p) {
+                const int32_t w = g.ja[p];
+                if (mark[w] == 0) { mark[w] = 2; q.push_back(w); touched.push_back(w); }
+            }
+        }
+        for (int32_t t : touched) mark[t] = 0;
+        return last;
+    };
+    std::vector<int32_t> tmp;
+    for (int32_t root = 0; root < n; ++root) {
+        if (seen[root]) continue;
+        int32_t start = bfsLast(root, tmp);
+        start = bfsLast(start, tmp);   // two sweeps approximate a peripheral node
+        size_t head = order.size();
+        order.push_back(start);
+        seen[start] = 1;
+        while (head < order.size()) {
+            const int32_t v = order[head++];
+            nb.clear();
+            for (int32_t p = g.ia[v]; p < g.ia[v + 1]; ++p)
+                if (!seen[g.ja[p]]) { seen[g.ja[p]] = 1; nb.push_back(g.ja[p]); }
+            std::sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) {
+                return deg(a) != deg(b) ? deg(a) < deg(b) : a < b;
+            });
+            order.insert(order.end(), nb.begin(), nb.end());
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    return order;
+}
+
+SellLayout buildSell(const GraphCSR& g, int window, bool reorder) {
     SellLayout L;
     const int64_t n = g.nrows;
     const int C = SellLayout::C;
     L.nrows = n;
     L.perm.resize(n);
     L.iperm.resize(n);
+    std::vector<int32_t> base(n);
+    if (reorder) base = reverseCuthillMcKee(g);
+    else
+        for (int64_t r = 0; r < n; ++r) base[r] = (int32_t)r;
     for (int64_t w0 = 0; w0 < n; w0 += window) {
         const int64_t w1 = std::min<int64_t>(n, w0 + window);
-        for (int64_t r = w0; r < w1; ++r) L.perm[r] = (int32_t)r;
+        for (int64_t r = w0; r < w1; ++r) L.perm[r] = base[r];
         std::stable_sort(L.perm.begin() + w0, L.perm.begin() + w1, [&](int32_t a, int32_t b) {
             return g.ia[a + 1] - g.ia[a] > g.ia[b + 1] - g.ia[b];
         });
@@ -500,7 +557,7 @@ SellLayout buildSell(const GraphCSR& g, int window) {
         const int width = (int)((L.slicePtr[s + 1] - L.slicePtr[s]) / C);
         for (int lane = 0; lane < C; ++lane) {
             const int64_t r = s * C + lane;
-            const int64_t self = r < n ? r : 0;
+            const int64_t self = r < n ? r : n - 1;   // padding rows point inside the slice's range
             const int32_t old = r < n ? L.perm[r] : 0;
             const int len = r < n ? g.ia[old + 1] - g.ia[old] : 0;
             for (int k = 0; k < width; ++k) {
@@ -515,6 +572,42 @@ SellLayout buildSell(const GraphCSR& g, int window) {
                 }
             }
         }
+    }
+    // packed columns: 16-bit offsets for slices spanning fewer than 2^16 columns
+    L.colBase.assign(L.nslices, -1);
+    L.cptr.assign(L.nslices, 0);
+    int64_t narrow = 0, units = 0;
+    for (int64_t s = 0; s < L.nslices; ++s) {
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (uint32_t q = L.slicePtr[s]; q < L.slicePtr[s + 1]; ++q) { lo = std::min(lo, L.col[q]); hi = std::max(hi, L.col[q]); }
+        const int64_t w = L.slicePtr[s + 1] - L.slicePtr[s];
+        L.cptr[s] = units;
+        if (w == 0 || (int64_t)hi - lo < 65536) {
+            L.colBase[s] = w == 0 ? 0 : lo;
+            narrow += w;
+            units += w;
+        } else {
+            units += 2 * w;
+        }
+    }
+    L.compactFraction = L.nslots ? (double)narrow / (double)L.nslots : 1.0;
+    // below a quarter of narrow slots the packed stream saves too little to pay
+    // for the second copy of the columns
+    if (L.compactFraction >= 0.25) {
+        L.compact = true;
+        L.packed.resize(units);
+        for (int64_t s = 0; s < L.nslices; ++s) {
+            uint16_t* out = L.packed.data() + L.cptr[s];
+            const uint32_t b = L.slicePtr[s];
+            if (L.colBase[s] >= 0) {
+                for (uint32_t q = b; q < L.slicePtr[s + 1]; ++q) out[q - b] = (uint16_t)(L.col[q] - L.colBase[s]);
+            } else {
+                std::memcpy(out, L.col.data() + b, sizeof(int32_t) * (L.slicePtr[s + 1] - b));
+            }
+        }
+    } else {
+        L.colBase.clear();
+        L.cptr.clear();
     }
     return L;
 }

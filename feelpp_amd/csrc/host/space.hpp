@@ -80,11 +80,25 @@ struct SellLayout {
     int64_t nrows = 0, nslices = 0, nslots = 0;
     std::vector<uint32_t> slicePtr;   // nslices + 1
     std::vector<int32_t> col;         // nslots, permuted column
+    // Packed column stream: slice s starts at packed[cptr[s]] (uint16 units).
+    // A slice whose columns span fewer than 2^16 stores 16-bit offsets from
+    // colBase[s] >= 0; any other slice (colBase[s] = -1) stores its int32
+    // columns as pairs of uint16 (cptr is always even, so they stay aligned).
+    bool compact = false;             // packed stream built (enough slices are narrow)
+    double compactFraction = 0.0;     // fraction of slots in narrow slices
+    std::vector<int32_t> colBase;     // nslices
+    std::vector<int64_t> cptr;        // nslices
+    std::vector<uint16_t> packed;
     std::vector<int32_t> slot2csr;    // nslots, CSR position or -1
     std::vector<int32_t> perm, iperm; // nrows
     std::vector<int32_t> diagSlot;    // nrows (new numbering)
 };
-SellLayout buildSell(const GraphCSR& g, int window);
+// reorder: apply reverse Cuthill-McKee before the window sort (bandwidth
+// reduction, so slices span few columns and x gathers stay local)
+SellLayout buildSell(const GraphCSR& g, int window, bool reorder = true);
+
+// Reverse Cuthill-McKee ordering of a symmetric CSR graph (new -> old)
+std::vector<int32_t> reverseCuthillMcKee(const GraphCSR& g);
 
 int defaultThreads();
 
