@@ -47,6 +47,12 @@ def lib():
         L.orc_assemble_scalar.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p,
                                           C.c_double, C.c_void_p, C.c_double, C.c_void_p,
                                           _i32p, _i32p, _f64p]
+        L.orc_assemble_scalar_mt.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p,
+                                             C.c_double, C.c_void_p, C.c_double, C.c_void_p, C.c_long,
+                                             _i32p, _i32p, _f64p, C.c_int]
+        L.orc_pcg_mt.argtypes = [C.c_long, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
+                                 C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int)]
         L.orc_assemble_elasticity.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p,
                                               _f64p, _f64p, _i32p, _i32p, _f64p]
         L.orc_assemble_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _i32p,
@@ -164,6 +170,21 @@ def assemble_scalar(dim, order, qorder, coords, elems, elem2dof, ia, ja, diff=1.
     return val
 
 
+def assemble_scalar_mt(dim, order, qorder, coords, elems, elem2dof, ia, ja, diff=1.0, kdiff=None,
+                       mass=0.0, kmass=None, threads=1):
+    """Threaded CPU baseline of assemble_scalar (row ownership, ghost elements recomputed)."""
+    val = np.zeros(ja.shape[0])
+    kd = None if kdiff is None else np.ascontiguousarray(kdiff, dtype=np.float64)
+    km = None if kmass is None else np.ascontiguousarray(kmass, dtype=np.float64)
+    rc = lib().orc_assemble_scalar_mt(dim, order, qorder, elems.shape[0], np.ascontiguousarray(coords),
+                                      np.ascontiguousarray(elems, dtype=np.int32),
+                                      np.ascontiguousarray(elem2dof, dtype=np.int32), diff, _ptr(kd), mass,
+                                      _ptr(km), ia.shape[0] - 1, ia, ja, val, threads)
+    if rc != 0:
+        raise RuntimeError(f"orc_assemble_scalar_mt failed ({rc})")
+    return val
+
+
 def assemble_elasticity(order, qorder, coords, elems, elem2dof, E, nu, ia, ja):
     val = np.zeros(ja.shape[0])
     rc = lib().orc_assemble_elasticity(order, qorder, elems.shape[0], np.ascontiguousarray(coords),
@@ -207,6 +228,16 @@ def spmv(ia, ja, val, x):
     y = np.zeros(ia.shape[0] - 1)
     lib().orc_spmv(ia.shape[0] - 1, ia, ja, val, np.ascontiguousarray(x, dtype=np.float64), y)
     return y
+
+
+def pcg_mt(ia, ja, val, b, x0=None, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, threads=1):
+    """Threaded CPU baseline of pcg (same algorithm, reassociated sums)."""
+    n = ia.shape[0] - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    its, res, reason = C.c_int(), C.c_double(), C.c_int()
+    lib().orc_pcg_mt(n, ia, ja, val, np.ascontiguousarray(b, dtype=np.float64), x, rtol, atol, dtol, maxit,
+                     threads, C.byref(its), C.byref(res), C.byref(reason))
+    return x, its.value, res.value, reason.value
 
 
 def pcg(ia, ja, val, b, x0=None, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000):

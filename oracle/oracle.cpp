@@ -24,6 +24,8 @@
  * The Krylov trajectory (iteration counts) has no reference fixture: parity
  * unpinned for iteration counts; the solution itself is pinned.
  */
+#include <omp.h>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -594,6 +596,43 @@ long orc_graph(int nldof, int ne, long ndof, const int* elem2dof, int* ia, int* 
 // kdiff / kmass are per-element P0 coefficients (nullptr -> 1).
 // Returns 0, or -1 on unsupported input, -2 if an entry falls outside the pattern.
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+// Element matrix of kd*grad(u).grad(v) + km*u*v on one simplex with vertex
+// coordinates X; Ae row-major (test index i, bilinearform.hpp:462-492).
+void scalar_element(const Tab& T, int dim, const double* X, double kd, double km, std::vector<double>& gphi,
+                    std::vector<double>& prod, std::vector<double>& Ae) {
+    const int n = T.n, Q = T.Q;
+    Geo g = geomap(dim, X);
+    for (int q = 0; q < Q; ++q) prod[q] = T.w[q] * g.J;
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < Q; ++q)
+            for (int r = 0; r < dim; ++r) {
+                double s = 0;
+                for (int c = 0; c < dim; ++c) s += g.B[r][c] * T.dphi[(i * Q + q) * dim + c];
+                gphi[(i * Q + q) * dim + r] = s;
+            }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            double s = 0;
+            for (int q = 0; q < Q; ++q) {
+                double ex = 0;
+                if (kd != 0) {
+                    double gg = 0;
+                    for (int r = 0; r < dim; ++r) gg += gphi[(j * Q + q) * dim + r] * gphi[(i * Q + q) * dim + r];
+                    ex += kd * gg;
+                }
+                if (km != 0) ex += km * T.phi[j * Q + q] * T.phi[i * Q + q];
+                s += prod[q] * ex;
+            }
+            Ae[i * n + j] = s;
+        }
+}
+}  // namespace
+
+extern "C" {
+
 int orc_assemble_scalar(int dim, int order, int qorder, int ne, const double* coords, const int* elems,
                         const int* elem2dof, double diff, const double* kdiff, double mass,
                         const double* kmass, const int* ia, const int* ja, double* val) {
@@ -604,38 +643,58 @@ int orc_assemble_scalar(int dim, int order, int qorder, int ne, const double* co
     double X[12];
     for (long e = 0; e < ne; ++e) {
         elem_coords(dim, coords, elems + (long)(dim + 1) * e, X);
-        Geo g = geomap(dim, X);
-        for (int q = 0; q < Q; ++q) prod[q] = T.w[q] * g.J;
-        for (int i = 0; i < n; ++i)
-            for (int q = 0; q < Q; ++q)
-                for (int r = 0; r < dim; ++r) {
-                    double s = 0;
-                    for (int c = 0; c < dim; ++c) s += g.B[r][c] * T.dphi[(i * Q + q) * dim + c];
-                    gphi[(i * Q + q) * dim + r] = s;
-                }
-        double kd = diff * (kdiff ? kdiff[e] : 1.0);
-        double km = mass * (kmass ? kmass[e] : 1.0);
-        for (int j = 0; j < n; ++j)
-            for (int i = 0; i < n; ++i) {
-                double s = 0;
-                for (int q = 0; q < Q; ++q) {
-                    double ex = 0;
-                    if (kd != 0) {
-                        double gg = 0;
-                        for (int r = 0; r < dim; ++r) gg += gphi[(j * Q + q) * dim + r] * gphi[(i * Q + q) * dim + r];
-                        ex += kd * gg;
-                    }
-                    if (km != 0) ex += km * T.phi[j * Q + q] * T.phi[i * Q + q];
-                    s += prod[q] * ex;
-                }
-                Ae[i * n + j] = s;   // row-major, test index i (bilinearform.hpp:462-492)
-            }
+        scalar_element(T, dim, X, diff * (kdiff ? kdiff[e] : 1.0), mass * (kmass ? kmass[e] : 1.0), gphi, prod, Ae);
         const int* l2g = elem2dof + (long)e * n;
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j)
                 if (!csr_add(ia, ja, val, l2g[i], l2g[j], Ae[i * n + j])) return -2;
     }
     return 0;
+}
+
+// Threaded stand-in for Feel++'s MPI assembly (a CPU baseline, not a parity
+// path): nthreads workers each own a contiguous block of rows, as an MPI rank
+// owns its dofs, and assemble every element that touches those rows (the
+// ghost layer is computed twice), adding only into their own rows.  Same
+// element matrices and the same per-row addition order as orc_assemble_scalar.
+int orc_assemble_scalar_mt(int dim, int order, int qorder, int ne, const double* coords, const int* elems,
+                           const int* elem2dof, double diff, const double* kdiff, double mass,
+                           const double* kmass, long nrows, const int* ia, const int* ja, double* val,
+                           int nthreads) {
+    Tab T;
+    if (!tabulate(dim, order, qorder, T)) return -1;
+    const int n = T.n, Q = T.Q;
+    if (nthreads < 1) nthreads = 1;
+    int bad = 0;
+#pragma omp parallel num_threads(nthreads) reduction(| : bad)
+    {
+        const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+        // rows split by equal nonzero counts
+        const long nnz = ia[nrows];
+        long r0 = 0, r1 = 0;
+        {
+            const long lo = nnz * t / nt, hi = nnz * (t + 1) / nt;
+            r0 = std::lower_bound(ia, ia + nrows, (int)lo) - ia;
+            r1 = t == nt - 1 ? nrows : std::lower_bound(ia, ia + nrows, (int)hi) - ia;
+        }
+        std::vector<double> gphi(n * Q * dim), prod(Q), Ae(n * n);
+        double X[12];
+        for (long e = 0; e < ne && !bad; ++e) {
+            const int* l2g = elem2dof + (long)e * n;
+            bool mine = false;
+            for (int i = 0; i < n; ++i) mine |= l2g[i] >= r0 && l2g[i] < r1;
+            if (!mine) continue;
+            elem_coords(dim, coords, elems + (long)(dim + 1) * e, X);
+            scalar_element(T, dim, X, diff * (kdiff ? kdiff[e] : 1.0), mass * (kmass ? kmass[e] : 1.0), gphi, prod,
+                           Ae);
+            for (int i = 0; i < n; ++i) {
+                if (l2g[i] < r0 || l2g[i] >= r1) continue;
+                for (int j = 0; j < n; ++j)
+                    if (!csr_add(ia, ja, val, l2g[i], l2g[j], Ae[i * n + j])) bad = 1;
+            }
+        }
+    }
+    return bad ? -2 : 0;
 }
 
 // Isotropic linear elasticity a = integrate(elements, inner(sigma(u), grad(v)))
@@ -876,6 +935,78 @@ int orc_pcg(long n, const int* ia, const int* ja, const double* val, const doubl
         beta = 0;
         for (long k = 0; k < n; ++k) beta += z[k] * r[k];
         if (i + 1 >= maxit) reason = -3;
+    }
+    if (!reason && its >= maxit) reason = -3;
+    if (its_out) *its_out = its;
+    if (resid_out) *resid_out = dp;
+    if (reason_out) *reason_out = reason;
+    return 0;
+}
+
+// orc_pcg with its loops threaded over rows (OpenMP, nthreads workers): a CPU
+// baseline standing in for PETSc's MPI KSPCG, not a parity path (the sums are
+// reassociated, so iterates differ from orc_pcg in the last bits).
+int orc_pcg_mt(long n, const int* ia, const int* ja, const double* val, const double* b, double* x,
+               double rtol, double atol, double dtol, int maxit, int nthreads, int* its_out, double* resid_out,
+               int* reason_out) {
+    std::vector<double> r(n), z(n), p(n), w(n), dinv(n);
+    if (nthreads < 1) nthreads = 1;
+    double dp = 0, beta = 0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : dp, beta)
+    for (long i = 0; i < n; ++i) {
+        double d = 0, s = 0;
+        for (int q = ia[i]; q < ia[i + 1]; ++q) {
+            if (ja[q] == i) d = val[q];
+            s += val[q] * x[ja[q]];
+        }
+        dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+        r[i] = b[i] - s;
+        z[i] = dinv[i] * r[i];
+        dp += z[i] * z[i];
+        beta += z[i] * r[i];
+    }
+    dp = std::sqrt(dp);
+    const double ttol = std::max(rtol * dp, atol), rnorm0 = dp;
+    int reason = 0, its = 0;
+    if (dp <= ttol) reason = dp < atol ? 3 : 2;
+    double betaold = 0, dpi = 0, dpiold = 0;
+    for (int it = 0; !reason && it < maxit; ++it) {
+        its = it + 1;
+        if (beta == 0.0) { reason = 3; break; }
+        const double bb = it == 0 ? 0.0 : beta / betaold;
+        double pw = 0;
+#pragma omp parallel num_threads(nthreads)
+        {
+#pragma omp for schedule(static)
+            for (long k = 0; k < n; ++k) p[k] = it == 0 ? z[k] : z[k] + bb * p[k];
+#pragma omp for schedule(static) reduction(+ : pw)
+            for (long k = 0; k < n; ++k) {
+                double s = 0;
+                for (int q = ia[k]; q < ia[k + 1]; ++q) s += val[q] * p[ja[q]];
+                w[k] = s;
+                pw += p[k] * s;
+            }
+        }
+        dpiold = dpi;
+        dpi = pw;
+        if (dpi == 0.0 || (it > 0 && dpi * dpiold <= 0.0)) { reason = -8; break; }
+        const double a = beta / dpi;
+        double zz = 0, zr = 0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : zz, zr)
+        for (long k = 0; k < n; ++k) {
+            x[k] += a * p[k];
+            r[k] -= a * w[k];
+            z[k] = dinv[k] * r[k];
+            zz += z[k] * z[k];
+            zr += z[k] * r[k];
+        }
+        dp = std::sqrt(zz);
+        if (dp != dp) { reason = -9; break; }
+        if (dp <= ttol) { reason = dp < atol ? 3 : 2; break; }
+        if (dp >= dtol * rnorm0) { reason = -4; break; }
+        betaold = beta;
+        beta = zr;
+        if (it + 1 >= maxit) reason = -3;
     }
     if (!reason && its >= maxit) reason = -3;
     if (its_out) *its_out = its;

@@ -182,3 +182,21 @@ def test_pcg_status_semantics(orc):
     x, its, res, reason = orc.pcg(ia2, ja2, np.array([1.0, 2.0, 2.0, 1.0]), np.array([1.0, 0.0]),
                                   rtol=1e-12, maxit=50)
     assert reason == -8 and its == 2
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_baseline_matches(orc, threads):
+    # the threaded CPU baseline assembles the same values (row ownership keeps
+    # the per-row addition order) and solves to the same solution
+    coords, elems = orc.mesh_kuhn(5, jitter=0.15, seed=11)
+    e2d, ndof = orc.doftable(3, 2, 1, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    k = orc.lognormal(len(elems), 0.0, 1.0, 5)
+    v1 = orc.assemble_scalar(3, 2, 4, coords, elems, e2d, ia, ja, diff=1.0, kdiff=k, mass=0.5)
+    vt = orc.assemble_scalar_mt(3, 2, 4, coords, elems, e2d, ia, ja, diff=1.0, kdiff=k, mass=0.5, threads=threads)
+    assert np.array_equal(v1, vt)
+    b = orc.assemble_rhs(3, 2, 1, 4, coords, elems, e2d, ndof, orc.FK_CONST, [1.0])
+    x1, its1, _, r1 = orc.pcg(ia, ja, v1, b, rtol=1e-12, maxit=5000)
+    xt, itst, _, rt = orc.pcg_mt(ia, ja, vt, b, rtol=1e-12, maxit=5000, threads=threads)
+    assert r1 > 0 and rt > 0 and abs(its1 - itst) <= 2
+    assert np.linalg.norm(x1 - xt) <= 1e-10 * np.linalg.norm(x1)
