@@ -5,6 +5,7 @@ The native library must be built (python -c "import __graft_entry__ as g; g.buil
 nothing here falls back to a CPU implementation: if the library or a GPU is
 missing, the calls raise.
 """
+import atexit
 import ctypes as C
 import os
 
@@ -47,7 +48,7 @@ class SolveOpts(C.Structure):
 
 class SolveData(C.Structure):
     _fields_ = [("converged", C.c_int), ("its", C.c_int), ("residual", C.c_double), ("reason", C.c_int),
-                ("outer_its", C.c_int), ("rnorm0", C.c_double)]
+                ("outer_its", C.c_int), ("rnorm0", C.c_double), ("fp64_its", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,6 +81,7 @@ _SIGS = {
     "feel_lagrange_nodes": (C.c_int, [C.c_int, C.c_int, _vp]),
     "feel_quadrature": (C.c_int, [C.c_int, C.c_int, _vp, _vp]),
     "feel_reference_tensors": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "feel_cg_condition_estimate": (C.c_int, [_vp, C.c_int, _vp]),
     # device
     "feel_hip_last_error": (C.c_char_p, []),
     "feel_hip_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -253,7 +255,33 @@ def reference_tensors(dim, order, qorder):
     return R, M, m
 
 
+def cg_condition_estimate(gamma, pap):
+    """Condition number of D^{-1} A from the coefficients of a CG run:
+    gamma_j = (r_j, z_j) for j = 0..k and pAp_j = (p_j, A p_j) for j < k."""
+    gamma, pap = np.asarray(gamma, dtype=np.float64), np.asarray(pap, dtype=np.float64)
+    k = len(pap)
+    h = np.ones(2 * (k + 1))
+    h[0::2][:len(gamma)] = gamma[:k + 1]
+    h[1:2 * k:2] = pap
+    out = C.c_double()
+    _check(lib().feel_cg_condition_estimate(h.ctypes.data_as(_vp), k, C.byref(out)), host=True)
+    return out.value
+
+
 # ============================================================ device side
+_exiting = False
+
+
+def _mark_exiting():
+    global _exiting
+    _exiting = True
+
+
+# At interpreter exit, objects caught in reference cycles are finalised in no
+# particular order; the process is ending, so handles are left to it.
+atexit.register(_mark_exiting)
+
+
 class _Handle:
     _destroy = None
 
@@ -267,6 +295,8 @@ class _Handle:
             self.h = None
 
     def __del__(self):
+        if _exiting:
+            return
         try:
             self.close()
         except Exception:
@@ -414,7 +444,7 @@ class Matrix(_Handle):
         _check(lib().feel_hip_mat_zero_rows(self.h, rows.h, values.h if values else None, rhs.h if rhs else None,
                                             mode, diag))
 
-    def solve(self, b, x, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, precision=FP64, inner_rtol=1e-4,
+    def solve(self, b, x, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, precision=FP64, inner_rtol=0.1,
               inner_maxit=100000, max_outer=50, check_every=0):
         o = SolveOpts(rtol, atol, dtol, maxit, precision, inner_rtol, inner_maxit, max_outer, check_every)
         out = SolveData()

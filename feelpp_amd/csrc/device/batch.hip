@@ -702,18 +702,19 @@ void feel_hip_batch_opts_default(feel_hip_batch_opts* o) {
 
 int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out) {
     if (!ctx || !out) return setError(FEEL_HIP_EINVAL, "null argument");
+    feel_hip_batch_opts o;
+    if (opts) o = *opts;
+    else feel_hip_batch_opts_default(&o);
+    if (o.precision != FEEL_HIP_FP32 && o.precision != FEEL_HIP_FP64)
+        return setError(FEEL_HIP_EINVAL, "batch precision must be FP32 (refined) or FP64");
     auto* B = new feel_hip_batch();
     B->ctx = ctx;
-    if (opts) B->opts = *opts;
-    else feel_hip_batch_opts_default(&B->opts);
-    if (B->opts.precision != FEEL_HIP_FP32 && B->opts.precision != FEEL_HIP_FP64) {
-        delete B;
-        return setError(FEEL_HIP_EINVAL, "batch precision must be FP32 (refined) or FP64");
-    }
+    B->opts = o;
+    retain(ctx);
     int rc = alloc(ctx, &B->gcount, 1);
     if (rc == FEEL_HIP_OK && hipMemsetAsync(B->gcount, 0, sizeof(int), ctx->stream) != hipSuccess)
         rc = setError(FEEL_HIP_EDEVICE, "hipMemset failed");
-    if (rc != FEEL_HIP_OK) { delete B; return rc; }
+    if (rc != FEEL_HIP_OK) { feel_hip_batch_destroy(B); return rc; }
     *out = B;
     return FEEL_HIP_OK;
 }
@@ -732,6 +733,7 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
         for (double* p : f.terms) c->release(p, sizeof(double) * f.nslots);
         c->release(f.mask, f.nslots);
         c->release(f.b64, sizeof(double) * f.n);
+        unref(f.g);
     }
     c->release(B->states, sizeof(CgState) * B->nStates);
     c->release(B->slots, sizeof(double) * 4 * B->nStates);
@@ -740,6 +742,7 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
     c->release(B->dSums, sizeof(double) * B->capInsts);
     c->release(B->gcount, sizeof(int));
     delete B;
+    unref(c);
     return FEEL_HIP_OK;
 }
 
@@ -785,6 +788,7 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
     FEEL_HIP_CHECK(hipGetLastError());
     FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
     if (zeroFlags) c->release(zeroFlags, f.n);
+    retain(g);   // the family reads the graph's SELL image while the batch lives
     B->fams.push_back(std::move(f));
     *family = (int)B->fams.size() - 1;
     return FEEL_HIP_OK;

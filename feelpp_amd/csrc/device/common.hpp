@@ -27,18 +27,36 @@ struct CgState {
     double ttol, rnorm0, dp, beta, dpi_prev;
     int maxit, its, done, reason;
     int iter;       // kernel-side iteration counter (number of p-updates launched and not skipped)
-    int pad[3];
+    int fresh;      // iteration at which the search direction (re)starts as p = z
+    int pad[2];
 };
 
 constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying vector kernels
 constexpr int kBlock = 256;
 constexpr int kSpmvGrid = 2048;       // grid of the solver SpMV (one p.Ap partial per block)
 constexpr int kSpmvBlock = 1024;
+constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
 
 }  // namespace feelpp_amd
 
+namespace feelpp_amd {
+void retain(feel_hip_ctx* c);
+void retain(feel_hip_space* s);
+void retain(feel_hip_graph* g);
+void unref(feel_hip_ctx* c);
+void unref(feel_hip_space* s);
+void unref(feel_hip_graph* g);
+}  // namespace feelpp_amd
+
+// Lifetime: a context, space or graph is reference counted.  Its creator holds
+// one reference and every object built on it (space -> ctx, graph -> space,
+// matrix -> graph, vector/rows/batch -> ctx, batch family -> graph) holds
+// another, so the *_destroy calls may come in any order; the memory goes when
+// the last reference does.  Counts are not atomic: one host thread per context.
 struct feel_hip_ctx {
+    int refs = 1;
     int device = 0;
+    int numCUs = 0;
     hipStream_t stream = nullptr;
     int64_t bytes = 0;
     // timing
@@ -51,6 +69,7 @@ struct feel_hip_ctx {
     double* part = nullptr;   // 3 x kReduceBlocks + kSpmvGrid doubles
     feelpp_amd::CgState* state = nullptr;
     feelpp_amd::CgState* hstate = nullptr;   // pinned host mirror
+    double* hist = nullptr;   // 2 x kHistMax CG coefficients (beta, p.Ap) of the running solve
 
     int alloc(void** p, size_t bytes);
     void release(void* p, size_t bytes);
@@ -60,6 +79,7 @@ struct feel_hip_ctx {
 };
 
 struct feel_hip_space {
+    int refs = 1;
     feel_hip_ctx* ctx = nullptr;
     int dim = 0, order = 0, ncomp = 1, nloc = 0, nldof = 0;
     int64_t nv = 0, ne = 0, ndof = 0;
@@ -83,6 +103,7 @@ struct feel_hip_space {
 };
 
 struct feel_hip_graph {
+    int refs = 1;
     feel_hip_space* space = nullptr;
     int64_t n = 0, nnz = 0;
     int32_t* ia = nullptr;

@@ -117,6 +117,72 @@ void feel_hip_ctx::collectTimers() {
     evUsed = 0;
 }
 
+namespace feelpp_amd {
+
+void retain(feel_hip_ctx* c) { ++c->refs; }
+void retain(feel_hip_space* s) { ++s->refs; }
+void retain(feel_hip_graph* g) { ++g->refs; }
+
+void unref(feel_hip_ctx* c) {
+    if (--c->refs > 0) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
+    c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
+    c->release(c->hist, sizeof(double) * 2 * kHistMax);
+    if (c->hstate) (void)hipHostFree(c->hstate);
+    for (auto& ev : c->events) {
+        (void)hipEventDestroy(ev.a);
+        (void)hipEventDestroy(ev.b);
+    }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void unref(feel_hip_space* s) {
+    if (--s->refs > 0) return;
+    feel_hip_ctx* c = s->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    c->release(s->coords, sizeof(double) * s->nv * s->dim);
+    c->release(s->elems, sizeof(int32_t) * s->ne * (s->dim + 1));
+    c->release(s->e2d, sizeof(int32_t) * s->ne * s->nldof);
+    c->release(s->rptr, sizeof(uint32_t) * (s->ndof + 1));
+    c->release(s->rentry, sizeof(uint32_t) * s->ne * s->nldof);
+    c->release(s->elemvec, sizeof(double) * s->ne * s->nldof);
+    c->release(s->geo, s->geoBytes);
+    for (auto& kv : s->rtab) c->release(kv.second, sizeof(double) * s->hrtab[kv.first].size());
+    for (auto& kv : s->qtab)
+        c->release(kv.second, sizeof(double) * (size_t)s->qnpts[kv.first] * (1 + s->nloc + s->dim + 1));
+    delete s;
+    unref(c);
+}
+
+void unref(feel_hip_graph* g) {
+    if (--g->refs > 0) return;
+    feel_hip_ctx* c = g->space->ctx;
+    (void)hipStreamSynchronize(c->stream);
+    c->release(g->ia, sizeof(int32_t) * (g->n + 1));
+    c->release(g->ja, sizeof(int32_t) * g->nnz);
+    c->release(g->diagpos, sizeof(int32_t) * g->n);
+    c->release(g->cptr, sizeof(uint32_t) * (g->nnz + 1));
+    c->release(g->centry, sizeof(uint32_t) * g->ncontrib);
+    c->release(g->slicePtr, sizeof(uint32_t) * (g->nslices + 1));
+    c->release(g->scol, sizeof(int32_t) * g->nslots);
+    c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
+    c->release(g->perm, sizeof(int32_t) * g->n);
+    c->release(g->sdiag, sizeof(int32_t) * g->n);
+    if (g->compact) {
+        c->release(g->colBase, sizeof(int32_t) * g->nslices);
+        c->release(g->ccptr, sizeof(int64_t) * g->nslices);
+        c->release(g->packed, sizeof(uint16_t) * g->npacked);
+    }
+    feel_hip_space* sp = g->space;
+    delete g;
+    unref(sp);
+}
+
+}  // namespace feelpp_amd
+
 extern "C" {
 
 const char* feel_hip_last_error(void) { return gLastError.c_str(); }
@@ -130,6 +196,9 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
     FEEL_HIP_CHECK(hipSetDevice(device));
     auto* c = new feel_hip_ctx();
     c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->numCUs = prop.multiProcessorCount;
+    if (c->numCUs < 1) c->numCUs = 1;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -137,6 +206,7 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
     }
     int rc = c->alloc((void**)&c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->state, sizeof(CgState) + 4 * sizeof(double));
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->hist, sizeof(double) * 2 * kHistMax);
     if (rc == FEEL_HIP_OK && hipHostMalloc((void**)&c->hstate, sizeof(CgState), hipHostMallocDefault) != hipSuccess)
         rc = setError(FEEL_HIP_ENOMEM, "hipHostMalloc failed");
     if (rc != FEEL_HIP_OK) {
@@ -148,18 +218,7 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
 }
 
 int feel_hip_ctx_destroy(feel_hip_ctx* c) {
-    if (!c) return FEEL_HIP_OK;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
-    c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
-    if (c->hstate) (void)hipHostFree(c->hstate);
-    for (auto& ev : c->events) {
-        (void)hipEventDestroy(ev.a);
-        (void)hipEventDestroy(ev.b);
-    }
-    if (c->stream) (void)hipStreamDestroy(c->stream);
-    delete c;
+    if (c) unref(c);
     return FEEL_HIP_OK;
 }
 
@@ -218,6 +277,7 @@ int feel_hip_space_create(feel_hip_ctx* ctx, int dim, int order, int ncomp, int6
     FEEL_HIP_CHECK(hipSetDevice(ctx->device));
     auto* s = new feel_hip_space();
     s->ctx = ctx;
+    retain(ctx);
     s->dim = dim; s->order = order; s->ncomp = ncomp; s->nloc = nloc; s->nldof = nldof;
     s->nv = nverts; s->ne = nelems; s->ndof = ndof;
     s->hElem2dof.assign(elem2dof, elem2dof + nelems * nldof);
@@ -245,20 +305,7 @@ int feel_hip_space_create(feel_hip_ctx* ctx, int dim, int order, int ncomp, int6
 }
 
 int feel_hip_space_destroy(feel_hip_space* s) {
-    if (!s) return FEEL_HIP_OK;
-    feel_hip_ctx* c = s->ctx;
-    (void)hipStreamSynchronize(c->stream);
-    c->release(s->coords, sizeof(double) * s->nv * s->dim);
-    c->release(s->elems, sizeof(int32_t) * s->ne * (s->dim + 1));
-    c->release(s->e2d, sizeof(int32_t) * s->ne * s->nldof);
-    c->release(s->rptr, sizeof(uint32_t) * (s->ndof + 1));
-    c->release(s->rentry, sizeof(uint32_t) * s->ne * s->nldof);
-    c->release(s->elemvec, sizeof(double) * s->ne * s->nldof);
-    c->release(s->geo, s->geoBytes);
-    for (auto& kv : s->rtab) c->release(kv.second, sizeof(double) * s->hrtab[kv.first].size());
-    for (auto& kv : s->qtab)
-        c->release(kv.second, sizeof(double) * (size_t)s->qnpts[kv.first] * (1 + s->nloc + s->dim + 1));
-    delete s;
+    if (s) unref(s);
     return FEEL_HIP_OK;
 }
 
@@ -378,6 +425,7 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
     FEEL_HIP_CHECK(hipSetDevice(ctx->device));
     auto* g = new feel_hip_graph();
     g->space = s;
+    retain(s);
     g->n = nrows;
     g->nnz = ia[nrows];
     g->hia.assign(ia, ia + nrows + 1);
@@ -440,25 +488,7 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
 }
 
 int feel_hip_graph_destroy(feel_hip_graph* g) {
-    if (!g) return FEEL_HIP_OK;
-    feel_hip_ctx* c = g->space->ctx;
-    (void)hipStreamSynchronize(c->stream);
-    c->release(g->ia, sizeof(int32_t) * (g->n + 1));
-    c->release(g->ja, sizeof(int32_t) * g->nnz);
-    c->release(g->diagpos, sizeof(int32_t) * g->n);
-    c->release(g->cptr, sizeof(uint32_t) * (g->nnz + 1));
-    c->release(g->centry, sizeof(uint32_t) * g->ncontrib);
-    c->release(g->slicePtr, sizeof(uint32_t) * (g->nslices + 1));
-    c->release(g->scol, sizeof(int32_t) * g->nslots);
-    c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
-    c->release(g->perm, sizeof(int32_t) * g->n);
-    c->release(g->sdiag, sizeof(int32_t) * g->n);
-    if (g->compact) {
-        c->release(g->colBase, sizeof(int32_t) * g->nslices);
-        c->release(g->ccptr, sizeof(int64_t) * g->nslices);
-        c->release(g->packed, sizeof(uint16_t) * g->npacked);
-    }
-    delete g;
+    if (g) unref(g);
     return FEEL_HIP_OK;
 }
 
@@ -472,6 +502,7 @@ int feel_hip_mat_create(feel_hip_graph* g, int precision, feel_hip_mat** out) {
     feel_hip_ctx* c = g->space->ctx;
     auto* A = new feel_hip_mat();
     A->g = g;
+    retain(g);
     A->precision = precision;
     int rc = FEEL_HIP_OK;
     if (precision & FEEL_HIP_FP64) rc = devAlloc(c, &A->v64, (size_t)g->nnz);
@@ -501,7 +532,9 @@ int feel_hip_mat_destroy(feel_hip_mat* A) {
     for (double* v : {A->bp64, A->xp64}) c->release(v, sizeof(double) * n);
     c->release(A->s64, sizeof(double) * A->g->nslots);
     c->release(A->s32, sizeof(float) * A->g->nslots);
+    feel_hip_graph* g = A->g;
     delete A;
+    unref(g);
     return FEEL_HIP_OK;
 }
 
@@ -637,6 +670,7 @@ int feel_hip_vec_create(feel_hip_ctx* c, int64_t n, feel_hip_vec** out) {
     v->n = n;
     int rc = devAlloc(c, &v->d, (size_t)n);
     if (rc != FEEL_HIP_OK) { delete v; return rc; }
+    retain(c);
     if (n) FEEL_HIP_CHECK(hipMemsetAsync(v->d, 0, sizeof(double) * n, c->stream));
     FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
     *out = v;
@@ -647,7 +681,9 @@ int feel_hip_vec_destroy(feel_hip_vec* v) {
     if (!v) return FEEL_HIP_OK;
     (void)hipStreamSynchronize(v->ctx->stream);
     v->ctx->release(v->d, sizeof(double) * v->n);
+    feel_hip_ctx* c = v->ctx;
     delete v;
+    unref(c);
     return FEEL_HIP_OK;
 }
 
@@ -822,6 +858,7 @@ int feel_hip_rows_create(feel_hip_ctx* c, int64_t n, int64_t nrows, const int32_
     }
     auto* r = new feel_hip_rows();
     r->ctx = c;
+    retain(c);
     r->n = n;
     r->nrows = nrows;
     int rc = upload(c, &r->rows, rows, (size_t)nrows);
@@ -836,7 +873,9 @@ int feel_hip_rows_destroy(feel_hip_rows* r) {
     (void)hipStreamSynchronize(r->ctx->stream);
     r->ctx->release(r->rows, sizeof(int32_t) * r->nrows);
     r->ctx->release(r->flag, r->n);
+    feel_hip_ctx* c = r->ctx;
     delete r;
+    unref(c);
     return FEEL_HIP_OK;
 }
 

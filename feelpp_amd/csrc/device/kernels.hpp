@@ -381,12 +381,15 @@ __global__ __launch_bounds__(kBlock) void kSpmv(int64_t n, const int32_t* __rest
 }
 
 // p = z + (beta/beta_prev) p, preceded by the convergence test on the
-// preconditioned residual norm (KSPConvergedDefault with UIRNorm).
+// preconditioned residual norm (KSPConvergedDefault with UIRNorm).  With
+// hist, block 0 records beta = (r, z) of iteration it at hist[2 it] (the
+// update kernel records p.Ap at hist[2 it + 1]): the Lanczos coefficients of
+// the run, from which the host estimates the condition number.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const T* __restrict__ z,
                                                        T* __restrict__ p, const double* __restrict__ zrPart,
                                                        const double* __restrict__ zzPart, CgState* st,
-                                                       double* betaSlots) {
+                                                       double* betaSlots, double* hist) {
     if (blockDone(st)) return;
     const double beta = reducePartials(zrPart);
     const double dp = sqrt(reducePartials(zzPart));
@@ -403,26 +406,33 @@ __global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const 
         if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
         if (reason) { st->done = 1; st->reason = reason; st->its = it; }
         betaSlots[it & 1] = beta;
+        if (hist && it < kHistMax) hist[2 * it] = beta;
     }
     if (reason) return;
-    const double bb = it == 0 ? 0.0 : beta / betaSlots[(it - 1) & 1];
+    // the direction (re)starts as p = z at it == st->fresh (0 unless a
+    // mixed-precision solve restarted it)
+    const bool first = it == st->fresh;
+    const double bb = first ? 0.0 : beta / betaSlots[(it - 1) & 1];
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = it == 0 ? z[i] : (T)((double)z[i] + bb * (double)p[i]);
+        p[i] = first ? z[i] : (T)((double)z[i] + bb * (double)p[i]);
 }
 
-// x += a p, r -= a w, z = D^{-1} r; partials of z.r and z.z.
+// x += a p, r -= a w, z = D^{-1} r; partials of z.r and z.z.  dpiPart holds
+// the npart block partials of p.w left by the SpMV.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __restrict__ x, T* __restrict__ r,
                                                     T* __restrict__ z, const T* __restrict__ p,
                                                     const T* __restrict__ w, const T* __restrict__ dinv,
-                                                    const double* __restrict__ dpiPart, double* __restrict__ zrPart,
-                                                    double* __restrict__ zzPart, CgState* st,
-                                                    const double* betaSlots, double* dpiSlots) {
+                                                    const double* __restrict__ dpiPart, int npart,
+                                                    double* __restrict__ zrPart, double* __restrict__ zzPart,
+                                                    CgState* st, const double* betaSlots, double* dpiSlots,
+                                                    double* hist) {
     if (blockDone(st)) return;
-    const double dpi = reducePartials(dpiPart, kSpmvGrid);
+    const double dpi = reducePartials(dpiPart, npart);
     const bool indefinite = dpi == 0.0 || (it > 0 && dpi * dpiSlots[(it - 1) & 1] <= 0.0);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         dpiSlots[it & 1] = dpi;
+        if (hist && it < kHistMax) hist[2 * it + 1] = dpi;
         if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
     }
     if (indefinite) return;
@@ -553,62 +563,79 @@ __device__ inline double blockReduceSumN(double v) {
     return r;
 }
 
+// Column stream of the SELL-C-sigma image (shared by the SpMV kernels).
+// COMPACT: columns come from the packed stream, where a slice spanning fewer
+// than 2^16 columns stores 16-bit offsets from colBase[s] (2 bytes per slot)
+// and any other slice (colBase[s] = -1) stores int32 columns.
+struct SellCols {
+    const uint32_t* __restrict__ slicePtr;
+    const int32_t* __restrict__ col;
+    const int32_t* __restrict__ colBase;
+    const int64_t* __restrict__ cptr;
+    const uint16_t* __restrict__ packed;
+};
+
+// Row (s*64 + lane) of A x for slice s, executed by one wavefront: coalesced
+// column-major value/column streams read with non-temporal loads (streamed
+// once; keeps L2 for the x gathers), U independent partial sums per lane for
+// memory-level parallelism.
+template <typename T, int U, bool COMPACT>
+__device__ inline T sellRow(const SellCols& S, int64_t s, int lane, const T* __restrict__ val,
+                            const T* __restrict__ x) {
+    const uint32_t base = S.slicePtr[s];
+    const int width = (int)((S.slicePtr[s + 1] - base) >> 6);
+    const T* v = val + base + lane;
+    T acc[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc[u] = 0;
+    // one wave per slice: the narrow/wide choice is uniform across the wave
+    const int32_t cb = COMPACT ? S.colBase[s] : -1;
+    if (cb >= 0) {
+        const uint16_t* c = S.packed + S.cptr[s] + lane;
+        const T* xs = x + cb;
+        int k = 0;
+        for (; k + U <= width; k += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
+                const uint16_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
+                acc[u] += vv * xs[cc];
+            }
+        }
+        for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * xs[c[(size_t)k * 64]];
+    } else {
+        const int32_t* c = COMPACT ? reinterpret_cast<const int32_t*>(S.packed + S.cptr[s]) + lane : S.col + base + lane;
+        int k = 0;
+        for (; k + U <= width; k += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
+                const int32_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
+                acc[u] += vv * x[cc];
+            }
+        }
+        for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
+    }
+    T a = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) a += acc[u];
+    return a;
+}
+
 // y = A x on the SELL-C-sigma image: one wavefront per 64-row slice, one lane
-// per row, coalesced column-major value/column streams read with non-temporal
-// loads (streamed once; keeps L2 for the x gathers), U independent partial
-// sums per lane for memory-level parallelism.  With part, also the block
-// partial of x.y (p.Ap).  Grid-stride over slices with a fixed grid.
-// COMPACT: columns stored as 16-bit offsets from a per-slice base (colBase),
-// 2 bytes per slot instead of 4.
+// per row, grid-stride over slices with a fixed grid.  With part, also the
+// block partial of x.y (p.Ap).
 template <typename T, int BS, int U, bool COMPACT>
-__global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
-                                                const int32_t* __restrict__ col, const int32_t* __restrict__ colBase,
-                                                const int64_t* __restrict__ cptr, const uint16_t* __restrict__ packed,
-                                                const T* __restrict__ val, const T* __restrict__ x, T* __restrict__ y,
-                                                double* __restrict__ part, const CgState* __restrict__ st) {
+__global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, SellCols S, const T* __restrict__ val,
+                                                const T* __restrict__ x, T* __restrict__ y, double* __restrict__ part,
+                                                const CgState* __restrict__ st) {
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * BS) >> 6;
     double dot = 0.0;
     for (int64_t s = wave; s < nslices; s += nwaves) {
-        const uint32_t base = slicePtr[s];
-        const int width = (int)((slicePtr[s + 1] - base) >> 6);
-        const T* v = val + base + lane;
-        T acc[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) acc[u] = 0;
-        // one wave per slice: the narrow/wide choice is uniform across the wave
-        const int32_t cb = COMPACT ? colBase[s] : -1;
-        if (cb >= 0) {
-            const uint16_t* c = packed + cptr[s] + lane;
-            const T* xs = x + cb;
-            int k = 0;
-            for (; k + U <= width; k += U) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
-                    const uint16_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
-                    acc[u] += vv * xs[cc];
-                }
-            }
-            for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * xs[c[(size_t)k * 64]];
-        } else {
-            const int32_t* c = COMPACT ? reinterpret_cast<const int32_t*>(packed + cptr[s]) + lane : col + base + lane;
-            int k = 0;
-            for (; k + U <= width; k += U) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
-                    const int32_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
-                    acc[u] += vv * x[cc];
-                }
-            }
-            for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
-        }
-        T a = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) a += acc[u];
+        const T a = sellRow<T, U, COMPACT>(S, s, lane, val, x);
         const int64_t r = s * 64 + lane;
         if (r < nrows) {
             y[r] = a;
@@ -619,6 +646,27 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
         const double t = blockReduceSumN<BS>(dot);
         if (threadIdx.x == 0) part[blockIdx.x] = t;
     }
+}
+
+// Reliable update, device half: r32 = r64 / sigma, z32 = D^{-1} r32, xs = 0,
+// partials of r.z and z.z for the next direction kernel.
+static __global__ __launch_bounds__(kBlock) void kCgStepReset(int64_t n, const double* __restrict__ r64, double invSigma,
+                                                       const float* __restrict__ dinv, float* __restrict__ r,
+                                                       float* __restrict__ z, float* __restrict__ xs,
+                                                       double* __restrict__ gzPart, double* __restrict__ zzPart) {
+    double zr = 0.0, zz = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float ri = (float)(r64[i] * invSigma);
+        r[i] = ri;
+        const float zi = dinv[i] * ri;
+        z[i] = zi;
+        xs[i] = 0.0f;
+        zr += (double)zi * (double)ri;
+        zz += (double)zi * (double)zi;
+    }
+    zr = blockReduceSum(zr);
+    zz = blockReduceSum(zz);
+    if (threadIdx.x == 0) { gzPart[blockIdx.x] = zr; zzPart[blockIdx.x] = zz; }
 }
 
 template <typename T>

@@ -10,6 +10,7 @@
 
 #include "common.hpp"
 #include "kernels.hpp"
+#include "../host/krylov.hpp"
 
 using namespace feelpp_amd;
 
@@ -21,16 +22,20 @@ namespace {
         if (rc_ != FEEL_HIP_OK) return rc_;  \
     } while (0)
 
+SellCols sellCols(const feel_hip_graph* g) { return SellCols{g->slicePtr, g->scol, g->colBase, g->ccptr, g->packed}; }
+
 // y = A x in the solver numbering (SELL image), optional p.Ap partials
+// Returns the grid used (the number of p.Ap partials written when part is set).
 template <typename T>
-void spmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st) {
+int spmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st) {
     constexpr int U = sizeof(T) == 8 ? 4 : 8;
     if (g->compact)
         hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U, true>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
-                           g->nslices, g->slicePtr, g->scol, g->colBase, g->ccptr, g->packed, val, x, y, part, st);
+                           g->nslices, sellCols(g), val, x, y, part, st);
     else
         hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U, false>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
-                           g->nslices, g->slicePtr, g->scol, g->colBase, g->ccptr, g->packed, val, x, y, part, st);
+                           g->nslices, sellCols(g), val, x, y, part, st);
+    return kSpmvGrid;
 }
 
 int gridN(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 8192)); }
@@ -102,12 +107,12 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
         const int chunk = checkEvery > 0 ? checkEvery : std::min(64, std::max(8, it / 4));
         for (int k = 0; k < chunk; ++k, ++it) {
             hipLaunchKernelGGL(kCgDirection<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, z, p, zrPart, zzPart,
-                               st, betaSlots);
+                               st, betaSlots, (double*)nullptr);
             c->tic(timerName);
-            spmv<T>(c, g, val, p, w, dpiPart, st);
+            const int npart = spmv<T>(c, g, val, p, w, dpiPart, st);
             c->toc();
             hipLaunchKernelGGL(kCgUpdate<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, x, r, z, p, w, dinv,
-                               dpiPart, zrPart, zzPart, st, betaSlots, dpiSlots);
+                               dpiPart, npart, zrPart, zzPart, st, betaSlots, dpiSlots, (double*)nullptr);
             if (it > maxit) break;
         }
         FEEL_HIP_CHECK(hipGetLastError());
@@ -143,6 +148,168 @@ int residual64(feel_hip_mat* A, const double* b, const double* x, double* r, dou
     return FEEL_HIP_OK;
 }
 
+struct MixedRun {
+    int its = 0, reason = 0, updates = 0, fp64Its = 0;
+    double dp = 0, rnorm0 = 0;
+};
+
+// Mixed precision pays while eps32 * kappa stays small: an FP32 operator and
+// FP32 Krylov vectors perturb CG by about that much per iteration, and beyond
+// kappa ~ 5e4 the FP32 iterations outnumber the FP64 ones by more than the
+// FP32 iteration is cheaper (about 0.6x the FP64 one, HBM-bound).
+constexpr double kMixedKappaLimit = 5e4;
+constexpr int kKappaMinIts = 30;
+// the estimate is also taken at host polls during the first iterations of a
+// chain (every kKappaEvery up to kKappaPollIts), so an ill-conditioned solve
+// hands over early instead of at its first reliable update
+constexpr int kKappaEvery = 128;
+constexpr int kKappaPollIts = 1024;
+
+// Jacobi-PCG in FP32 with reliable updates in FP64 (solver numbering, on
+// A->bp64 / A->xp64).  One FP32 CG (the direction / SpMV / update kernels of
+// the FP64 solver) runs on the scaled residual r / sigma and accumulates its
+// correction in xs.  Whenever its preconditioned residual norm has dropped by
+// inner_rtol since the last reliable update, or reaches the tolerance, the
+// device stops and the host folds sigma * xs into the FP64 solution,
+// recomputes the true FP64 residual and tests it (KSPConvergedDefault on the
+// FP64 preconditioned residual), then restarts the FP32 residual from it and
+// continues with the same search direction, so the Krylov recurrence is not
+// restarted: beta = (r_new, z_new) / (r_old, z_old).
+// A reliable update whose true residual exceeds the FP32 estimate by more than
+// this factor restarts the search direction (the FP32 recurrence drifted).
+constexpr double kDriftRestart = 2.0;
+
+int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, MixedRun& out) {
+    feel_hip_graph* g = A->g;
+    feel_hip_ctx* c = g->space->ctx;
+    const int64_t n = g->n;
+    double* zrPart = c->part;
+    double* zzPart = c->part + kReduceBlocks;
+    double* dpiPart = c->part + 3 * kReduceBlocks;   // kSpmvGrid partials
+    CgState* st = c->state;
+    double* betaSlots = reinterpret_cast<double*>(st + 1);
+    double* dpiSlots = betaSlots + 2;
+    double dp = 0;
+    TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
+    const double rnorm0 = dp, ttol = std::max(o.rtol * dp, o.atol);
+    const double delta = o.inner_rtol > 0 && o.inner_rtol < 1 ? o.inner_rtol : 0.1;
+    out.rnorm0 = rnorm0;
+    out.dp = dp;
+    if (dp != dp) { out.reason = -9; return FEEL_HIP_OK; }
+    if (dp <= ttol) { out.reason = dp < o.atol ? 3 : 2; return FEEL_HIP_OK; }
+    if (o.maxit == 0) { out.reason = -3; return FEEL_HIP_OK; }
+    // the device works in units of sigma: its residual norm starts at 1 and
+    // its stopping threshold is stop / sigma
+    const double sigma = dp;
+    CgState h{};
+    h.rtol = 0.0; h.dtol = o.dtol; h.maxit = o.maxit;
+    int it = 0;
+    auto reset = [&](double dpNow, bool restart) -> int {
+        const double stop = std::max(ttol, delta * dpNow) / sigma;
+        h.atol = stop;   // threshold at iteration 0
+        h.ttol = stop;   // threshold afterwards
+        h.done = 0; h.reason = 0; h.its = it;
+        if (restart) h.fresh = it;
+        FEEL_HIP_CHECK(hipMemcpyAsync(st, &h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(kCgStepReset, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, A->r64, 1.0 / sigma,
+                           A->dinv32, A->r32, A->z32, A->x32, zrPart, zzPart);
+        FEEL_HIP_CHECK(hipGetLastError());
+        // h must stay valid until the copy has run
+        FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+        return FEEL_HIP_OK;
+    };
+    TRY_CALL(reset(dp, true));
+    int reason = 0;
+    // Launches queued past the iteration that stops the device are early
+    // exits; the chunk is sized from the observed convergence rate so that
+    // few are wasted before each reliable update.
+    double lastDp = 1.0;
+    int lastIt = it, chunk = 16, lastKappa = 0;
+    // condition estimate of the current Lanczos chain (since the last direction restart)
+    auto tooIllConditioned = [&](bool& ill) -> int {
+        const int k = std::min(it, kHistMax) - h.fresh;
+        ill = false;
+        if (k < kKappaMinIts) return FEEL_HIP_OK;
+        std::vector<double> hh(2 * (size_t)k);
+        FEEL_HIP_CHECK(hipMemcpy(hh.data(), c->hist + 2 * (size_t)h.fresh, sizeof(double) * hh.size(),
+                                 hipMemcpyDeviceToHost));
+        ill = cgConditionEstimate(hh.data(), k) > kMixedKappaLimit;
+        return FEEL_HIP_OK;
+    };
+    while (true) {
+        if (o.check_every > 0) chunk = o.check_every;
+        for (int k = 0; k < chunk && it <= o.maxit; ++k, ++it) {
+            hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->z32,
+                               A->p32, zrPart, zzPart, st, betaSlots, c->hist);
+            c->tic("spmv_fp32");
+            const int npart = spmv<float>(c, g, A->s32, A->p32, A->w32, dpiPart, st);
+            c->toc();
+            hipLaunchKernelGGL(kCgUpdate<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->x32,
+                               A->r32, A->z32, A->p32, A->w32, A->dinv32, dpiPart, npart, zrPart, zzPart, st,
+                               betaSlots, dpiSlots, c->hist);
+        }
+        FEEL_HIP_CHECK(hipGetLastError());
+        FEEL_HIP_CHECK(hipMemcpyAsync(c->hstate, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+        bool handover = false;
+        if (!c->hstate->done && it - h.fresh >= kKappaMinIts && it - h.fresh <= kKappaPollIts &&
+            it - lastKappa >= kKappaEvery && it <= o.maxit) {
+            lastKappa = it;
+            TRY_CALL(tooIllConditioned(handover));
+        }
+        if (!c->hstate->done && !handover) {
+            if (it > o.maxit + 1) return setError(FEEL_HIP_EDEVICE, "CG state machine did not terminate");
+            const double est = c->hstate->dp;
+            const double rate = it > lastIt && est > 0 && est < lastDp ? std::log(est / lastDp) / (it - lastIt) : 0.0;
+            const double left = rate < 0 ? std::log(c->hstate->ttol / est) / rate : 64.0;
+            chunk = (int)std::min(64.0, std::max(4.0, std::ceil(left)));
+            lastDp = est;
+            lastIt = it;
+            continue;
+        }
+        if (!handover) {
+            it = c->hstate->its;
+            reason = c->hstate->reason;
+        }
+        h.rnorm0 = c->hstate->rnorm0;   // set by the device at iteration 0
+        const double estimate = c->hstate->dp * sigma;
+        // fold the FP32 correction into the FP64 solution and take the true residual
+        hipLaunchKernelGGL(kAxpyF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, sigma, A->x32, A->xp64);
+        FEEL_HIP_CHECK(hipGetLastError());
+        TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
+        ++out.updates;
+        if (reason < 0) break;   // -3, -4, -8, -9 from the device tests
+        if (dp != dp) { reason = -9; break; }
+        if (dp <= ttol) { reason = dp < o.atol ? 3 : 2; break; }
+        if (dp >= o.dtol * rnorm0) { reason = -4; break; }
+        if (it >= o.maxit || out.updates >= o.max_outer + 1) { reason = -3; break; }
+        if (!handover) TRY_CALL(tooIllConditioned(handover));
+        if (handover) {
+            // finish in FP64 Jacobi-CG from the current solution, same absolute tolerance
+            TRY_CALL(allocWork(c, &A->r64, n));
+            TRY_CALL(allocWork(c, &A->z64, n));
+            TRY_CALL(allocWork(c, &A->p64, n));
+            TRY_CALL(allocWork(c, &A->w64, n));
+            CgRun run;
+            TRY_CALL(runCg<double>(A, A->s64, A->dinv64, A->bp64, A->xp64, false, 0.0, ttol, o.dtol * rnorm0 / dp,
+                                   o.maxit - it, o.check_every, A->r64, A->z64, A->p64, A->w64, "spmv_fp64", run));
+            out.fp64Its = run.its;
+            it += run.its;
+            dp = run.dp;
+            reason = run.reason > 0 ? (dp < o.atol ? 3 : 2) : run.reason;
+            break;
+        }
+        TRY_CALL(reset(dp, dp > kDriftRestart * estimate));
+        lastDp = dp / sigma;
+        lastIt = it;
+        chunk = 16;
+    }
+    out.its = it;
+    out.reason = reason;
+    out.dp = dp;
+    return FEEL_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -156,7 +323,7 @@ void feel_hip_solve_opts_default(feel_hip_solve_opts* o) {
     o->dtol = 1e5;
     o->maxit = 1000;
     o->precision = FEEL_HIP_FP64;
-    o->inner_rtol = 1e-4;
+    o->inner_rtol = 0.1;
     o->inner_maxit = 100000;
     o->max_outer = 50;
     o->check_every = 0;
@@ -196,11 +363,12 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
         out->residual = run.dp;
         out->rnorm0 = run.rnorm0;
         out->outer_its = 1;
+        out->fp64_its = run.its;
         out->converged = run.reason > 0;
         return FEEL_HIP_OK;
     }
     if (o.precision != FEEL_HIP_FP32) return setError(FEEL_HIP_EINVAL, "solve precision must be FP64 or FP32");
-    // FP32 Jacobi-CG inside FP64 iterative refinement: FP64 residual, FP64 stopping test.
+    // FP32 Jacobi-CG with FP64 reliable updates: FP64 solution, residual and stopping test.
     if (!A->v64) return setError(FEEL_HIP_EINVAL, "mixed-precision solve needs FP64 matrix values for the residual");
     TRY_CALL(allocWork(c, &A->r64, n));
     TRY_CALL(allocWork(c, &A->w64, n));
@@ -209,44 +377,23 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
     TRY_CALL(allocWork(c, &A->p32, n));
     TRY_CALL(allocWork(c, &A->w32, n));
     TRY_CALL(allocWork(c, &A->x32, n));
-    TRY_CALL(allocWork(c, &A->b32, n));
     TRY_CALL(allocWork(c, &A->bp64, n));
     TRY_CALL(allocWork(c, &A->xp64, n));
     TRY_CALL(ensureSell(A, true, true));
     hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, b->d, A->bp64);
     hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, x->d, A->xp64);
-    double dp = 0;
-    TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
-    const double rnorm0 = dp, ttol = std::max(o.rtol * dp, o.atol);
-    int total = 0, outer = 0, reason = 0;
-    while (true) {
-        if (dp != dp) { reason = -9; break; }
-        if (dp <= ttol) { reason = dp < o.atol ? 3 : 2; break; }
-        if (outer > 0 && dp >= o.dtol * rnorm0) { reason = -4; break; }
-        if (total >= o.maxit || outer >= o.max_outer) { reason = -3; break; }
-        const double s = 1.0 / dp;
-        hipLaunchKernelGGL(kScaleToF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, A->r64, s, A->b32);
-        FEEL_HIP_CHECK(hipMemsetAsync(A->x32, 0, sizeof(float) * n, c->stream));
-        CgRun run;
-        const int budget = std::min(o.inner_maxit, o.maxit - total);
-        TRY_CALL(runCg<float>(A, A->s32, A->dinv32, A->b32, A->x32, true, o.inner_rtol, 0.0, 1e5, budget, o.check_every,
-                              A->r32, A->z32, A->p32, A->w32, "spmv_fp32", run));
-        total += run.its;
-        ++outer;
-        if (run.reason == -8 || run.reason == -9) { reason = run.reason; break; }
-        hipLaunchKernelGGL(kAxpyF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, dp, A->x32, A->xp64);
-        FEEL_HIP_CHECK(hipGetLastError());
-        TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
-    }
+    MixedRun run;
+    TRY_CALL(runMixedCg(A, o, run));
     hipLaunchKernelGGL(kPermuteOut, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, A->xp64, x->d);
     FEEL_HIP_CHECK(hipGetLastError());
     c->collectTimers();
-    out->its = total;
-    out->reason = reason;
-    out->residual = dp;
-    out->rnorm0 = rnorm0;
-    out->outer_its = outer;
-    out->converged = reason > 0;
+    out->its = run.its;
+    out->reason = run.reason;
+    out->residual = run.dp;
+    out->rnorm0 = run.rnorm0;
+    out->outer_its = run.updates;
+    out->fp64_its = run.fp64Its;
+    out->converged = run.reason > 0;
     return FEEL_HIP_OK;
 }
 

@@ -7,6 +7,7 @@
 #include "feelpp_host.h"
 #include "fe.hpp"
 #include "space.hpp"
+#include "krylov.hpp"
 
 using namespace feelpp_amd;
 
@@ -180,6 +181,17 @@ int feel_reference_tensors(int dim, int order, int qorder, double* R, double* M,
     if (M) std::memcpy(M, T.M.data(), sizeof(double) * T.M.size());
     if (m) std::memcpy(m, T.m.data(), sizeof(double) * T.m.size());
     return T.n;
+    HOST_CATCH
+}
+
+int feel_cg_condition_estimate(const double* gamma_pap, int k, double* kappa) {
+    HOST_TRY
+    if (!gamma_pap || !kappa || k < 0) return (int)fail("bad argument");
+    for (int j = 0; j < k; ++j)
+        if (!(gamma_pap[2 * j] > 0) || !(gamma_pap[2 * j + 1] > 0))
+            return (int)fail("coefficients must be positive");
+    *kappa = cgConditionEstimate(gamma_pap, k);
+    return 0;
     HOST_CATCH
 }
 

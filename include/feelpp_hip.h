@@ -80,24 +80,31 @@ typedef struct {
     double atol;        /* ksp-atol */
     double dtol;        /* ksp-dtol */
     int maxit;          /* ksp-maxit */
-    int precision;      /* FEEL_HIP_FP64: plain FP64 Jacobi-CG; FEEL_HIP_FP32: FP32 Jacobi-CG inside
-                           FP64 iterative refinement (FP64 residual and stopping test) */
-    double inner_rtol;  /* FP32 inner-solve relative reduction per refinement step */
-    int inner_maxit;
-    int max_outer;
+    int precision;      /* FEEL_HIP_FP64: plain FP64 Jacobi-CG; FEEL_HIP_FP32: FP32 Jacobi-CG with
+                           FP64 reliable updates (FP64 solution, residual and stopping test) */
+    double inner_rtol;  /* FP32: reduction of the residual estimate between reliable updates (0.1) */
+    int inner_maxit;    /* unused by the single solve */
+    int max_outer;      /* FP32: cap on reliable updates */
     int check_every;    /* host polls the device convergence flag every this many iterations (0 = auto) */
 } feel_hip_solve_opts;
 
 typedef struct {
     int converged;      /* SolveData::isConverged */
-    int its;            /* total Jacobi-CG iterations (A*p products), all refinement steps included */
+    int its;            /* total Jacobi-CG iterations (A*p products) */
     double residual;    /* final ||B r|| (preconditioned norm), FP64 */
     int reason;         /* KSPConvergedReason-like: 2 rtol, 3 atol, -3 maxit, -4 dtol, -8 indefinite, -9 nan */
-    int outer_its;      /* refinement steps (1 for FP64) */
+    int outer_its;      /* FP64 residual evaluations after the first (reliable updates; 1 for FP64) */
     double rnorm0;      /* initial ||B r_0|| */
+    int fp64_its;       /* iterations run in FP64: all of them for FP64; for FP32, those after the solve
+                           found the matrix too ill-conditioned for FP32 (Lanczos estimate of kappa) */
 } feel_hip_solve_data;
 
 /* ---- lifetime ------------------------------------------------------------ */
+/* Handles are reference counted the way MatrixSparse/GraphCSR shared_ptrs are
+   (feelalg/matrixsparse.hpp:113): each object keeps what it was built on
+   (space -> context, graph -> space, matrix -> graph, vector/rows/batch ->
+   context, batch family -> graph) alive, so the *_destroy calls may come in
+   any order.  One host thread per context. */
 const char* feel_hip_last_error(void);
 int feel_hip_ctx_create(int device, feel_hip_ctx** out);
 int feel_hip_ctx_destroy(feel_hip_ctx* ctx);

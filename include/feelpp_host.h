@@ -53,6 +53,14 @@ int feel_lagrange_nodes(int dim, int order, double* nodes);
 int feel_quadrature(int dim, int order, double* pts, double* w);
 int feel_reference_tensors(int dim, int order, int qorder, double* R, double* M, double* m);
 
+/* Condition number of the Jacobi-preconditioned operator estimated from k CG
+ * iterations, given as k + 1 pairs (gamma_j = (r_j, z_j), pAp_j = (p_j, A p_j)):
+ * the extreme eigenvalues of the Lanczos tridiagonal of the run.  No Feel++
+ * counterpart in this path (PETSc offers it as KSPComputeExtremeSingularValues);
+ * the FP32 solve uses it to decide when the matrix is too ill-conditioned for
+ * FP32 and the rest of the solve must run in FP64. */
+int feel_cg_condition_estimate(const double* gamma_pap, int k, double* kappa);
+
 #ifdef __cplusplus
 }
 #endif

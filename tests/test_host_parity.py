@@ -107,3 +107,42 @@ def test_host_rejects_bad_input():
         fa.doftable(3, 2, 1, np.array([[0, 1, 2, 99]], np.int32), 4)
     with pytest.raises(fa.FeelHipError):
         fa.doftable(3, 9, 1, np.array([[0, 1, 2, 3]], np.int32), 4)
+
+
+def test_cg_condition_estimate(orc):
+    # Lanczos estimate from the coefficients of a Jacobi-CG run converges to
+    # kappa(D^{-1/2} A D^{-1/2}) computed by a dense eigensolver
+    coords, elems = orc.mesh_kuhn(3, jitter=0.2, seed=5)
+    e2d, ndof = orc.doftable(3, 2, 1, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    k = orc.lognormal(len(elems), 0.0, 1.0, 9)
+    val = orc.assemble_scalar(3, 2, 2, coords, elems, e2d, ia, ja, diff=1.0, kdiff=k, mass=0.3)
+    A = np.zeros((ndof, ndof))
+    for r in range(ndof):
+        A[r, ja[ia[r]:ia[r + 1]]] = val[ia[r]:ia[r + 1]]
+    d = np.diag(A)
+    S = A / np.sqrt(np.outer(d, d))
+    ev = np.linalg.eigvalsh(S)
+    kappa = ev[-1] / ev[0]
+    b = np.sin(np.arange(ndof) * 0.37) + 0.1
+    x, r = np.zeros(ndof), b.copy()
+    z = r / d
+    p = z.copy()
+    gam, pap = [z @ r], []
+    for _ in range(ndof):
+        w = A @ p
+        pap.append(p @ w)
+        a = gam[-1] / pap[-1]
+        x += a * p
+        r -= a * w
+        z = r / d
+        gam.append(z @ r)
+        if gam[-1] <= 1e-28 * gam[0]:
+            break
+        p = z + gam[-1] / gam[-2] * p
+    est = fa.cg_condition_estimate(gam, pap)
+    assert abs(est - kappa) <= 1e-6 * kappa
+    # few iterations: a lower bound
+    assert fa.cg_condition_estimate(gam[:6], pap[:5]) <= kappa * (1 + 1e-12)
+    with pytest.raises(RuntimeError):
+        fa.cg_condition_estimate([1.0, -1.0, 1.0], [1.0, 1.0])
