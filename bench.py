@@ -94,6 +94,17 @@ def time_form(prob, precision, rtol, steps, warmup, inner_rtol):
     return total, lat, its, timers
 
 
+def read_traffic(path):
+    """'traffic <kernel> <bytes>' lines of a committed counter summary."""
+    out = {}
+    if os.path.exists(path):
+        for ln in open(path):
+            f = ln.split()
+            if len(f) == 3 and f[0] == "traffic":
+                out[f[1]] = int(f[2])
+    return out
+
+
 def cpu_threads():
     """Host threads for the all-cores baseline: OMP_NUM_THREADS when set (the
     GPU box sets it to the job's 16-core share), else the visible CPUs."""
@@ -297,6 +308,38 @@ def bench_beam(fa, args):
     print(json.dumps(line), flush=True)
 
 
+def bench_kernels(fa, args):
+    """Short fixed runs of the hot kernels for hardware-counter passes: a plain
+    device copy of known size (calibrates the HBM byte counters), then on the
+    C2 problem 3 assemblies (scatter kernel) and 20-iteration FP64 and FP32
+    solves (SELL SpMV kernels)."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy kernel: {nbytes} bytes read + {nbytes} bytes written per launch, 5 launches")
+    del a, b
+    order, qorder = args.order, 2 * args.order
+    host = build_problem(fa, args.n, order, 0.15, 2024)
+    ctx = fa.Context(0)
+    prob = Problem(fa, ctx, host, order, qorder)
+    for _ in range(3):
+        prob.A.assemble(fa.DIFFUSION, qorder, kdiff=prob.k, precision=fa.FP64_FP32)
+    fa.assemble_linear(prob.space, prob.b, fa.SRC_CONST, order, (1.0,))
+    prob.A.zero_rows(prob.rows, None, prob.b, mode=fa.ELIMINATION_SYMMETRIC)
+    for prec in (fa.FP64, fa.FP32):
+        prob.x.zero()
+        st = prob.A.solve(prob.b, prob.x, rtol=1e-30, maxit=20, precision=prec, inner_rtol=1e-30, check_every=20)
+        log(f"precision {prec}: {st}")
+    ctx.synchronize()
+    print(json.dumps({"kernels": "copy 1 GiB x5; C2 assembly x3; FP64 and FP32 solves of 20 iterations",
+                      "nnz": prob.nnz, "dofs": prob.ndof, "elements": prob.ne}), flush=True)
+
+
 def bench_serving(fa, args):
     """Poisson-arrival stream of affine Poisson requests (C4's size/order mix) at several rates."""
     ctx = fa.Context(0)
@@ -361,7 +404,7 @@ def main():
                          "batched engine: FP32 inner reduction per refinement step (default 1e-4)")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam"])
+    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam", "kernels"])
     ap.add_argument("--orders", default="1,2,3,4")
     ap.add_argument("--sizes", default="8,12,16,24,32")
     ap.add_argument("--problems", type=int, default=512)
@@ -382,6 +425,8 @@ def main():
         if args.rtol == 1e-10:
             args.rtol = 1e-9
         return bench_batched(fa, args)
+    if args.config == "kernels":
+        return bench_kernels(fa, args)
     if args.config == "beam":
         if args.orders == "1,2,3,4":
             args.orders = "3,4"
@@ -424,6 +469,24 @@ def main():
         bytes_spmv = 8 * nnz + 12 * n + 4
     avg_ms = ms / max(nl, 1)
     achieved = bytes_spmv / (avg_ms * 1e-3) / 1e9 if nl else 0.0
+    spmv_kernel = "kSellSpmv<double,1024,4,true>" if spmv_name == "spmv_fp64" else "kSellSpmv<float,1024,8,true>"
+    # HBM bytes per launch from the committed counter passes (calibrated FETCH_SIZE + WRITE_SIZE),
+    # valid for the workload they were taken on
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_p2_kuhn64.txt"))
+    measured_workload = args.n == 64 and order == 2
+    # scatter (owner-computes gather of element contributions into CSR values, FP64 + FP32 copies):
+    # algorithmic bytes = nnz*(8+4) values + E*nloc*4 elem2dof + E*8 coefficient + E*4*4 + V*3*8 geometry
+    ng, gms = R["timers"]["assemble_gather"]
+    nv = (args.n + 1) ** 3
+    nloc = (order + 1) * (order + 2) * (order + 3) // 6
+    bytes_scatter = 12 * nnz + prob.ne * nloc * 4 + prob.ne * 8 + prob.ne * 16 + nv * 24
+    g_ms = gms / max(ng, 1)
+    g_ach = bytes_scatter / (g_ms * 1e-3) / 1e9 if ng else 0.0
+    scatter = {"kernel": "kGatherScalar<100,6,8,true>" if order == 2 else "kGatherScalar", "bound": "hbm",
+               "achieved": round(g_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(g_ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(g_ms * 1e3, 2),
+               "algorithmic_bytes": bytes_scatter,
+               "traffic": traffic.get("kGatherScalar<100,6,8,true>") if measured_workload else None}
 
     cpu = None
     if not args.no_cpu:
@@ -461,15 +524,16 @@ def main():
                   for f, v in results.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": spmv_name,
+            "kernel": spmv_kernel,
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic.get(spmv_kernel) if measured_workload else None,
             "avg_launch_us": round(avg_ms * 1e3, 2),
             "launches": nl,
             "algorithmic_bytes": bytes_spmv,
+            "scatter": scatter,
         },
         "cpu_baseline": cpu and {k: v for k, v in cpu.items()
                                  if k in ("value", "unit", "cores", "kind", "sample", "one_core")},
