@@ -473,7 +473,7 @@ class Batch(_Handle):
     """Batched / continuously batched solves of affine families A(mu) = sum_t mu_t A_t."""
     _destroy = "feel_hip_batch_destroy"
 
-    def __init__(self, ctx, rtol=1e-8, inner_rtol=1e-4, precision=FP32, hbm_budget=0, use_graph=True,
+    def __init__(self, ctx, rtol=1e-8, inner_rtol=1e-2, precision=FP32, hbm_budget=0, use_graph=True,
                  maxit=100000, max_outer=50):
         o = BatchOpts()
         lib().feel_hip_batch_opts_default(C.byref(o))

@@ -2,8 +2,7 @@
 // Poisson families: A(mu) = sum_t mu_t A_t over resident term matrices
 // (the affine decomposition Feel++'s reduced-basis layer assembles once,
 // mor/mor/feel/feelmor/modelcrbbase.hpp:1244), homogeneous Dirichlet rows,
-// Jacobi-CG with FP32 inner solves inside FP64 iterative refinement (or FP64
-// throughout).  Every in-flight instance advances in the same kernel launches:
+// Jacobi-CG in FP32 with FP64 reliable updates (or FP64 throughout).  Every in-flight instance advances in the same kernel launches:
 // launches run over a work list of (instance, slice range) items, each
 // instance keeps its own CG state and its own iteration origin, so instances
 // admitted at different times share launches (continuous batching).
@@ -30,11 +29,16 @@ namespace {
 
 constexpr int kItemSlices = 64;   // slices (64 rows each) per work item
 constexpr int kMaxTerms = 8;
+// a reliable update whose true residual exceeds the FP32 estimate by more
+// than this factor restarts the search direction
+constexpr double kBatchDriftRestart = 2.0;
 
 struct BInst {
     int64_t n, nslices;
     const uint32_t* slicePtr;
     const int32_t* col;
+    SellCols cols;             // column stream (packed 16/32-bit when compact)
+    int compact;
     const int32_t* sdiag;
     const uint8_t* mask;       // per slot: 0 keep, 1 zero, 2 unit diagonal (Dirichlet)
     const double* b64;         // family rhs, solver numbering, Dirichlet rows zero
@@ -49,7 +53,7 @@ struct BInst {
     int nitems;
     int start;                 // global iteration at which the current inner solve began
     int ibeg, iend;            // item range of this instance in the work list
-    double scale;              // inner rhs = scale * r64
+    double scale;              // FP32 residual = scale * r64 (scale = 1 / sigma, fixed per request)
     double xscale;             // x64 += xscale * xT
 };
 struct BItem {
@@ -217,7 +221,8 @@ __global__ __launch_bounds__(kBlock) void kBInnerInit(const BItem* __restrict__ 
     if (threadIdx.x == 0) { I.part[blockIdx.x - I.ibeg] = zr; I.part[I.nitems + blockIdx.x - I.ibeg] = zz; }
 }
 
-// CG direction with the convergence test (as kCgDirection, per instance)
+// CG direction with the convergence test (as kCgDirection, per instance); the
+// direction (re)starts as p = z at it == st->fresh
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ items, const BInst* __restrict__ insts,
                                                      const int* __restrict__ gcount) {
@@ -244,17 +249,20 @@ __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ 
         I.slots[it & 1] = beta;
     }
     if (reason) return;
-    const double bb = it == 0 ? 0.0 : beta / I.slots[(it - 1) & 1];
+    const bool first = it == st->fresh;
+    const double bb = first ? 0.0 : beta / I.slots[(it - 1) & 1];
     int64_t r0, r1;
     itemRows(I, item, r0, r1);
     T* p = tp<T>(I.pT);
     const T* z = tp<T>(I.zT);
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) p[r] = it == 0 ? z[r] : (T)((double)z[r] + bb * (double)p[r]);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) p[r] = first ? z[r] : (T)((double)z[r] + bb * (double)p[r]);
 }
 
-// w = A p on the item's slices, partial p.Ap
+// w = A p on the item's slices (one wavefront per slice, the SELL row kernel
+// of the single solver), partial p.Ap
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
+    constexpr int U = sizeof(T) == 8 ? 4 : 8;
     const BItem item = items[blockIdx.x];
     const BInst& I = insts[item.inst];
     if (instDone(I.st)) return;
@@ -262,22 +270,10 @@ __global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items
     const T* val = tp<T>(I.valT);
     const T* x = tp<T>(I.pT);
     T* y = tp<T>(I.wT);
+    const SellCols S = I.cols;
     double dot = 0.0;
     for (int s = item.s0 + (int)(threadIdx.x >> 6); s < item.s1; s += kBlock / 64) {
-        const uint32_t base = I.slicePtr[s];
-        const int width = (int)((I.slicePtr[s + 1] - base) >> 6);
-        T a0 = 0, a1 = 0;
-        int k = 0;
-        for (; k + 1 < width; k += 2) {
-            const size_t s0 = base + (size_t)k * 64 + lane, s1 = s0 + 64;
-            a0 += __builtin_nontemporal_load(val + s0) * x[__builtin_nontemporal_load(I.col + s0)];
-            a1 += __builtin_nontemporal_load(val + s1) * x[__builtin_nontemporal_load(I.col + s1)];
-        }
-        if (k < width) {
-            const size_t s0 = base + (size_t)k * 64 + lane;
-            a0 += val[s0] * x[I.col[s0]];
-        }
-        const T acc = a0 + a1;
+        const T acc = I.compact ? sellRow<T, U, true>(S, s, lane, val, x) : sellRow<T, U, false>(S, s, lane, val, x);
         const int64_t r = (int64_t)s * 64 + lane;
         if (r < I.n) {
             y[r] = acc;
@@ -366,7 +362,11 @@ struct Request {
     int buf = -1, state = -1;
     int start = 0;
     double dp = 0, rnorm0 = 0, ttol = 0, xscale = 0;
+    double sigma = 0;      // FP32 residual scale (the initial FP64 residual norm)
+    double estimate = 0;   // FP32 estimate of the FP64 residual norm when the device stopped
     int its = 0, outer = 0, reason = 0;
+    int fresh = 0;         // iteration at which the search direction last (re)started
+    double devRnorm0 = 0;  // the device's initial residual norm (FP32 units), for the dtol test
     bool unpermuted = false;   // solution copied to Feel++ numbering
 };
 
@@ -464,6 +464,8 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
         I.nslices = f.nslices;
         I.slicePtr = f.g->slicePtr;
         I.col = f.g->scol;
+        I.cols = SellCols{f.g->slicePtr, f.g->scol, f.g->colBase, f.g->ccptr, f.g->packed};
+        I.compact = f.g->compact ? 1 : 0;
         I.sdiag = f.g->sdiag;
         I.mask = f.mask;
         I.b64 = f.b64;
@@ -477,7 +479,7 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
         I.part = ib.part;
         I.nitems = f.nitems;
         I.start = r->start;
-        I.scale = r->dp > 0 ? 1.0 / r->dp : 0.0;
+        I.scale = r->sigma > 0 ? 1.0 / r->sigma : 0.0;
         I.xscale = r->xscale;
         I.ibeg = (int)items.size();
         for (int64_t s = 0; s < f.nslices; s += kItemSlices)
@@ -599,6 +601,7 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         TRY_CALL(residualNorms(B, admitted));
         for (Request* r : admitted) {
             r->rnorm0 = r->dp;
+            r->sigma = r->dp;
             r->ttol = std::max(B->opts.rtol * r->rnorm0, B->opts.atol);
             r->phase = Phase::Resid;
         }
@@ -623,18 +626,29 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
                                c->stream, f.n, f.g->perm, ib.x64, ib.out);
         }
     }
+    // FP32 CG with reliable updates: the device stops once its residual
+    // estimate has dropped by inner_rtol since the last FP64 residual (or
+    // reaches the tolerance); the FP32 residual then restarts from the true one
+    // and CG continues with the same direction and iteration count.
     std::vector<CgState> fresh(toInner.size());   // kept alive until buildWork synchronises
     if (!toInner.empty()) {
         for (size_t i = 0; i < toInner.size(); ++i) {
             Request* r = toInner[i];
+            const bool restart = r->outer == 0 || r->dp > kBatchDriftRestart * r->estimate;
+            if (restart) r->fresh = r->its;
             CgState& h = fresh[i];
             h = CgState{};
-            h.rtol = B->opts.inner_rtol;
-            h.atol = 0;
+            const double stop = std::max(r->ttol, B->opts.inner_rtol * r->dp) / r->sigma;
+            h.rtol = 0.0;
+            h.atol = stop;
+            h.ttol = stop;
+            h.rnorm0 = r->devRnorm0;
             h.dtol = 1e5;
-            h.maxit = std::min(B->opts.inner_maxit, B->opts.maxit - r->its);
+            h.maxit = B->opts.maxit;
+            h.fresh = r->fresh;
             FEEL_HIP_CHECK(hipMemcpyAsync(B->states + r->state, &h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-            r->start = B->hgcount;
+            // the chunk's first iteration runs as iteration r->its of this request
+            r->start = B->hgcount - r->its;
             r->phase = Phase::Inner;
         }
         TRY_CALL(buildWork(B, toInner, &nItems));
@@ -656,10 +670,12 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         for (Request* r : inner) {
             const CgState& st = hs[r->state];
             if (!st.done) continue;
-            r->its += st.its;
+            r->its = st.its;
             r->outer += 1;
-            if (st.reason == -8 || st.reason == -9) { finish(B, *r, st.reason); continue; }
-            r->xscale = r->dp;   // x64 += dp * x_inner (the inner rhs was r64 / dp)
+            if (r->outer == 1) r->devRnorm0 = st.rnorm0;
+            r->estimate = st.dp * r->sigma;
+            if (st.reason == -8 || st.reason == -9 || st.reason == -4) { finish(B, *r, st.reason); continue; }
+            r->xscale = r->sigma;   // x64 += sigma * x_fp32 (the FP32 residual is r64 / sigma)
             ended.push_back(r);
         }
         // 5. FP64 residual of the instances whose inner solve ended
@@ -691,7 +707,7 @@ void feel_hip_batch_opts_default(feel_hip_batch_opts* o) {
     if (!o) return;
     o->rtol = 1e-8;
     o->atol = 1e-50;
-    o->inner_rtol = 1e-4;
+    o->inner_rtol = 1e-2;
     o->maxit = 100000;
     o->inner_maxit = 100000;
     o->max_outer = 50;

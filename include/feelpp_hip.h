@@ -205,7 +205,7 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
 typedef struct feel_hip_batch feel_hip_batch;
 typedef struct {
     double rtol, atol;        /* outer FP64 test ||D^-1 r|| <= max(rtol ||D^-1 b||, atol) */
-    double inner_rtol;        /* FP32 inner reduction per refinement step */
+    double inner_rtol;        /* FP32: residual reduction between FP64 reliable updates */
     int maxit, inner_maxit, max_outer;
     int precision;            /* FEEL_HIP_FP32 (refined) or FEEL_HIP_FP64 */
     int64_t hbm_budget;       /* bytes of instance buffers allowed at once, 0 = unlimited */
