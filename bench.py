@@ -482,11 +482,11 @@ def main():
     bytes_scatter = 12 * nnz + prob.ne * nloc * 4 + prob.ne * 8 + prob.ne * 16 + nv * 24
     g_ms = gms / max(ng, 1)
     g_ach = bytes_scatter / (g_ms * 1e-3) / 1e9 if ng else 0.0
-    scatter = {"kernel": "kGatherScalar<100,6,8,true>" if order == 2 else "kGatherScalar", "bound": "hbm",
+    scatter = {"kernel": "kGatherScalar<100,8,true>" if order == 2 else "kGatherScalar", "bound": "hbm",
                "achieved": round(g_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                "frac": round(g_ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(g_ms * 1e3, 2),
                "algorithmic_bytes": bytes_scatter,
-               "traffic": traffic.get("kGatherScalar<100,6,8,true>") if measured_workload else None}
+               "traffic": traffic.get("kGatherScalar<100,8,true>") if measured_workload else None}
 
     cpu = None
     if not args.no_cpu:

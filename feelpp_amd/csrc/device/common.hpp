@@ -35,7 +35,9 @@ constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying vector k
 constexpr int kBlock = 256;
 constexpr int kSpmvGrid = 2048;       // grid of the solver SpMV (one p.Ap partial per block)
 constexpr int kSpmvBlock = 1024;
-constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
+constexpr int kHistMax = 8192;
+constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
+constexpr int kSegTask = 256;         // scatter entries per task (whole slots)        // CG coefficient history kept for the condition estimate
 
 }  // namespace feelpp_amd
 
@@ -109,8 +111,14 @@ struct feel_hip_graph {
     int32_t* ia = nullptr;
     int32_t* ja = nullptr;
     int32_t* diagpos = nullptr;
-    uint32_t* cptr = nullptr;     // nnz+1, into centry
-    uint32_t* centry = nullptr;   // element-matrix entry ids e*nldof^2 + i*nldof + j
+    // gather map: per CSR slot (in slot order) the element-matrix entry ids
+    // e*nldof^2 + i*nldof + j landing in it, ascending element order; bit 31
+    // marks a slot's last entry.  Tasks: runs of whole slots for the scatter.
+    uint32_t* centry = nullptr;   // ncontrib
+    uint32_t* taskSeg = nullptr;  // ntasks+1: first segment (non-empty slot) of each task
+    uint32_t* taskK = nullptr;    // ntasks+1: first entry of each task
+    int32_t* segSlot = nullptr;   // nseg: CSR slot of each segment, only when some slot has no entry
+    int64_t ntasks = 0, nseg = 0;
     int64_t ncontrib = 0;
     std::vector<int32_t> hia, hja;
     // SELL-C-sigma image used by the solver (permuted numbering)

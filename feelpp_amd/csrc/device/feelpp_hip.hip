@@ -164,8 +164,10 @@ void unref(feel_hip_graph* g) {
     c->release(g->ia, sizeof(int32_t) * (g->n + 1));
     c->release(g->ja, sizeof(int32_t) * g->nnz);
     c->release(g->diagpos, sizeof(int32_t) * g->n);
-    c->release(g->cptr, sizeof(uint32_t) * (g->nnz + 1));
     c->release(g->centry, sizeof(uint32_t) * g->ncontrib);
+    c->release(g->taskSeg, sizeof(uint32_t) * (g->ntasks + 1));
+    c->release(g->taskK, sizeof(uint32_t) * (g->ntasks + 1));
+    if (g->segSlot) c->release(g->segSlot, sizeof(int32_t) * g->nseg);
     c->release(g->slicePtr, sizeof(uint32_t) * (g->nslices + 1));
     c->release(g->scol, sizeof(int32_t) * g->nslots);
     c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
@@ -316,28 +318,46 @@ int64_t feel_hip_space_nelements(const feel_hip_space* s) { return s ? s->ne : -
 
 namespace {
 
-// reference tensors [Rsym(NS) | M] for the scalar forms, cached per quadrature order
+// scatter launch: one wavefront per task at most (4 per 256-thread block)
+#define SEG_ARGS g->ntasks, g->taskSeg, g->taskK, g->centry, g->segSlot, s->geo, rt, v64, v32, add
+int segGrid(const feel_hip_graph* g) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((g->ntasks + kBlock / 64 - 1) / (kBlock / 64), 8192));
+}
+
+// A replacing assembly writes only slots that receive element entries; a
+// pattern with slots no element touches gets them cleared first.
+int clearEmptySlots(feel_hip_ctx* c, const feel_hip_graph* g, double* v64, float* v32, int add) {
+    if (add || !g->segSlot) return FEEL_HIP_OK;
+    if (v64) FEEL_HIP_CHECK(hipMemsetAsync(v64, 0, sizeof(double) * g->nnz, c->stream));
+    if (v32) FEEL_HIP_CHECK(hipMemsetAsync(v32, 0, sizeof(float) * g->nnz, c->stream));
+    return FEEL_HIP_OK;
+}
+
+// reference tensors of the scalar forms, entry-major: row ij holds
+// [Rsym_0 .. Rsym_{NS-1} | M | 0 padding] (ST = 8 in 3D, 4 in 2D doubles),
+// cached per quadrature order
 int scalarTensors(feel_hip_space* s, int qorder, const double** out) {
     auto it = s->rtab.find(qorder);
     if (it != s->rtab.end()) { *out = it->second; return FEEL_HIP_OK; }
     ReferenceTensors T = referenceTensors(s->dim, s->order, qorder);
-    const int n = T.n, nn = n * n, d = s->dim, ns = numSym(d);
-    std::vector<double> h((size_t)(ns + 1) * nn);
+    const int n = T.n, nn = n * n, d = s->dim, ns = numSym(d), st = d == 3 ? 8 : 4;
+    std::vector<double> h((size_t)st * nn, 0.0);
     auto R = [&](int a, int b, int ij) { return T.R[(size_t)(a * d + b) * nn + ij]; };
     for (int ij = 0; ij < nn; ++ij) {
+        double* row = h.data() + (size_t)ij * st;
         if (d == 3) {
-            h[0 * nn + ij] = R(0, 0, ij);
-            h[1 * nn + ij] = R(1, 1, ij);
-            h[2 * nn + ij] = R(2, 2, ij);
-            h[3 * nn + ij] = R(0, 1, ij) + R(1, 0, ij);
-            h[4 * nn + ij] = R(0, 2, ij) + R(2, 0, ij);
-            h[5 * nn + ij] = R(1, 2, ij) + R(2, 1, ij);
+            row[0] = R(0, 0, ij);
+            row[1] = R(1, 1, ij);
+            row[2] = R(2, 2, ij);
+            row[3] = R(0, 1, ij) + R(1, 0, ij);
+            row[4] = R(0, 2, ij) + R(2, 0, ij);
+            row[5] = R(1, 2, ij) + R(2, 1, ij);
         } else {
-            h[0 * nn + ij] = R(0, 0, ij);
-            h[1 * nn + ij] = R(1, 1, ij);
-            h[2 * nn + ij] = R(0, 1, ij) + R(1, 0, ij);
+            row[0] = R(0, 0, ij);
+            row[1] = R(1, 1, ij);
+            row[2] = R(0, 1, ij) + R(1, 0, ij);
         }
-        h[(size_t)ns * nn + ij] = T.M[ij];
+        row[ns] = T.M[ij];
     }
     double* dptr = nullptr;
     TRY_CALL(upload(s->ctx, &dptr, h.data(), h.size()));
@@ -347,14 +367,18 @@ int scalarTensors(feel_hip_space* s, int qorder, const double** out) {
     return FEEL_HIP_OK;
 }
 
-// full reference tensors R_ab (a,b < 3, 9 x n^2) for the vector forms, cached per quadrature order
+// full reference tensors R_ab (a,b < 3) for the vector forms, cached per quadrature order
 // under key -(qorder+1) so they do not collide with the scalar tables
 int fullTensors(feel_hip_space* s, int qorder, const double** out) {
     const int key = -(qorder + 1);
     auto it = s->rtab.find(key);
     if (it != s->rtab.end()) { *out = it->second; return FEEL_HIP_OK; }
     ReferenceTensors T = referenceTensors(s->dim, s->order, qorder);
-    std::vector<double> h(T.R.begin(), T.R.end());
+    // entry-major, 10 doubles per local pair (i, j): R_ab at a*3+b, zero pad
+    const size_t nn = (size_t)T.n * T.n;
+    std::vector<double> h(10 * nn, 0.0);
+    for (size_t ij = 0; ij < nn; ++ij)
+        for (int t = 0; t < 9; ++t) h[ij * 10 + t] = T.R[t * nn + ij];
     double* dptr = nullptr;
     TRY_CALL(upload(s->ctx, &dptr, h.data(), h.size()));
     s->rtab[key] = dptr;
@@ -455,12 +479,47 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
         return setError(FEEL_HIP_EPATTERN, e.what());
     }
     g->ncontrib = (int64_t)gm.entry.size();
-    std::vector<uint32_t> cptr(gm.ptr.begin(), gm.ptr.end());
+    if ((uint64_t)s->ne * s->nldof * s->nldof >= (1ULL << 31)) {
+        delete g;
+        return setError(FEEL_HIP_EUNSUPPORTED, "space too large for 31-bit element-entry ids");
+    }
+    // end-of-slot flags, segments (slots with entries) and tasks of whole slots
+    std::vector<uint32_t> taskSeg, taskK;
+    std::vector<int32_t> segSlot;
+    {
+        bool empty = false;
+        for (int64_t p = 0; p < g->nnz; ++p) empty |= gm.ptr[p + 1] == gm.ptr[p];
+        uint32_t seg = 0;
+        int64_t acc = 0;
+        taskSeg.push_back(0);
+        taskK.push_back(0);
+        for (int64_t p = 0; p < g->nnz; ++p) {
+            const int64_t b = gm.ptr[p], e = gm.ptr[p + 1];
+            if (e == b) continue;
+            gm.entry[e - 1] |= 0x80000000u;
+            if (empty) segSlot.push_back((int32_t)p);
+            ++seg;
+            acc += e - b;
+            if (acc >= kSegTask) {
+                taskSeg.push_back(seg);
+                taskK.push_back((uint32_t)e);
+                acc = 0;
+            }
+        }
+        if (acc > 0) {
+            taskSeg.push_back(seg);
+            taskK.push_back((uint32_t)g->ncontrib);
+        }
+        g->nseg = seg;
+        g->ntasks = (int64_t)taskSeg.size() - 1;
+    }
     int rc = upload(ctx, &g->ia, ia, (size_t)nrows + 1);
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->ja, ja, (size_t)g->nnz);
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->diagpos, diag.data(), diag.size());
-    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->cptr, cptr.data(), cptr.size());
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->centry, gm.entry.data(), gm.entry.size());
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->taskSeg, taskSeg.data(), taskSeg.size());
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->taskK, taskK.data(), taskK.size());
+    if (rc == FEEL_HIP_OK && !segSlot.empty()) rc = upload(ctx, &g->segSlot, segSlot.data(), segSlot.size());
     if (rc == FEEL_HIP_OK) {
         SellLayout L = buildSell(gc, 1024);
         g->nslices = L.nslices;
@@ -743,19 +802,20 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
             return setError(FEEL_HIP_EINVAL, "elasticity needs E and nu per element");
         const double* rt = nullptr;
         TRY_CALL(fullTensors(s, d->quad_order, &rt));
-        TRY_CALL(ensureGeo(s, sizeof(double) * 81 * s->ne));
+        TRY_CALL(ensureGeo(s, sizeof(double) * kElasticF * s->ne));
         c->tic("assemble_geo");
         hipLaunchKernelGGL(kGeoElastic, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->coords, s->elems,
                            d->young->d, d->poisson->d, s->geo);
         c->toc();
         FEEL_HIP_CHECK(hipGetLastError());
-        const int grid = gridFor(g->nnz, 8192);
+        const int grid = segGrid(g);
+        TRY_CALL(clearEmptySlots(c, g, v64, v32, add));
         c->tic("assemble_gather");
         switch (s->order) {
-            case 1: hipLaunchKernelGGL(kGatherElastic<4>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
-            case 2: hipLaunchKernelGGL(kGatherElastic<10>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
-            case 3: hipLaunchKernelGGL(kGatherElastic<20>, dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
-            case 4: hipLaunchKernelGGL((kGatherElastic<35, false>), dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, s->geo, rt, v64, v32, add); break;
+            case 1: hipLaunchKernelGGL(kGatherElastic<4>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
+            case 2: hipLaunchKernelGGL(kGatherElastic<10>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
+            case 3: hipLaunchKernelGGL(kGatherElastic<20>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
+            case 4: hipLaunchKernelGGL((kGatherElastic<35, false>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
             default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no elasticity kernel for this order");
         }
         c->toc();
@@ -794,21 +854,19 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
                            kds, km, kms, s->geo);
     c->toc();
     FEEL_HIP_CHECK(hipGetLastError());
-    const int grid = gridFor(g->nnz, 8192);
+    const int grid = segGrid(g);
+    TRY_CALL(clearEmptySlots(c, g, v64, v32, add));
     c->tic("assemble_gather");
-#define GATHER(NN, NS, ST)                                                                                      \
-    hipLaunchKernelGGL((kGatherScalar<NN, NS, ST>), dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr, g->centry, \
-                       s->geo, rt, v64, v32, add)
+#define GATHER(NN, ST) hipLaunchKernelGGL((kGatherScalar<NN, ST>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS)
     const int key = s->dim * 10 + s->order;
     switch (key) {
-        case 21: GATHER(9, 3, 4); break;
-        case 22: GATHER(36, 3, 4); break;
-        case 31: GATHER(16, 6, 8); break;
-        case 32: GATHER(100, 6, 8); break;
-        case 33: GATHER(400, 6, 8); break;
+        case 21: GATHER(9, 4); break;
+        case 22: GATHER(36, 4); break;
+        case 31: GATHER(16, 8); break;
+        case 32: GATHER(100, 8); break;
+        case 33: GATHER(400, 8); break;
         case 34:
-            hipLaunchKernelGGL((kGatherScalar<1225, 6, 8, false>), dim3(grid), dim3(kBlock), 0, c->stream, g->nnz, g->cptr,
-                               g->centry, s->geo, rt, v64, v32, add);
+            hipLaunchKernelGGL((kGatherScalar<1225, 8, false>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS);
             break;
         default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no scatter kernel for this space");
     }

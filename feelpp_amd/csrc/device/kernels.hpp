@@ -136,44 +136,107 @@ __global__ __launch_bounds__(kBlock) void kGeoScalar(int64_t ne, const double* _
     }
 }
 
-// Owner-computes scatter: one thread per CSR slot p sums the element entries
-// that land in p (ascending element order), each entry evaluated from the
-// element factors F and the reference tensors staged in LDS.
-// Tables larger than the LDS budget (P4: 7 x 1225 doubles) are read through L1/L2.
-template <int NN, int NS, int STRIDE, bool USE_LDS = true>
-__global__ __launch_bounds__(kBlock) void kGatherScalar(int64_t nnz, const uint32_t* __restrict__ cptr,
+// ---------------------------------------------------------------- scatter
+// Wavefront-level segmented reduction of element contributions into CSR
+// slots.  The gather map lists, slot by slot, the element-matrix entries that
+// land in each CSR slot in ascending element order (the order MatSetValues
+// ADD applies them); the top bit of an entry marks the last contribution of
+// its slot.  A task is a run of whole slots (about kSegTask contributions);
+// a wavefront walks its tasks 64 contributions at a time, one lane per
+// contribution, so the entry ids are read coalesced and the element-factor
+// gathers of a chunk are independent.  A shuffle-based inclusive segmented
+// scan (fixed tree, so results are run-to-run identical) sums each slot; the
+// lane holding a slot's last contribution writes it.  Slot index = task's
+// first segment + ends before the lane; segSlot maps segments to CSR slots
+// when the pattern has slots without contributions (else identity).
+template <class Value>
+__device__ inline void segScatter(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
+                                  const uint32_t* __restrict__ taskK, const uint32_t* __restrict__ centry,
+                                  const int32_t* __restrict__ segSlot, double* __restrict__ v64,
+                                  float* __restrict__ v32, int add, const Value& value) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int64_t t = wave; t < ntasks; t += nwaves) {
+        uint32_t seg = taskSeg[t];
+        const uint32_t kend = taskK[t + 1];
+        double carry = 0.0;
+        bool open = false;
+        for (uint32_t k = taskK[t]; k < kend; k += 64) {
+            const uint32_t kk = k + lane;
+            const bool valid = kk < kend;
+            const uint32_t c = valid ? centry[kk] : 0u;
+            const bool isEnd = valid && (c >> 31);
+            double x = valid ? value(c & 0x7fffffffu) : 0.0;
+            const uint64_t ends = __ballot(isEnd);
+            bool head = lane == 0 ? !open : (((ends >> (lane - 1)) & 1ull) != 0);
+            if (lane == 0 && open) x = carry + x;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const double y = __shfl_up(x, off, 64);
+                const int hy = __shfl_up((int)head, off, 64);
+                if (lane >= off) {
+                    if (!head) x = y + x;
+                    head = head || hy;
+                }
+            }
+            if (isEnd) {
+                const uint32_t sidx = seg + (uint32_t)__popcll(ends & below);
+                const int64_t p = segSlot ? (int64_t)segSlot[sidx] : (int64_t)sidx;
+                if (v64) {
+                    const double v = add ? v64[p] + x : x;
+                    v64[p] = v;
+                    if (v32) v32[p] = (float)v;
+                } else if (v32) {
+                    v32[p] = add ? (float)((double)v32[p] + x) : (float)x;
+                }
+            }
+            seg += (uint32_t)__popcll(ends);
+            const int last = (int)((kend - 1 - k) < 63u ? (kend - 1 - k) : 63u);
+            carry = __shfl(x, last, 64);
+            open = ((ends >> last) & 1ull) == 0;
+        }
+    }
+}
+
+// Scalar forms: entry (i,j) of element e = sum_t F[e][t] R[ij][t], with the
+// element factors F (ST doubles per element, zero padded) and the reference
+// tensors entry-major (zero padded), both read with 16-byte loads and summed
+// in the order t = 0 .. ST-1.  Tables larger than the LDS budget (P4: 1225 x 8
+// doubles) are read through L1/L2.
+template <int NN, int ST, bool USE_LDS = true>
+__global__ __launch_bounds__(kBlock) void kGatherScalar(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
+                                                        const uint32_t* __restrict__ taskK,
                                                         const uint32_t* __restrict__ centry,
+                                                        const int32_t* __restrict__ segSlot,
                                                         const double* __restrict__ F,
                                                         const double* __restrict__ rtab,
                                                         double* __restrict__ v64, float* __restrict__ v32,
                                                         int add) {
-    __shared__ double Rs[USE_LDS ? (NS + 1) * NN : 1];
+    constexpr int H = ST / 2;
+    __shared__ double2 Rs[USE_LDS ? NN * H : 1];
     if constexpr (USE_LDS) {
-        for (int i = threadIdx.x; i < (NS + 1) * NN; i += kBlock) Rs[i] = rtab[i];
+        for (int i = threadIdx.x; i < NN * H; i += kBlock) Rs[i] = reinterpret_cast<const double2*>(rtab)[i];
         __syncthreads();
     }
-    const double* R = USE_LDS ? Rs : rtab;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t b = cptr[p], e = cptr[p + 1];
-        double s = 0.0;
-        for (uint32_t k = b; k < e; ++k) {
-            const uint32_t c = centry[k];
-            const uint32_t el = c / NN;
-            const uint32_t ij = c - el * NN;
-            const double* f = F + (size_t)el * STRIDE;
-            double a = 0.0;
+    const double2* R = USE_LDS ? Rs : reinterpret_cast<const double2*>(rtab);
+    const double2* F2 = reinterpret_cast<const double2*>(F);
+    auto value = [&](uint32_t c) {
+        const uint32_t el = c / NN;
+        const uint32_t ij = c - el * NN;
+        const double2* f = F2 + (size_t)el * H;
+        const double2* r = R + (size_t)ij * H;
+        double a = 0.0;
 #pragma unroll
-            for (int t = 0; t <= NS; ++t) a += f[t] * R[t * NN + ij];
-            s += a;
+        for (int h = 0; h < H; ++h) {
+            const double2 fv = f[h], rv = r[h];
+            a += fv.x * rv.x;
+            a += fv.y * rv.y;
         }
-        if (v64) {
-            const double v = add ? v64[p] + s : s;
-            v64[p] = v;
-            if (v32) v32[p] = (float)v;
-        } else if (v32) {
-            v32[p] = add ? (float)((double)v32[p] + s) : (float)s;
-        }
-    }
+        return a;
+    };
+    segScatter(ntasks, taskSeg, taskK, centry, segSlot, v64, v32, add, value);
 }
 
 // Per-element factors of isotropic linear elasticity inner(sigma(u), grad v),
@@ -181,7 +244,8 @@ __global__ __launch_bounds__(kBlock) void kGatherScalar(int64_t nnz, const uint3
 // With B = K^{-T}, C = B^T B and R_ab(i,j) = sum_q w_q d_a phi_i d_b phi_j on the
 // reference element, the (c,i),(d,j) entry is sum_ab Ke[c][d][a][b] R_ab(i,j) with
 //   Ke[c][d][a][b] = J (lambda B_ca B_db + mu B_da B_cb + mu delta_cd C_ab).
-// Layout: 81 doubles per element, index ((c*3+d)*3+a)*3+b.
+// Layout: kElasticF = 90 doubles per element, block (c,d) at (c*3+d)*10 holding
+// index a*3+b and a zero pad (16-byte aligned blocks).
 static __global__ __launch_bounds__(kBlock) void kGeoElastic(int64_t ne, const double* __restrict__ coords,
                                                              const int32_t* __restrict__ elems,
                                                              const double* __restrict__ young,
@@ -202,77 +266,59 @@ static __global__ __launch_bounds__(kBlock) void kGeoElastic(int64_t ne, const d
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b) C[a][b] = B[0][a] * B[0][b] + B[1][a] * B[1][b] + B[2][a] * B[2][b];
-        double* f = F + e * 81;
+        double* f = F + e * kElasticF;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
 #pragma unroll
                     for (int b = 0; b < 3; ++b)
-                        f[((c * 3 + d) * 3 + a) * 3 + b] =
+                        f[(c * 3 + d) * 10 + a * 3 + b] =
                             J * (lam * B[c][a] * B[d][b] + mu * B[d][a] * B[c][b] + (c == d ? mu * C[a][b] : 0.0));
+                f[(c * 3 + d) * 10 + 9] = 0.0;
+            }
     }
 }
 
-// Owner-computes scatter for the vector elasticity form: local entry
-// ij = (c*n + i) * 3n + (d*n + j) of element e; R (9 x n^2) staged in LDS.
+// Vector elasticity form: local entry ij = (c*n + i) * 3n + (d*n + j) of
+// element e = sum_ab F_e[c,d,a,b] R_ab(i,j); reference tensors entry-major,
+// R[(i*n + j)*10 + a*3 + b] with a zero pad (16-byte loads, summed in the
+// order t = a*3 + b), staged in LDS when they fit.
 template <int N, bool USE_LDS = true>
-__global__ __launch_bounds__(kBlock) void kGatherElastic(int64_t nnz, const uint32_t* __restrict__ cptr,
+__global__ __launch_bounds__(kBlock) void kGatherElastic(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
+                                                         const uint32_t* __restrict__ taskK,
                                                          const uint32_t* __restrict__ centry,
+                                                         const int32_t* __restrict__ segSlot,
                                                          const double* __restrict__ F,
                                                          const double* __restrict__ rtab,
                                                          double* __restrict__ v64, float* __restrict__ v32, int add) {
     constexpr int NN = N * N, N3 = 3 * N, NN3 = N3 * N3;
-    __shared__ double Rs[USE_LDS ? 9 * NN : 1];
+    __shared__ double2 Rs[USE_LDS ? 5 * NN : 1];
     if constexpr (USE_LDS) {
-        for (int i = threadIdx.x; i < 9 * NN; i += kBlock) Rs[i] = rtab[i];
+        for (int i = threadIdx.x; i < 5 * NN; i += kBlock) Rs[i] = reinterpret_cast<const double2*>(rtab)[i];
         __syncthreads();
     }
-    const double* R = USE_LDS ? Rs : rtab;
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (uint32_t k = cptr[p]; k < cptr[p + 1]; ++k) {
-            const uint32_t c = centry[k];
-            const uint32_t el = c / NN3;
-            const uint32_t loc = c - el * NN3;
-            const uint32_t rl = loc / N3, cl = loc - rl * N3;
-            const uint32_t cc = rl / N, i = rl - cc * N, dd = cl / N, j = cl - dd * N;
-            const double* f = F + (size_t)el * 81 + (cc * 3 + dd) * 9;
-            const uint32_t ij = i * N + j;
-            double a = 0.0;
+    const double2* R = USE_LDS ? Rs : reinterpret_cast<const double2*>(rtab);
+    const double2* F2 = reinterpret_cast<const double2*>(F);
+    auto value = [&](uint32_t c) {
+        const uint32_t el = c / NN3;
+        const uint32_t loc = c - el * NN3;
+        const uint32_t rl = loc / N3, cl = loc - rl * N3;
+        const uint32_t cc = rl / N, i = rl - cc * N, dd = cl / N, j = cl - dd * N;
+        const double2* f = F2 + (size_t)el * (kElasticF / 2) + (cc * 3 + dd) * 5;
+        const double2* r = R + (size_t)(i * N + j) * 5;
+        double a = 0.0;
 #pragma unroll
-            for (int t = 0; t < 9; ++t) a += f[t] * R[t * NN + ij];
-            s += a;
+        for (int h = 0; h < 5; ++h) {
+            const double2 fv = f[h], rv = r[h];
+            a += fv.x * rv.x;
+            a += fv.y * rv.y;
         }
-        if (v64) {
-            const double v = add ? v64[p] + s : s;
-            v64[p] = v;
-            if (v32) v32[p] = (float)v;
-        } else if (v32) {
-            v32[p] = add ? (float)((double)v32[p] + s) : (float)s;
-        }
-    }
-}
-
-// Generic scatter of materialised element matrices (elem[e*NN + ij]).
-static __global__ __launch_bounds__(kBlock) void kGatherElemMat(int64_t nnz, const uint32_t* __restrict__ cptr,
-                                                         const uint32_t* __restrict__ centry,
-                                                         const double* __restrict__ elem,
-                                                         double* __restrict__ v64, float* __restrict__ v32,
-                                                         int add) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz; p += (int64_t)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (uint32_t k = cptr[p]; k < cptr[p + 1]; ++k) s += elem[centry[k]];
-        if (v64) {
-            const double v = add ? v64[p] + s : s;
-            v64[p] = v;
-            if (v32) v32[p] = (float)v;
-        } else if (v32) {
-            v32[p] = add ? (float)((double)v32[p] + s) : (float)s;
-        }
-    }
+        return a;
+    };
+    segScatter(ntasks, taskSeg, taskK, centry, segSlot, v64, v32, add, value);
 }
 
 // Element vectors of f*id(v) (linearformcontext.hpp:294-305):
@@ -292,22 +338,33 @@ __global__ __launch_bounds__(kBlock) void kElemRhs(int64_t ne, int nl, int ncomp
         double Ki[DIM][DIM];
         const double J = affineMap<DIM>(coords, el, Ki);
         double* out = be + e * (size_t)nl * ncomp;
-        for (int i = 0; i < nl * ncomp; ++i) out[i] = 0.0;
-        for (int q = 0; q < Q; ++q) {
-            double f[3] = {p0, p1, p2};
-            if (kind == FEEL_HIP_SRC_SINSIN) {
-                double x[DIM];
+        double X[DIM + 1][DIM];
+        if (kind == FEEL_HIP_SRC_SINSIN)
 #pragma unroll
-                for (int c = 0; c < DIM; ++c) x[c] = 0.0;
-                for (int v = 0; v <= DIM; ++v)
+            for (int v = 0; v <= DIM; ++v)
 #pragma unroll
-                    for (int c = 0; c < DIM; ++c) x[c] += coords[(size_t)el[v] * DIM + c] * lam[v * Q + q];
-                f[0] = p0 * sin(M_PI * x[0]) * sin(M_PI * x[1]);
-            }
-            const double wJ = w[q] * J;
-            for (int c = 0; c < ncomp; ++c) {
-                const double fc = (kind == FEEL_HIP_SRC_VECTOR) ? f[c] : f[0];
-                for (int i = 0; i < nl; ++i) out[c * nl + i] += wJ * fc * phi[i * Q + q];
+                for (int c = 0; c < DIM; ++c) X[v][c] = coords[(size_t)el[v] * DIM + c];
+        // each entry is accumulated over q in a register and written once
+        for (int c = 0; c < ncomp; ++c) {
+            const double fconst = kind == FEEL_HIP_SRC_VECTOR ? (c == 0 ? p0 : c == 1 ? p1 : p2) : p0;
+            for (int i = 0; i < nl; ++i) {
+                double acc = 0.0;
+                for (int q = 0; q < Q; ++q) {
+                    double fc = fconst;
+                    if (kind == FEEL_HIP_SRC_SINSIN) {
+                        double x[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) x[d] = 0.0;
+#pragma unroll
+                        for (int v = 0; v <= DIM; ++v)
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) x[d] += X[v][d] * lam[v * Q + q];
+                        fc = p0 * sin(M_PI * x[0]) * sin(M_PI * x[1]);
+                    }
+                    const double wJ = w[q] * J;
+                    acc += wJ * fc * phi[i * Q + q];
+                }
+                out[c * nl + i] = acc;
             }
         }
     }
