@@ -444,7 +444,7 @@ class Matrix(_Handle):
         _check(lib().feel_hip_mat_zero_rows(self.h, rows.h, values.h if values else None, rhs.h if rhs else None,
                                             mode, diag))
 
-    def solve(self, b, x, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, precision=FP64, inner_rtol=0.1,
+    def solve(self, b, x, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, precision=FP64, inner_rtol=1e-2,
               inner_maxit=100000, max_outer=50, check_every=0):
         o = SolveOpts(rtol, atol, dtol, maxit, precision, inner_rtol, inner_maxit, max_outer, check_every)
         out = SolveData()

@@ -33,7 +33,8 @@ struct CgState {
 
 constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying vector kernels
 constexpr int kBlock = 256;
-constexpr int kSpmvGrid = 2048;       // grid of the solver SpMV (one p.Ap partial per block)
+constexpr int kSpmvGrid = 2048;       // default grid of the solver SpMV (one p.Ap partial per block)
+constexpr int kSpmvPartCap = 4096;    // largest SpMV grid (partials buffer capacity)
 constexpr int kSpmvBlock = 1024;
 constexpr int kHistMax = 8192;
 constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
@@ -68,7 +69,7 @@ struct feel_hip_ctx {
     size_t evUsed = 0;
     std::map<std::string, std::pair<int64_t, double>> timers;
     // CG scratch: partial sums (3 arrays of kReduceBlocks doubles) and state
-    double* part = nullptr;   // 3 x kReduceBlocks + kSpmvGrid doubles
+    double* part = nullptr;   // 3 x kReduceBlocks + kSpmvPartCap doubles
     feelpp_amd::CgState* state = nullptr;
     feelpp_amd::CgState* hstate = nullptr;   // pinned host mirror
     double* hist = nullptr;   // 2 x kHistMax CG coefficients (beta, p.Ap) of the running solve
@@ -119,6 +120,9 @@ struct feel_hip_graph {
     uint32_t* taskK = nullptr;    // ntasks+1: first entry of each task
     int32_t* segSlot = nullptr;   // nseg: CSR slot of each segment, only when some slot has no entry
     int64_t ntasks = 0, nseg = 0;
+    // SpMV launch shape picked by the autotuner, per precision (0 FP64, 1 FP32):
+    // index into the candidate list, -1 until the first solve measures them
+    int spmvShape[2] = {-1, -1};
     int64_t ncontrib = 0;
     std::vector<int32_t> hia, hja;
     // SELL-C-sigma image used by the solver (permuted numbering)

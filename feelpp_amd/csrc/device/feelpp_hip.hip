@@ -127,7 +127,7 @@ void unref(feel_hip_ctx* c) {
     if (--c->refs > 0) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
+    c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvPartCap));
     c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
     c->release(c->hist, sizeof(double) * 2 * kHistMax);
     if (c->hstate) (void)hipHostFree(c->hstate);
@@ -206,7 +206,7 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
         delete c;
         return setError(FEEL_HIP_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    int rc = c->alloc((void**)&c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvGrid));
+    int rc = c->alloc((void**)&c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvPartCap));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->state, sizeof(CgState) + 4 * sizeof(double));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->hist, sizeof(double) * 2 * kHistMax);
     if (rc == FEEL_HIP_OK && hipHostMalloc((void**)&c->hstate, sizeof(CgState), hipHostMallocDefault) != hipSuccess)

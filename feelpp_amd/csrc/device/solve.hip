@@ -24,18 +24,82 @@ namespace {
 
 SellCols sellCols(const feel_hip_graph* g) { return SellCols{g->slicePtr, g->scol, g->colBase, g->ccptr, g->packed}; }
 
-// y = A x in the solver numbering (SELL image), optional p.Ap partials
-// Returns the grid used (the number of p.Ap partials written when part is set).
-template <typename T>
-int spmv(feel_hip_ctx* c, const feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st) {
+// SpMV launch shapes the autotuner chooses from: (block size, grid).  The
+// per-lane unroll is fixed per precision, so every shape computes each row
+// with the same operations in the same order; shapes differ only in how rows
+// map to waves and in the number of p.Ap block partials.
+struct SpmvShape { int block, grid; };
+constexpr SpmvShape kSpmvShapes[] = {{1024, 2048}, {1024, 1024}, {512, 2048}, {512, 1024}, {256, 1024},
+                                     {256, 2048}, {256, 4096}};
+constexpr int kNumSpmvShapes = sizeof(kSpmvShapes) / sizeof(kSpmvShapes[0]);
+// a shape replaces the default only when it is this much faster, so that the
+// pick is stable from run to run where shapes are close
+constexpr double kShapeMargin = 0.95;
+
+template <typename T, int BS, bool COMPACT>
+void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, int grid, const T* val, const T* x, T* y, double* part,
+                const CgState* st) {
     constexpr int U = sizeof(T) == 8 ? 4 : 8;
-    if (g->compact)
-        hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U, true>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
-                           g->nslices, sellCols(g), val, x, y, part, st);
-    else
-        hipLaunchKernelGGL((kSellSpmv<T, kSpmvBlock, U, false>), dim3(kSpmvGrid), dim3(kSpmvBlock), 0, c->stream, g->n,
-                           g->nslices, sellCols(g), val, x, y, part, st);
-    return kSpmvGrid;
+    hipLaunchKernelGGL((kSellSpmv<T, BS, U, COMPACT>), dim3(grid), dim3(BS), 0, c->stream, g->n, g->nslices,
+                       sellCols(g), val, x, y, part, st);
+}
+
+template <typename T>
+void launchShape(feel_hip_ctx* c, const feel_hip_graph* g, const SpmvShape& sh, const T* val, const T* x, T* y,
+                 double* part, const CgState* st) {
+    if (g->compact) {
+        if (sh.block == 1024) launchSpmv<T, 1024, true>(c, g, sh.grid, val, x, y, part, st);
+        else if (sh.block == 512) launchSpmv<T, 512, true>(c, g, sh.grid, val, x, y, part, st);
+        else launchSpmv<T, 256, true>(c, g, sh.grid, val, x, y, part, st);
+    } else {
+        if (sh.block == 1024) launchSpmv<T, 1024, false>(c, g, sh.grid, val, x, y, part, st);
+        else if (sh.block == 512) launchSpmv<T, 512, false>(c, g, sh.grid, val, x, y, part, st);
+        else launchSpmv<T, 256, false>(c, g, sh.grid, val, x, y, part, st);
+    }
+}
+
+// Autotuner: times every candidate shape on this graph's matrix (2 warm-up
+// and 5 timed launches each, HIP events on the context stream) and keeps the
+// fastest, or the default unless another beats it by the margin.  Runs once
+// per graph and precision; part receives partials of at most kSpmvPartCap blocks.
+template <typename T>
+int tuneSpmv(feel_hip_ctx* c, feel_hip_graph* g, const T* val, const T* x, T* y, double* part) {
+    int& pick = g->spmvShape[sizeof(T) == 8 ? 0 : 1];
+    hipEvent_t a, b;
+    FEEL_HIP_CHECK(hipEventCreate(&a));
+    FEEL_HIP_CHECK(hipEventCreate(&b));
+    double best = 1e300, def = 1e300;
+    int bestI = 0;
+    for (int i = 0; i < kNumSpmvShapes; ++i) {
+        for (int w = 0; w < 2; ++w) launchShape<T>(c, g, kSpmvShapes[i], val, x, y, part, nullptr);
+        (void)hipEventRecord(a, c->stream);
+        for (int r = 0; r < 5; ++r) launchShape<T>(c, g, kSpmvShapes[i], val, x, y, part, nullptr);
+        (void)hipEventRecord(b, c->stream);
+        FEEL_HIP_CHECK(hipEventSynchronize(b));
+        float ms = 0;
+        FEEL_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        if (i == 0) def = ms;
+        if (ms < best) { best = ms; bestI = i; }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    FEEL_HIP_CHECK(hipGetLastError());
+    pick = best < kShapeMargin * def ? bestI : 0;
+    return FEEL_HIP_OK;
+}
+
+// y = A x in the solver numbering (SELL image), optional p.Ap partials, with
+// the graph's tuned launch shape (tuned on the first call).  Returns the
+// number of partials written (the grid).
+template <typename T>
+int spmv(feel_hip_ctx* c, feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st,
+         int* npart) {
+    const int idx = sizeof(T) == 8 ? 0 : 1;
+    if (g->spmvShape[idx] < 0) TRY_CALL(tuneSpmv<T>(c, g, val, x, y, c->part + 3 * kReduceBlocks));
+    const SpmvShape& sh = kSpmvShapes[g->spmvShape[idx]];
+    launchShape<T>(c, g, sh, val, x, y, part, st);
+    if (npart) *npart = sh.grid;
+    return FEEL_HIP_OK;
 }
 
 int gridN(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 8192)); }
@@ -89,7 +153,7 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
     const int64_t n = g->n;
     double* zrPart = c->part;
     double* zzPart = c->part + kReduceBlocks;
-    double* dpiPart = c->part + 3 * kReduceBlocks;   // kSpmvGrid partials
+    double* dpiPart = c->part + 3 * kReduceBlocks;   // up to kSpmvPartCap partials
     CgState* st = c->state;
     double* betaSlots = reinterpret_cast<double*>(st + 1);
     double* dpiSlots = betaSlots + 2;
@@ -99,7 +163,7 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
     if (x0zero)
         FEEL_HIP_CHECK(hipMemsetAsync(w, 0, sizeof(T) * n, c->stream));
     else
-        spmv<T>(c, g, val, x, w, nullptr, nullptr);
+        TRY_CALL(spmv<T>(c, g, val, x, w, nullptr, nullptr, nullptr));
     hipLaunchKernelGGL(kCgInit<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, b, w, r, z, dinv, zrPart, zzPart);
     FEEL_HIP_CHECK(hipGetLastError());
     int it = 0;
@@ -109,7 +173,8 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
             hipLaunchKernelGGL(kCgDirection<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, z, p, zrPart, zzPart,
                                st, betaSlots, (double*)nullptr);
             c->tic(timerName);
-            const int npart = spmv<T>(c, g, val, p, w, dpiPart, st);
+            int npart = 0;
+            TRY_CALL(spmv<T>(c, g, val, p, w, dpiPart, st, &npart));
             c->toc();
             hipLaunchKernelGGL(kCgUpdate<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, x, r, z, p, w, dinv,
                                dpiPart, npart, zrPart, zzPart, st, betaSlots, dpiSlots, (double*)nullptr);
@@ -133,7 +198,7 @@ int residual64(feel_hip_mat* A, const double* b, const double* x, double* r, dou
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
     c->tic("spmv_fp64");
-    spmv<double>(c, g, A->s64, x, w, nullptr, nullptr);
+    TRY_CALL(spmv<double>(c, g, A->s64, x, w, nullptr, nullptr, nullptr));
     c->toc();
     hipLaunchKernelGGL(kResidual64, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, g->n, b, w, A->dinv64, r,
                        c->part + 2 * kReduceBlocks);
@@ -185,14 +250,14 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, MixedRun& out) {
     const int64_t n = g->n;
     double* zrPart = c->part;
     double* zzPart = c->part + kReduceBlocks;
-    double* dpiPart = c->part + 3 * kReduceBlocks;   // kSpmvGrid partials
+    double* dpiPart = c->part + 3 * kReduceBlocks;   // up to kSpmvPartCap partials
     CgState* st = c->state;
     double* betaSlots = reinterpret_cast<double*>(st + 1);
     double* dpiSlots = betaSlots + 2;
     double dp = 0;
     TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
     const double rnorm0 = dp, ttol = std::max(o.rtol * dp, o.atol);
-    const double delta = o.inner_rtol > 0 && o.inner_rtol < 1 ? o.inner_rtol : 0.1;
+    const double delta = o.inner_rtol > 0 && o.inner_rtol < 1 ? o.inner_rtol : 1e-2;
     out.rnorm0 = rnorm0;
     out.dp = dp;
     if (dp != dp) { out.reason = -9; return FEEL_HIP_OK; }
@@ -242,7 +307,8 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, MixedRun& out) {
             hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->z32,
                                A->p32, zrPart, zzPart, st, betaSlots, c->hist);
             c->tic("spmv_fp32");
-            const int npart = spmv<float>(c, g, A->s32, A->p32, A->w32, dpiPart, st);
+            int npart = 0;
+            TRY_CALL(spmv<float>(c, g, A->s32, A->p32, A->w32, dpiPart, st, &npart));
             c->toc();
             hipLaunchKernelGGL(kCgUpdate<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->x32,
                                A->r32, A->z32, A->p32, A->w32, A->dinv32, dpiPart, npart, zrPart, zzPart, st,
@@ -323,7 +389,7 @@ void feel_hip_solve_opts_default(feel_hip_solve_opts* o) {
     o->dtol = 1e5;
     o->maxit = 1000;
     o->precision = FEEL_HIP_FP64;
-    o->inner_rtol = 0.1;
+    o->inner_rtol = 1e-2;
     o->inner_maxit = 100000;
     o->max_outer = 50;
     o->check_every = 0;

@@ -82,7 +82,7 @@ typedef struct {
     int maxit;          /* ksp-maxit */
     int precision;      /* FEEL_HIP_FP64: plain FP64 Jacobi-CG; FEEL_HIP_FP32: FP32 Jacobi-CG with
                            FP64 reliable updates (FP64 solution, residual and stopping test) */
-    double inner_rtol;  /* FP32: reduction of the residual estimate between reliable updates (0.1) */
+    double inner_rtol;  /* FP32: reduction of the residual estimate between reliable updates (1e-2) */
     int inner_maxit;    /* unused by the single solve */
     int max_outer;      /* FP32: cap on reliable updates */
     int check_every;    /* host polls the device convergence flag every this many iterations (0 = auto) */
