@@ -43,6 +43,8 @@ constexpr int kSegTask = 256;         // scatter entries per task (whole slots) 
 }  // namespace feelpp_amd
 
 namespace feelpp_amd {
+// gather map of the scatter for (space, graph), built on the device (gathermap.hip)
+int buildGatherMapDevice(feel_hip_space* s, feel_hip_graph* g);
 void retain(feel_hip_ctx* c);
 void retain(feel_hip_space* s);
 void retain(feel_hip_graph* g);

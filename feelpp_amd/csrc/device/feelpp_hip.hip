@@ -465,61 +465,13 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
         }
         diag[r] = (int32_t)(p - ja);
     }
-    DofTable dt;
-    dt.dim = s->dim; dt.order = s->order; dt.ncomp = s->ncomp; dt.nloc = s->nloc; dt.nldof = s->nldof;
-    dt.ne = s->ne; dt.ndof = s->ndof;
-    dt.elem2dof = s->hElem2dof;
     GraphCSR gc;
     gc.nrows = nrows; gc.nnz = g->nnz; gc.ia = g->hia; gc.ja = g->hja;
-    GatherMap gm;
-    try {
-        gm = buildGatherMap(dt, gc);
-    } catch (const std::runtime_error& e) {
-        delete g;
-        return setError(FEEL_HIP_EPATTERN, e.what());
-    }
-    g->ncontrib = (int64_t)gm.entry.size();
-    if ((uint64_t)s->ne * s->nldof * s->nldof >= (1ULL << 31)) {
-        delete g;
-        return setError(FEEL_HIP_EUNSUPPORTED, "space too large for 31-bit element-entry ids");
-    }
-    // end-of-slot flags, segments (slots with entries) and tasks of whole slots
-    std::vector<uint32_t> taskSeg, taskK;
-    std::vector<int32_t> segSlot;
-    {
-        bool empty = false;
-        for (int64_t p = 0; p < g->nnz; ++p) empty |= gm.ptr[p + 1] == gm.ptr[p];
-        uint32_t seg = 0;
-        int64_t acc = 0;
-        taskSeg.push_back(0);
-        taskK.push_back(0);
-        for (int64_t p = 0; p < g->nnz; ++p) {
-            const int64_t b = gm.ptr[p], e = gm.ptr[p + 1];
-            if (e == b) continue;
-            gm.entry[e - 1] |= 0x80000000u;
-            if (empty) segSlot.push_back((int32_t)p);
-            ++seg;
-            acc += e - b;
-            if (acc >= kSegTask) {
-                taskSeg.push_back(seg);
-                taskK.push_back((uint32_t)e);
-                acc = 0;
-            }
-        }
-        if (acc > 0) {
-            taskSeg.push_back(seg);
-            taskK.push_back((uint32_t)g->ncontrib);
-        }
-        g->nseg = seg;
-        g->ntasks = (int64_t)taskSeg.size() - 1;
-    }
     int rc = upload(ctx, &g->ia, ia, (size_t)nrows + 1);
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->ja, ja, (size_t)g->nnz);
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->diagpos, diag.data(), diag.size());
-    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->centry, gm.entry.data(), gm.entry.size());
-    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->taskSeg, taskSeg.data(), taskSeg.size());
-    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->taskK, taskK.data(), taskK.size());
-    if (rc == FEEL_HIP_OK && !segSlot.empty()) rc = upload(ctx, &g->segSlot, segSlot.data(), segSlot.size());
+    // gather map of the scatter, built on the device from elem2dof and ia/ja
+    if (rc == FEEL_HIP_OK) rc = buildGatherMapDevice(s, g);
     if (rc == FEEL_HIP_OK) {
         SellLayout L = buildSell(gc, 1024);
         g->nslices = L.nslices;

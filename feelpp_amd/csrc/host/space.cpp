@@ -436,38 +436,6 @@ std::vector<uint8_t> boundaryDofs(const SimplexMesh& m, const DofTable& dt, int 
     return isb;
 }
 
-GatherMap buildGatherMap(const DofTable& dt, const GraphCSR& g, int nthreads) {
-    if (nthreads <= 0) nthreads = defaultThreads();
-    const int nl = dt.nldof;
-    const int64_t nn = (int64_t)nl * nl, tot = dt.ne * nn;
-    if (tot >= (1LL << 32)) throw std::overflow_error("too many element-matrix entries for a 32-bit map");
-    std::vector<int32_t> pos(tot);
-    std::atomic<bool> outside{false};
-    parallelFor(dt.ne, nthreads, [&](int64_t b, int64_t e) {
-        for (int64_t el = b; el < e && !outside.load(std::memory_order_relaxed); ++el) {
-            const int32_t* l2g = &dt.elem2dof[el * nl];
-            for (int i = 0; i < nl; ++i) {
-                const int32_t* rb = g.ja.data() + g.ia[l2g[i]];
-                const int32_t* re = g.ja.data() + g.ia[l2g[i] + 1];
-                for (int j = 0; j < nl; ++j) {
-                    const int32_t* p = std::lower_bound(rb, re, l2g[j]);
-                    if (p == re || *p != l2g[j]) { outside = true; return; }
-                    pos[el * nn + i * nl + j] = (int32_t)(p - g.ja.data());
-                }
-            }
-        }
-    });
-    if (outside) throw std::runtime_error("element entry outside the sparsity graph");
-    GatherMap gm;
-    gm.ptr.assign(g.nnz + 1, 0);
-    for (int64_t k = 0; k < tot; ++k) gm.ptr[pos[k] + 1]++;
-    for (int64_t p = 0; p < g.nnz; ++p) gm.ptr[p + 1] += gm.ptr[p];
-    gm.entry.resize(tot);
-    std::vector<int64_t> cur(gm.ptr.begin(), gm.ptr.end() - 1);
-    for (int64_t k = 0; k < tot; ++k) gm.entry[cur[pos[k]]++] = (uint32_t)k;
-    return gm;
-}
-
 std::vector<int32_t> reverseCuthillMcKee(const GraphCSR& g) {
     const int64_t n = g.nrows;
     std::vector<int32_t> order;

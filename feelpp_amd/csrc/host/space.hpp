@@ -52,15 +52,6 @@ GraphCSR buildGraph(const DofTable& dt, int nthreads = 0);
 // vertices all lie on x == 0 (sel = 1).  Returns a 0/1 flag per dof.
 std::vector<uint8_t> boundaryDofs(const SimplexMesh& m, const DofTable& dt, int sel);
 
-// Gather map for the owner-computes scatter: for CSR slot p, the element-matrix
-// entries (e * nldof^2 + i * nldof + j) that add into it, in ascending element
-// order (the order in which MatSetValues(ADD) would add them).
-struct GatherMap {
-    std::vector<int64_t> ptr;     // nnz + 1
-    std::vector<uint32_t> entry;  // sum of contributions
-};
-GatherMap buildGatherMap(const DofTable& dt, const GraphCSR& g, int nthreads = 0);
-
 // dof -> (element, local dof) incidence in element order: entry = e*nldof + l
 struct Incidence {
     std::vector<int64_t> ptr;     // ndof + 1
