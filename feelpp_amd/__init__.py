@@ -122,6 +122,8 @@ _SIGS = {
     "feel_hip_mat_zero_rows": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double]),
     "feel_hip_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "feel_hip_solve_pcg": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveData)]),
+    "feel_hip_norm_l2": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
+    "feel_hip_norm_semi_h1": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
     "feel_hip_batch_opts_default": (None, [C.POINTER(BatchOpts)]),
     "feel_hip_batch_create": (C.c_int, [_vp, C.POINTER(BatchOpts), C.POINTER(_vp)]),
     "feel_hip_batch_destroy": (C.c_int, [_vp]),
@@ -461,6 +463,38 @@ class Rows(_Handle):
         _check(lib().feel_hip_rows_create(ctx.h, n, rows.shape[0], rows, C.byref(h)))
         super().__init__(h, keep=(ctx,))
         self.rows = rows
+
+
+FN_ZERO, FN_LINEAR, FN_SINSIN = 0, 1, 2
+
+
+class Function(C.Structure):
+    _fields_ = [("kind", C.c_int), ("c", C.c_double * 4)]
+
+
+def _functions(space, funcs):
+    if funcs is None:
+        return None
+    arr = (Function * space.ncomp)()
+    for i, (kind, params) in enumerate(funcs):
+        arr[i].kind = kind
+        for k, v in enumerate(list(params)[:4]):
+            arr[i].c[k] = v
+    return arr
+
+
+def norm_l2(space, u, funcs=None, quad_order=-1):
+    """normL2(elements(mesh), u - f): funcs gives (kind, [c0..c3]) per component (None: f = 0)."""
+    out = C.c_double()
+    _check(lib().feel_hip_norm_l2(space.h, u.h, _functions(space, funcs), quad_order, C.byref(out)))
+    return out.value
+
+
+def norm_semi_h1(space, u, funcs=None, quad_order=-1):
+    """H1 seminorm of u - f over elements(mesh)."""
+    out = C.c_double()
+    _check(lib().feel_hip_norm_semi_h1(space.h, u.h, _functions(space, funcs), quad_order, C.byref(out)))
+    return out.value
 
 
 def assemble_linear(space, b, kind, quad_order, params=(1.0,), add=False):

@@ -105,6 +105,9 @@ struct feel_hip_space {
     // per quadrature order: weights, basis values, P1 geometry basis values
     std::map<int, double*> qtab;
     std::map<int, int> qnpts;
+    // per quadrature order, for the norms: weights, values, gradients, P1 geometry values
+    std::map<int, std::pair<double*, size_t>> ntab;   // device table, doubles
+    std::map<int, int> nnpts;
 };
 
 struct feel_hip_graph {

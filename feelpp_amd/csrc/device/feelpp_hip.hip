@@ -153,6 +153,7 @@ void unref(feel_hip_space* s) {
     for (auto& kv : s->rtab) c->release(kv.second, sizeof(double) * s->hrtab[kv.first].size());
     for (auto& kv : s->qtab)
         c->release(kv.second, sizeof(double) * (size_t)s->qnpts[kv.first] * (1 + s->nloc + s->dim + 1));
+    for (auto& kv : s->ntab) c->release(kv.second.first, sizeof(double) * kv.second.second);
     delete s;
     unref(c);
 }

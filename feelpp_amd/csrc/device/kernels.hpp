@@ -370,6 +370,105 @@ __global__ __launch_bounds__(kBlock) void kElemRhs(int64_t ne, int nl, int ncomp
     }
 }
 
+// ---------------------------------------------------------------- norms
+// Reference functions of the error norms, one per component.
+struct NormFn {
+    int kind[3];
+    double c[3][4];
+};
+
+template <int DIM>
+__device__ inline double normFnValue(const NormFn& f, int comp, const double* x, double* grad) {
+    const double* c = f.c[comp];
+    if (f.kind[comp] == FEEL_HIP_FN_LINEAR) {
+        double v = c[0];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) { v += c[d + 1] * x[d]; grad[d] = c[d + 1]; }
+        return v;
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) grad[d] = 0.0;
+    if (f.kind[comp] == FEEL_HIP_FN_SINSIN) {
+        const double sx = sin(M_PI * x[0]), sy = sin(M_PI * x[1]);
+        grad[0] = c[0] * M_PI * cos(M_PI * x[0]) * sy;
+        grad[1] = c[0] * M_PI * sx * cos(M_PI * x[1]);
+        return c[0] * sx * sy;
+    }
+    return 0.0;
+}
+
+// Block partials of integrate(elements, inner(u - f)) (semi = 0) or of
+// integrate(elements, inner(grad(u) - grad(f))) (semi = 1) on affine simplices
+// (feelvf/norml2.hpp:37-57, normsemih1.hpp): u_h(x_q) = sum_l phi_l(q) u_l,
+// grad u_h = sum_l u_l B grad phi_l(q) with B = K^{-T}, x_q = sum_v lambda_v(q) X_v.
+// tab: w[Q] | phi[nloc*Q] | dphi[(l*Q+q)*DIM+a] | lambda[(DIM+1)*Q].
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void kElemNorm(int64_t ne, int nloc, int ncomp, int Q,
+                                                    const double* __restrict__ coords,
+                                                    const int32_t* __restrict__ elems,
+                                                    const int32_t* __restrict__ e2d, const double* __restrict__ u,
+                                                    const double* __restrict__ tab, NormFn fn, int semi,
+                                                    double* __restrict__ part) {
+    const double* w = tab;
+    const double* phi = tab + Q;
+    const double* dphi = phi + (size_t)nloc * Q;
+    const double* lam = dphi + (size_t)nloc * Q * DIM;
+    const int nldof = nloc * ncomp;
+    double acc = 0.0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t* el = elems + e * (DIM + 1);
+        double Ki[DIM][DIM];
+        const double J = affineMap<DIM>(coords, el, Ki);
+        double X[DIM + 1][DIM];
+#pragma unroll
+        for (int v = 0; v <= DIM; ++v)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) X[v][d] = coords[(size_t)el[v] * DIM + d];
+        for (int q = 0; q < Q; ++q) {
+            double x[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                x[d] = 0.0;
+#pragma unroll
+                for (int v = 0; v <= DIM; ++v) x[d] += X[v][d] * lam[v * Q + q];
+            }
+            const double wJ = w[q] * J;
+            for (int c = 0; c < ncomp; ++c) {
+                const int32_t* l2g = e2d + e * nldof + c * nloc;
+                double gf[DIM];
+                const double fv = normFnValue<DIM>(fn, c, x, gf);
+                if (!semi) {
+                    double uh = 0.0;
+                    for (int l = 0; l < nloc; ++l) uh += phi[(size_t)l * Q + q] * u[l2g[l]];
+                    const double d = uh - fv;
+                    acc += wJ * d * d;
+                } else {
+                    double g[DIM];
+#pragma unroll
+                    for (int r = 0; r < DIM; ++r) g[r] = 0.0;
+                    for (int l = 0; l < nloc; ++l) {
+                        const double ul = u[l2g[l]];
+                        const double* dp = dphi + ((size_t)l * Q + q) * DIM;
+#pragma unroll
+                        for (int r = 0; r < DIM; ++r) {
+                            double s = 0.0;
+#pragma unroll
+                            for (int a = 0; a < DIM; ++a) s += Ki[a][r] * dp[a];
+                            g[r] += ul * s;
+                        }
+                    }
+                    double n2 = 0.0;
+#pragma unroll
+                    for (int r = 0; r < DIM; ++r) n2 += (g[r] - gf[r]) * (g[r] - gf[r]);
+                    acc += wJ * n2;
+                }
+            }
+        }
+    }
+    acc = blockReduceSum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 static __global__ __launch_bounds__(kBlock) void kGatherVec(int64_t n, const uint32_t* __restrict__ rptr,
                                                      const uint32_t* __restrict__ rentry,
                                                      const double* __restrict__ be, double* __restrict__ b,

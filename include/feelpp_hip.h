@@ -99,6 +99,15 @@ typedef struct {
                            found the matrix too ill-conditioned for FP32 (Lanczos estimate of kappa) */
 } feel_hip_solve_data;
 
+/* ---- post-solve evaluation ------------------------------------------------ */
+/* Reference functions for error norms: LINEAR c0 + c1 x + c2 y + c3 z,
+ * SINSIN c0 sin(pi x) sin(pi y). */
+enum { FEEL_HIP_FN_ZERO = 0, FEEL_HIP_FN_LINEAR = 1, FEEL_HIP_FN_SINSIN = 2 };
+typedef struct {
+    int kind;
+    double c[4];
+} feel_hip_function;
+
 /* ---- lifetime ------------------------------------------------------------ */
 /* Handles are reference counted the way MatrixSparse/GraphCSR shared_ptrs are
    (feelalg/matrixsparse.hpp:113): each object keeps what it was built on
@@ -192,6 +201,14 @@ int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const fee
 void feel_hip_solve_opts_default(feel_hip_solve_opts* o);
 int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, const feel_hip_solve_opts* opts,
                        feel_hip_solve_data* out);
+
+/* normL2(_range=elements(mesh), _expr=idv(u) - f) = sqrt(|integrate(inner(u - f))|)
+ * (feelvf/norml2.hpp:37-57) and the H1 seminorm of u - f (feelvf/normsemih1.hpp):
+ * f holds one function per component (null: zero); quad_order < 0 picks
+ * 2k (+2 for SINSIN), the order of the squared integrand. */
+int feel_hip_norm_l2(feel_hip_space* s, const feel_hip_vec* u, const feel_hip_function* f, int quad_order, double* out);
+int feel_hip_norm_semi_h1(feel_hip_space* s, const feel_hip_vec* u, const feel_hip_function* f, int quad_order,
+                          double* out);
 
 /* ---- batched and continuously batched solves ------------------------------
  * Affine-parameter families A(mu) = sum_t mu_t A_t (the affine decomposition of

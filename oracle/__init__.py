@@ -53,6 +53,9 @@ def lib():
         L.orc_pcg_mt.argtypes = [C.c_long, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_double, C.c_double,
                                  C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
                                  C.POINTER(C.c_int)]
+        L.orc_norm.restype = C.c_double
+        L.orc_norm.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p, _f64p, _i32p,
+                               _f64p, C.c_int]
         L.orc_assemble_elasticity.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, _i32p, _i32p,
                                               _f64p, _f64p, _i32p, _i32p, _f64p]
         L.orc_assemble_rhs.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _f64p, _i32p,
@@ -228,6 +231,22 @@ def spmv(ia, ja, val, x):
     y = np.zeros(ia.shape[0] - 1)
     lib().orc_spmv(ia.shape[0] - 1, ia, ja, val, np.ascontiguousarray(x, dtype=np.float64), y)
     return y
+
+
+FN_ZERO, FN_LINEAR, FN_SINSIN = 0, 1, 2
+
+
+def norm(dim, order, ncomp, qorder, coords, elems, elem2dof, u, funcs=None, semi=False):
+    """L2 norm (or H1 seminorm) of u - f over all elements; funcs: per component (kind, [c0..c3])."""
+    funcs = funcs or [(FN_ZERO, [0.0] * 4)] * ncomp
+    kinds = np.array([k for k, _ in funcs], dtype=np.int32)
+    params = np.array([list(p) + [0.0] * (4 - len(p)) for _, p in funcs], dtype=np.float64).ravel()
+    v = lib().orc_norm(dim, order, ncomp, qorder, elems.shape[0], np.ascontiguousarray(coords),
+                       np.ascontiguousarray(elems, dtype=np.int32), np.ascontiguousarray(elem2dof, dtype=np.int32),
+                       np.ascontiguousarray(u, dtype=np.float64), kinds, params, int(semi))
+    if v < 0:
+        raise RuntimeError("orc_norm failed")
+    return v
 
 
 def pcg_mt(ia, ja, val, b, x0=None, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, threads=1):

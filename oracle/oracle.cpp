@@ -877,6 +877,60 @@ int orc_dirichlet(int mode, long n, const int* ia, const int* ja, double* val, d
     return 0;
 }
 
+// normL2(_range=elements(mesh), _expr=idv(u) - f) = sqrt(|sum_e sum_q w_q J (u_h - f)^2|)
+// (feelvf/norml2.hpp:37-57) and the H1 seminorm with grad(u_h) - grad(f)
+// (feelvf/normsemih1.hpp), per component c of a Pch/Pchv space (local dof
+// nloc*c + l).  f per component: kind 0 zero, 1 c0 + c1 x + c2 y + c3 z,
+// 2 c0 sin(pi x) sin(pi y).  x_q = sum_v lambda_v(q) X_v on the [-1,1] simplex.
+double orc_norm(int dim, int order, int ncomp, int qorder, int ne, const double* coords, const int* elems,
+                const int* elem2dof, const double* u, const int* kinds, const double* params, int semi) {
+    Tab T;
+    if (!tabulate(dim, order, qorder, T)) return -1.0;
+    const int n = T.n, Q = T.Q, nldof = n * ncomp;
+    double X[12];
+    double total = 0.0;
+    for (long e = 0; e < ne; ++e) {
+        elem_coords(dim, coords, elems + (long)(dim + 1) * e, X);
+        Geo g = geomap(dim, X);
+        for (int q = 0; q < Q; ++q) {
+            double lam[4], x[3] = {0, 0, 0}, sum = 0;
+            for (int c = 0; c < dim; ++c) { lam[c + 1] = 0.5 * (1.0 + T.pts[q * dim + c]); sum += lam[c + 1]; }
+            lam[0] = 1.0 - sum;
+            for (int d = 0; d < dim; ++d)
+                for (int v = 0; v <= dim; ++v) x[d] += lam[v] * X[v * dim + d];
+            const double wJ = T.w[q] * g.J;
+            for (int c = 0; c < ncomp; ++c) {
+                const int* l2g = elem2dof + (long)e * nldof + c * n;
+                const double* pc = params + 4 * c;
+                double fv = 0.0, gf[3] = {0, 0, 0};
+                if (kinds[c] == 1) {
+                    fv = pc[0];
+                    for (int d = 0; d < dim; ++d) { fv += pc[d + 1] * x[d]; gf[d] = pc[d + 1]; }
+                } else if (kinds[c] == 2) {
+                    fv = pc[0] * std::sin(M_PI * x[0]) * std::sin(M_PI * x[1]);
+                    gf[0] = pc[0] * M_PI * std::cos(M_PI * x[0]) * std::sin(M_PI * x[1]);
+                    gf[1] = pc[0] * M_PI * std::sin(M_PI * x[0]) * std::cos(M_PI * x[1]);
+                }
+                if (!semi) {
+                    double uh = 0.0;
+                    for (int l = 0; l < n; ++l) uh += T.phi[l * Q + q] * u[l2g[l]];
+                    total += wJ * (uh - fv) * (uh - fv);
+                } else {
+                    double gu[3] = {0, 0, 0};
+                    for (int l = 0; l < n; ++l)
+                        for (int r = 0; r < dim; ++r) {
+                            double s2 = 0;
+                            for (int a = 0; a < dim; ++a) s2 += g.B[r][a] * T.dphi[(l * Q + q) * dim + a];
+                            gu[r] += u[l2g[l]] * s2;
+                        }
+                    for (int r = 0; r < dim; ++r) total += wJ * (gu[r] - gf[r]) * (gu[r] - gf[r]);
+                }
+            }
+        }
+    }
+    return std::sqrt(std::fabs(total));
+}
+
 // y = A x (MatMult_SeqAIJ)
 void orc_spmv(long n, const int* ia, const int* ja, const double* val, const double* x, double* y) {
     for (long r = 0; r < n; ++r) {

@@ -200,3 +200,41 @@ def test_threaded_baseline_matches(orc, threads):
     xt, itst, _, rt = orc.pcg_mt(ia, ja, vt, b, rtol=1e-12, maxit=5000, threads=threads)
     assert r1 > 0 and rt > 0 and abs(its1 - itst) <= 2
     assert np.linalg.norm(x1 - xt) <= 1e-10 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (3, 1), (3, 2), (3, 3), (3, 4)])
+def test_oracle_norms_closed_forms(orc, dim, order):
+    # pins of the norm restatement by what the mathematics fixes:
+    # ||1||_L2 = sqrt(measure); the interpolant of an affine function has zero
+    # L2 and H1-seminorm error; |grad(a.x)|^2 integrates to |a|^2 measure
+    if dim == 2:
+        coords, elems = orc.mesh_square(6, jitter=0.2, seed=3)
+    else:
+        coords, elems = orc.mesh_kuhn(3, jitter=0.15, seed=3)
+    e2d, ndof = orc.doftable(dim, order, 1, elems)
+    nodes = orc.lagrange_nodes(dim, order)
+    pos, mism = dof_positions(coords, elems, e2d, nodes, ndof)
+    assert mism < 1e-12
+    zero = np.zeros(ndof)
+    one = [(orc.FN_LINEAR, [1.0, 0, 0, 0])]
+    assert abs(orc.norm(dim, order, 1, 2 * order, coords, elems, e2d, zero, one) - 1.0) < 1e-13
+    a = np.array([0.3, -1.2, 0.7, 2.0][:dim + 1])
+    lin = [(orc.FN_LINEAR, list(a))]
+    u = a[0] + pos @ a[1:]
+    assert orc.norm(dim, order, 1, 2 * order, coords, elems, e2d, u, lin) < 1e-13
+    assert orc.norm(dim, order, 1, 2 * order, coords, elems, e2d, u, lin, semi=True) < 1e-12
+    h1 = orc.norm(dim, order, 1, 2 * order, coords, elems, e2d, u, None, semi=True)
+    assert abs(h1 - np.linalg.norm(a[1:])) < 1e-12
+
+
+def test_oracle_norm_convergence(orc):
+    # interpolation error of sin(pi x) sin(pi y) in L2 decays like h^(k+1)
+    errs = []
+    for n in (8, 16):
+        coords, elems = orc.mesh_square(n, jitter=0.0, seed=0)
+        e2d, ndof = orc.doftable(2, 2, 1, elems)
+        pos, _ = dof_positions(coords, elems, e2d, orc.lagrange_nodes(2, 2), ndof)
+        u = np.sin(np.pi * pos[:, 0]) * np.sin(np.pi * pos[:, 1])
+        errs.append(orc.norm(2, 2, 1, 6, coords, elems, e2d, u, [(orc.FN_SINSIN, [1.0, 0, 0, 0])]))
+    rate = np.log2(errs[0] / errs[1])
+    assert 2.8 < rate < 3.2, rate
