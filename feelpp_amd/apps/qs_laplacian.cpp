@@ -1,7 +1,9 @@
 // Quickstart Laplacian on the MI355X backend, following cgLaplacian
 // (feelpp/feel/feelpde/cg_laplacian.hpp:94-147): l = integrate(f*id(v));
 // a = integrate(inner(k*gradt(u),grad(v))) [+ mass]; a += on(boundary, g = 0);
-// a.solve().  Prints one JSON line; --out PREFIX writes PREFIX.sol (raw doubles).
+// a.solve(); normL2 / normSemiH1 of the solution (and its L2 error when the
+// exact solution is known).  Prints one JSON line; --out PREFIX writes
+// PREFIX.sol (raw doubles).
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -65,11 +67,19 @@ int main(int argc, char** argv) {
         }
         double nrm = 0;
         for (double xi : x) nrm += xi * xi;
+        // post-solve evaluation: normL2 and H1 seminorm of u; with k = 1 and the
+        // sin(pi x) sin(pi y) source in 2D the exact solution is f / (2 pi^2 + mass)
+        const double l2 = normL2(elements(mesh), u), h1 = normSemiH1(elements(mesh), u);
+        double l2err = -1.0;
+        if (dim == 2 && source == "sinsin" && kappa == "one")
+            l2err = normL2(elements(mesh), u, {sinsinFn(1.0 / (2.0 * M_PI * M_PI + mass))});
         std::printf("{\"app\": \"qs_laplacian\", \"dim\": %d, \"order\": %d, \"elements\": %lld, \"ndof\": %lld, "
                     "\"form\": \"%s\", \"converged\": %d, \"its\": %d, \"outer\": %d, \"residual\": %.6e, "
-                    "\"norm\": %.17g, \"setup_s\": %.3f, \"assemble_s\": %.4f, \"solve_s\": %.4f}\n",
+                    "\"norm\": %.17g, \"norm_l2\": %.17g, \"norm_semi_h1\": %.17g, \"l2_error\": %.17g, "
+                    "\"setup_s\": %.3f, \"assemble_s\": %.4f, \"solve_s\": %.4f}\n",
                     dim, order, (long long)mesh->numElements(), (long long)Vh->nDof(), a.recognised().c_str(),
-                    (int)r.converged, r.iterations, r.outer, r.residual, std::sqrt(nrm), tSetup, tAssemble, tSolve);
+                    (int)r.converged, r.iterations, r.outer, r.residual, std::sqrt(nrm), l2, h1, l2err, tSetup,
+                    tAssemble, tSolve);
         return r.converged ? 0 : 1;
     } catch (std::exception const& ex) {
         std::fprintf(stderr, "qs_laplacian: %s\n", ex.what());

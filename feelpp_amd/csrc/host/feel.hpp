@@ -161,6 +161,28 @@ class FunctionSpace : public std::enable_shared_from_this<FunctionSpace> {
     feel_hip_space* M_space = nullptr;
     int M_next_id = 0;
 };
+// Reference functions of the error norms (per component)
+inline feel_hip_function linearFn(double c0, double cx = 0, double cy = 0, double cz = 0) {
+    return feel_hip_function{FEEL_HIP_FN_LINEAR, {c0, cx, cy, cz}};
+}
+inline feel_hip_function sinsinFn(double c0) { return feel_hip_function{FEEL_HIP_FN_SINSIN, {c0, 0, 0, 0}}; }
+
+// normL2(_range=elements(mesh), _expr=idv(u) - f) (feelvf/norml2.hpp:37-57); f per
+// component, empty for f = 0; quad < 0 takes the order of the squared integrand
+inline double normL2(ElementsRange const&, Element const& u, std::vector<feel_hip_function> const& f = {},
+                     int quad = -1) {
+    double out = 0;
+    check(feel_hip_norm_l2(u.functionSpace()->handle(), u.vec(), f.empty() ? nullptr : f.data(), quad, &out));
+    return out;
+}
+// H1 seminorm of u - f (feelvf/normsemih1.hpp)
+inline double normSemiH1(ElementsRange const&, Element const& u, std::vector<feel_hip_function> const& f = {},
+                         int quad = -1) {
+    double out = 0;
+    check(feel_hip_norm_semi_h1(u.functionSpace()->handle(), u.vec(), f.empty() ? nullptr : f.data(), quad, &out));
+    return out;
+}
+
 template <int K>
 space_ptr Pch(mesh_ptr mesh, std::shared_ptr<Backend> be = backend()) {
     return std::make_shared<FunctionSpace>(mesh, K, 1, be);

@@ -53,6 +53,15 @@ def test_qs_laplacian_c1_like(orc, tmp_path):
     assert info["form"] == "mass+diffusion"
     xo = oracle_poisson(orc, 2, 32, 1, 2, 2, None, orc.FK_SINSIN, 1.0, 0.2, 1)
     assert rel_fro(x, xo) < 1e-10
+    # post-solve evaluation vs the oracle's norms of its own solution, and the
+    # O(h^2) L2 error against the exact solution sin(pi x) sin(pi y) / (2 pi^2 + 1)
+    coords, elems = orc.mesh_square(32, jitter=0.2, seed=1)
+    e2d, _ = orc.doftable(2, 1, 1, elems)
+    assert abs(info["norm_l2"] - orc.norm(2, 1, 1, 2, coords, elems, e2d, xo)) < 1e-9 * info["norm_l2"]
+    assert abs(info["norm_semi_h1"] - orc.norm(2, 1, 1, 2, coords, elems, e2d, xo, semi=True)) < 1e-9 * info["norm_semi_h1"]
+    exact = [(orc.FN_SINSIN, [1.0 / (2 * np.pi ** 2 + 1.0), 0, 0, 0])]
+    assert abs(info["l2_error"] - orc.norm(2, 1, 1, 4, coords, elems, e2d, xo, exact)) < 1e-6 * info["l2_error"]
+    assert info["l2_error"] < 5e-3 * info["norm_l2"]
 
 
 def test_qs_laplacian_mixed_precision(orc, tmp_path):
