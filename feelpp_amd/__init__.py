@@ -69,6 +69,8 @@ _SIGS = {
                                       C.c_uint64, _f64p, _i32p]),
     "feel_mesh_write_msh22": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, _f64p, C.c_int64, _i32p]),
     "feel_mesh_read_msh22": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int64), _vp, C.POINTER(C.c_int64), _vp]),
+    "feel_mesh_write_msh22_solution": (C.c_int, [C.c_char_p, C.c_int, C.c_int64, _f64p, C.c_int64, _i32p, C.c_int,
+                                                 C.c_int, _i32p, _f64p, C.c_char_p, C.c_double, C.c_int]),
     "feel_field_lognormal": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_uint64, _f64p]),
     "feel_field_uniform": (C.c_int, [C.c_int64, C.c_double, C.c_double, C.c_int, C.c_uint64, _f64p]),
     "feel_doftable_build": (C.c_int64, [C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, _i32p, _i32p]),
@@ -178,6 +180,15 @@ def write_msh22(path, coords, elems):
     dim = coords.shape[1]
     _check(lib().feel_mesh_write_msh22(path.encode(), dim, coords.shape[0], np.ascontiguousarray(coords),
                                        elems.shape[0], np.ascontiguousarray(elems, dtype=np.int32)), host=True)
+
+
+def write_msh22_solution(path, coords, elems, order, elem2dof, values, ncomp=1, name="u", time=0.0, step=0):
+    """Mesh + $ElementNodeData of a Pk field (vertex values), as Feel++'s Gmsh exporter writes it."""
+    dim = coords.shape[1]
+    _check(lib().feel_mesh_write_msh22_solution(
+        path.encode(), dim, coords.shape[0], np.ascontiguousarray(coords), elems.shape[0],
+        np.ascontiguousarray(elems, dtype=np.int32), order, ncomp, np.ascontiguousarray(elem2dof, dtype=np.int32),
+        np.ascontiguousarray(values, dtype=np.float64), name.encode(), time, step), host=True)
 
 
 def read_msh22(path, dim):

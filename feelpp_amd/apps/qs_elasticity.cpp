@@ -14,7 +14,7 @@ using namespace feelhip;
 int main(int argc, char** argv) {
     int nx = 40, ny = 10, nz = 10, order = 2, seed = 7;
     double rtol = 1e-10, jitter = 0.0;
-    std::string precision = "fp64", out;
+    std::string precision = "fp64", out, exportPath;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&]() { return std::string(i + 1 < argc ? argv[++i] : ""); };
@@ -27,6 +27,7 @@ int main(int argc, char** argv) {
         else if (a == "--rtol") rtol = std::stod(next());
         else if (a == "--precision") precision = next();
         else if (a == "--out") out = next();
+        else if (a == "--export") exportPath = next();
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     try {
@@ -63,6 +64,7 @@ int main(int argc, char** argv) {
             std::ofstream os(out + ".sol", std::ios::binary);
             os.write(reinterpret_cast<const char*>(x.data()), sizeof(double) * x.size());
         }
+        if (!exportPath.empty()) exportGmsh(exportPath, u, "displacement");
         std::printf("{\"app\": \"qs_elasticity\", \"order\": %d, \"elements\": %lld, \"ndof\": %lld, \"form\": \"%s\", "
                     "\"converged\": %d, \"its\": %d, \"outer\": %d, \"residual\": %.6e, \"setup_s\": %.3f, "
                     "\"assemble_s\": %.4f, \"solve_s\": %.4f}\n",

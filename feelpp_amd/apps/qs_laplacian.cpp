@@ -17,6 +17,7 @@ using namespace feelhip;
 int main(int argc, char** argv) {
     int dim = 3, n = 8, order = 2, seed = 2024;
     double jitter = 0.15, rtol = 1e-10, mass = 0.0;
+    std::string exportPath;
     std::string kappa = "lognormal", source = "one", precision = "fp64", ontype = "elimination_symmetric", out, mesh_path;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
@@ -34,6 +35,7 @@ int main(int argc, char** argv) {
         else if (a == "--on.type") ontype = next();
         else if (a == "--mesh") mesh_path = next();
         else if (a == "--out") out = next();
+        else if (a == "--export") exportPath = next();
         else { std::fprintf(stderr, "unknown option %s\n", a.c_str()); return 2; }
     }
     try {
@@ -65,6 +67,7 @@ int main(int argc, char** argv) {
             std::ofstream os(out + ".sol", std::ios::binary);
             os.write(reinterpret_cast<const char*>(x.data()), sizeof(double) * x.size());
         }
+        if (!exportPath.empty()) exportGmsh(exportPath, u, "u");
         double nrm = 0;
         for (double xi : x) nrm += xi * xi;
         // post-solve evaluation: normL2 and H1 seminorm of u; with k = 1 and the

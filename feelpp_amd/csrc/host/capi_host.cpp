@@ -70,6 +70,23 @@ int feel_mesh_write_msh22(const char* path, int dim, int64_t nv, const double* c
     HOST_CATCH
 }
 
+int feel_mesh_write_msh22_solution(const char* path, int dim, int64_t nv, const double* coords, int64_t ne,
+                                   const int32_t* elems, int order, int ncomp, const int32_t* elem2dof,
+                                   const double* values, const char* name, double time, int step) {
+    HOST_TRY
+    if (!path || !coords || !elems || !elem2dof || !values || ncomp < 1 || ncomp > 3)
+        return (int)fail("bad argument");
+    if (order < 1 || order > maxSupportedOrder(dim)) return (int)fail("unsupported order");
+    std::ofstream os(path);
+    if (!os) return (int)fail(std::string("cannot open ") + path);
+    SimplexMesh m = view(dim, nv, coords, ne, elems);
+    writeMsh22(m, os);
+    writeMsh22ElementNodeData(m, nLocalDof(dim, order) * ncomp, ncomp, elem2dof, values, name ? name : "u", time, step,
+                              os);
+    return os.good() ? 0 : (int)fail("write failed");
+    HOST_CATCH
+}
+
 int feel_mesh_read_msh22(const char* path, int dim, int64_t* nv, double* coords, int64_t* ne, int32_t* elems) {
     HOST_TRY
     std::ifstream is(path);

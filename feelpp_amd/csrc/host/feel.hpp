@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "feelpp_hip.h"
+#include "feelpp_host.h"
 #include "fe.hpp"
 #include "space.hpp"
 
@@ -181,6 +182,18 @@ inline double normSemiH1(ElementsRange const&, Element const& u, std::vector<fee
     double out = 0;
     check(feel_hip_norm_semi_h1(u.functionSpace()->handle(), u.vec(), f.empty() ? nullptr : f.data(), quad, &out));
     return out;
+}
+
+// Gmsh export of a field: the mesh and an $ElementNodeData block with the
+// values at element vertices (feelfilters/exportergmsh_impl.hpp:1210-1324)
+inline void exportGmsh(std::string const& path, Element const& u, std::string const& name = "u", double time = 0.0,
+                       int step = 0) {
+    auto Vh = u.functionSpace();
+    auto const& m = Vh->mesh()->m;
+    auto const& dt = Vh->dofTable();
+    const auto vals = u.values();
+    check(feel_mesh_write_msh22_solution(path.c_str(), m.dim, m.nv, m.coords.data(), m.ne, m.elems.data(), dt.order,
+                                         dt.ncomp, dt.elem2dof.data(), vals.data(), name.c_str(), time, step));
 }
 
 template <int K>

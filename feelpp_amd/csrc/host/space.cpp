@@ -175,6 +175,22 @@ void writeMsh22(const SimplexMesh& m, std::ostream& os) {
     os << "$EndElements\n";
 }
 
+void writeMsh22ElementNodeData(const SimplexMesh& m, int nldof, int ncomp, const int32_t* elem2dof,
+                               const double* values, const std::string& name, double time, int step,
+                               std::ostream& os) {
+    const int nvert = m.dim + 1, nloc = nldof / ncomp;
+    os.precision(17);
+    os << "$ElementNodeData\n1\n\"" << name << "\"\n1\n" << time << "\n3\n" << step << "\n" << ncomp << "\n" << m.ne
+       << "\n";
+    for (int64_t e = 0; e < m.ne; ++e) {
+        os << e + 1 << ' ' << nvert;
+        for (int v = 0; v < nvert; ++v)
+            for (int c = 0; c < ncomp; ++c) os << ' ' << values[elem2dof[e * nldof + c * nloc + v]];
+        os << '\n';
+    }
+    os << "$EndElementNodeData\n";
+}
+
 SimplexMesh readMsh22(std::istream& is, int dim) {
     SimplexMesh m;
     m.dim = dim;

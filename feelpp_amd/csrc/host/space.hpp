@@ -27,6 +27,13 @@ SimplexMesh kuhnBoxMesh(int nx, int ny, int nz, double lx, double ly, double lz,
 // element id = insertion order of elements of the mesh dimension (:2631,:2851),
 // point id = file node id (:2050; kept as an order-preserving relabelling).
 void writeMsh22(const SimplexMesh& m, std::ostream& os);
+// $ElementNodeData block of a Pk field (ncomp components, local dof nloc*c + l)
+// with the values at each element's vertices, as Feel++'s Gmsh exporter writes
+// nodal fields (feelfilters/exportergmsh_impl.hpp:1210-1324); elements are
+// numbered 1..ne as writeMsh22 writes them.
+void writeMsh22ElementNodeData(const SimplexMesh& m, int nldof, int ncomp, const int32_t* elem2dof,
+                               const double* values, const std::string& name, double time, int step,
+                               std::ostream& os);
 SimplexMesh readMsh22(std::istream& is, int dim);
 
 // Per-element random fields

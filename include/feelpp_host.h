@@ -22,6 +22,12 @@ int feel_mesh_kuhn_box(int nx, int ny, int nz, double lx, double ly, double lz, 
                        double* coords, int32_t* elems);
 /* Gmsh MSH 2.2 writer/reader with Feel++ import semantics (feelfilters/importergmsh.hpp:2040-2066,2851-2883) */
 int feel_mesh_write_msh22(const char* path, int dim, int64_t nv, const double* coords, int64_t ne, const int32_t* elems);
+/* mesh followed by an $ElementNodeData block of a Pk field (values at each element's
+ * vertices, ncomp per vertex), the layout of Feel++'s Gmsh exporter
+ * (feelfilters/exportergmsh_impl.hpp:1210-1324) */
+int feel_mesh_write_msh22_solution(const char* path, int dim, int64_t nv, const double* coords, int64_t ne,
+                                   const int32_t* elems, int order, int ncomp, const int32_t* elem2dof,
+                                   const double* values, const char* name, double time, int step);
 /* returns nv and ne through pointers; call with null arrays first to size them */
 int feel_mesh_read_msh22(const char* path, int dim, int64_t* nv, double* coords, int64_t* ne, int32_t* elems);
 

@@ -8,6 +8,8 @@ import subprocess
 import numpy as np
 import pytest
 
+import feelpp_amd as fa
+
 from helpers import rel_fro
 
 pytestmark = pytest.mark.gpu
@@ -88,3 +90,20 @@ def test_qs_elasticity_beam(orc, tmp_path, order):
     orc.dirichlet(1, ia, ja, val, b, isb)
     xo, _, _, reason = orc.pcg(ia, ja, val, b, rtol=1e-14, maxit=100000)
     assert rel_fro(x, xo) < 1e-10
+
+
+def test_qs_laplacian_gmsh_export(orc, tmp_path):
+    # --export writes the mesh and the solution's vertex values as Feel++'s Gmsh
+    # exporter lays them out; they match the raw solution at the vertex dofs
+    path = str(tmp_path / "u.msh")
+    info, x = run("feelpp_qs_laplacian", ["--dim", 3, "--n", 4, "--order", 2, "--seed", 5, "--jitter", 0.15,
+                                          "--rtol", 1e-12, "--export", path], tmp_path)
+    coords, elems = fa.read_msh22(path, 3)
+    e2d, ndof = orc.doftable(3, 2, 1, elems)
+    assert ndof == x.shape[0]
+    text = open(path).read()
+    block = text[text.index("$ElementNodeData"):text.index("$EndElementNodeData")].split("\n")[1:-1]
+    assert block[1] == '"u"' and int(block[7]) == len(elems)
+    for e, line in enumerate(block[8:]):
+        vals = np.array([float(t) for t in line.split()[2:]])
+        assert np.array_equal(vals, x[e2d[e, :4]])

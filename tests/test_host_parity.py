@@ -146,3 +146,30 @@ def test_cg_condition_estimate(orc):
     assert fa.cg_condition_estimate(gam[:6], pap[:5]) <= kappa * (1 + 1e-12)
     with pytest.raises(RuntimeError):
         fa.cg_condition_estimate([1.0, -1.0, 1.0], [1.0, 1.0])
+
+
+@pytest.mark.parametrize("dim,order,ncomp", [(2, 1, 1), (2, 2, 1), (3, 2, 1), (3, 3, 3)])
+def test_msh22_solution_export(orc, tmp_path, dim, order, ncomp):
+    # Feel++'s Gmsh exporter layout: the mesh, then $ElementNodeData with one
+    # line per element "id nvert v..." (values at the element's vertices)
+    if dim == 2:
+        coords, elems = fa.mesh_unit_square(4, jitter=0.1, seed=2)
+    else:
+        coords, elems = fa.mesh_kuhn_box(2, jitter=0.1, seed=2)
+    e2d, ndof = fa.doftable(dim, order, ncomp, elems, coords.shape[0])
+    vals = np.cos(np.arange(ndof) * 0.7)
+    path = str(tmp_path / "sol.msh")
+    fa.write_msh22_solution(path, coords, elems, order, e2d, vals, ncomp=ncomp, name="u", time=0.5, step=3)
+    c2, e2 = fa.read_msh22(path, dim)             # the mesh part still reads back
+    assert np.array_equal(e2, elems) and np.allclose(c2, coords, rtol=0, atol=1e-15)
+    text = open(path).read()
+    block = text[text.index("$ElementNodeData"):text.index("$EndElementNodeData")].split("\n")[1:-1]
+    assert block[0] == "1" and block[1] == '"u"' and float(block[3]) == 0.5
+    assert int(block[5]) == 3 and int(block[6]) == ncomp and int(block[7]) == len(elems)
+    nloc = e2d.shape[1] // ncomp
+    for e, line in enumerate(block[8:]):
+        f = line.split()
+        assert int(f[0]) == e + 1 and int(f[1]) == dim + 1
+        got = np.array([float(t) for t in f[2:]]).reshape(dim + 1, ncomp)
+        want = np.array([[vals[e2d[e, c * nloc + v]] for c in range(ncomp)] for v in range(dim + 1)])
+        assert np.array_equal(got, want)
