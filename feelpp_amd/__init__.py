@@ -115,6 +115,10 @@ _SIGS = {
     "feel_hip_vec_get": (C.c_int, [_vp, _f64p]),
     "feel_hip_vec_zero": (C.c_int, [_vp]),
     "feel_hip_vec_dot": (C.c_int, [_vp, _vp, C.POINTER(C.c_double)]),
+    "feel_hip_vec_axpy": (C.c_int, [_vp, C.c_double, _vp]),
+    "feel_hip_vec_scale": (C.c_int, [_vp, C.c_double]),
+    "feel_hip_vec_copy": (C.c_int, [_vp, _vp]),
+    "feel_hip_vec_set_rows": (C.c_int, [_vp, _vp, _vp]),
     "feel_hip_vec_size": (C.c_int64, [_vp]),
     "feel_hip_vec_device_ptr": (_vp, [_vp]),
     "feel_hip_assemble_bilinear": (C.c_int, [_vp, C.POINTER(BilinearDesc), C.c_int]),
@@ -378,6 +382,20 @@ class Vector(_Handle):
         out = C.c_double()
         _check(lib().feel_hip_vec_dot(self.h, other.h, C.byref(out)))
         return out.value
+
+    def axpy(self, a, x):
+        """self += a x (Vector::add(a, v))"""
+        _check(lib().feel_hip_vec_axpy(self.h, a, x.h))
+
+    def scale(self, a):
+        _check(lib().feel_hip_vec_scale(self.h, a))
+
+    def copy_from(self, src):
+        _check(lib().feel_hip_vec_copy(self.h, src.h))
+
+    def set_rows(self, rows, values=None):
+        """self[r] = values[r] (0 without values) for the rows of a Rows set"""
+        _check(lib().feel_hip_vec_set_rows(self.h, rows.h, values.h if values else None))
 
 
 class Space(_Handle):

@@ -719,6 +719,39 @@ int feel_hip_vec_zero(feel_hip_vec* v) {
     return FEEL_HIP_OK;
 }
 
+int feel_hip_vec_axpy(feel_hip_vec* y, double a, const feel_hip_vec* x) {
+    if (!y || !x || y->n != x->n) return setError(FEEL_HIP_EINVAL, "bad argument");
+    feel_hip_ctx* c = y->ctx;
+    if (y->n) hipLaunchKernelGGL(kAxpby, dim3(gridFor(y->n)), dim3(kBlock), 0, c->stream, y->n, a, x->d, 1.0, y->d);
+    FEEL_HIP_CHECK(hipGetLastError());
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_scale(feel_hip_vec* v, double a) {
+    if (!v) return setError(FEEL_HIP_EINVAL, "null vector");
+    feel_hip_ctx* c = v->ctx;
+    if (v->n) hipLaunchKernelGGL(kAxpby, dim3(gridFor(v->n)), dim3(kBlock), 0, c->stream, v->n, a, v->d, 0.0, v->d);
+    FEEL_HIP_CHECK(hipGetLastError());
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_copy(feel_hip_vec* dst, const feel_hip_vec* src) {
+    if (!dst || !src || dst->n != src->n) return setError(FEEL_HIP_EINVAL, "bad argument");
+    if (dst->n)
+        FEEL_HIP_CHECK(hipMemcpyAsync(dst->d, src->d, sizeof(double) * dst->n, hipMemcpyDeviceToDevice, dst->ctx->stream));
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_vec_set_rows(feel_hip_vec* v, const feel_hip_rows* rows, const feel_hip_vec* values) {
+    if (!v || !rows || rows->n != v->n || (values && values->n != v->n)) return setError(FEEL_HIP_EINVAL, "bad argument");
+    feel_hip_ctx* c = v->ctx;
+    if (rows->nrows)
+        hipLaunchKernelGGL(kSetRows, dim3(gridFor(rows->nrows)), dim3(kBlock), 0, c->stream, rows->nrows, rows->rows,
+                           values ? values->d : nullptr, v->d);
+    FEEL_HIP_CHECK(hipGetLastError());
+    return FEEL_HIP_OK;
+}
+
 int feel_hip_vec_dot(const feel_hip_vec* a, const feel_hip_vec* b, double* out) {
     if (!a || !b || !out || a->n != b->n) return setError(FEEL_HIP_EINVAL, "bad argument");
     feel_hip_ctx* c = a->ctx;

@@ -675,6 +675,20 @@ static __global__ __launch_bounds__(kBlock) void kCastF32(int64_t n, const doubl
         b[i] = (float)a[i];
 }
 
+// y = a x + b y (b = 0: y = a x)
+static __global__ __launch_bounds__(kBlock) void kAxpby(int64_t n, double a, const double* __restrict__ x, double b,
+                                                 double* __restrict__ y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
+}
+
+// v[r] = values ? values[r] : 0 for the listed rows
+static __global__ __launch_bounds__(kBlock) void kSetRows(int64_t nrows, const int32_t* __restrict__ rows,
+                                                   const double* __restrict__ values, double* __restrict__ v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows; i += (int64_t)gridDim.x * blockDim.x)
+        v[rows[i]] = values ? values[rows[i]] : 0.0;
+}
+
 static __global__ __launch_bounds__(kBlock) void kDotPartial(int64_t n, const double* __restrict__ a,
                                                       const double* __restrict__ b, double* __restrict__ part) {
     double s = 0.0;

@@ -169,6 +169,12 @@ int feel_hip_vec_set(feel_hip_vec* v, const double* host);                      
 int feel_hip_vec_get(const feel_hip_vec* v, double* host);                      /* device -> host */
 int feel_hip_vec_zero(feel_hip_vec* v);
 int feel_hip_vec_dot(const feel_hip_vec* a, const feel_hip_vec* b, double* out); /* dot(), vector.hpp:544 */
+int feel_hip_vec_axpy(feel_hip_vec* y, double a, const feel_hip_vec* x);          /* add(a, v), vector.hpp:474 */
+int feel_hip_vec_scale(feel_hip_vec* v, double a);                                /* scale(a), vector.hpp:576 */
+int feel_hip_vec_copy(feel_hip_vec* dst, const feel_hip_vec* src);                /* operator=, vector.hpp:271 */
+/* v[r] = values[r] (0 when values is null) for the rows of `rows`: the rhs part of
+ * on() re-applied to a new right-hand side (feelvf/on.hpp:457-701) */
+int feel_hip_vec_set_rows(feel_hip_vec* v, const feel_hip_rows* rows, const feel_hip_vec* values);
 int64_t feel_hip_vec_size(const feel_hip_vec* v);
 /* raw device pointer (double*) of the vector's storage */
 void* feel_hip_vec_device_ptr(feel_hip_vec* v);

@@ -107,3 +107,32 @@ def test_qs_laplacian_gmsh_export(orc, tmp_path):
     for e, line in enumerate(block[8:]):
         vals = np.array([float(t) for t in line.split()[2:]])
         assert np.array_equal(vals, x[e2d[e, :4]])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_heat_toolbox_backward_euler(orc, tmp_path, precision):
+    # BDF1 heat steps as the heat toolbox assembles them, against the same
+    # steps on the oracle's matrices with a direct solver
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    dim, n, order, dt, steps, q = 2, 12, 2, 0.02, 5, 3.0
+    info, x = run("feelpp_heat", ["--dim", dim, "--n", n, "--order", order, "--seed", 4, "--jitter", 0.1, "--dt", dt,
+                                  "--steps", steps, "--q", q, "--rtol", 1e-12, "--precision", precision], tmp_path)
+    assert info["converged"] == 1 and info["form"] == "mass+diffusion"
+    coords, elems = orc.mesh_square(n, jitter=0.1, seed=4)
+    e2d, ndof = orc.doftable(dim, order, 1, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    va = orc.assemble_scalar(dim, order, 2 * order, coords, elems, e2d, ia, ja, diff=1.0, mass=1.0 / dt)
+    vm = orc.assemble_scalar(dim, order, 2 * order, coords, elems, e2d, ia, ja, diff=0.0, mass=1.0)
+    F = orc.assemble_rhs(dim, order, 1, order, coords, elems, e2d, ndof, orc.FK_CONST, [q])
+    isb = orc.boundary_dofs(dim, order, 1, coords, elems, e2d, ndof)
+    orc.dirichlet(1, ia, ja, va, np.zeros(ndof), isb)
+    A = sp.csr_matrix((va, ja, ia), shape=(ndof, ndof)).tocsc()
+    M = sp.csr_matrix((vm, ja, ia), shape=(ndof, ndof))
+    T = np.zeros(ndof)
+    for _ in range(steps):
+        b = F + M @ T / dt
+        b[isb.astype(bool)] = 0.0
+        T = spla.spsolve(A, b)
+    assert rel_fro(x, T) < (1e-9 if precision == "fp64" else 1e-8)
+    assert abs(info["norm_l2"] - orc.norm(dim, order, 1, 2 * order, coords, elems, e2d, T)) < 1e-8 * info["norm_l2"]
