@@ -202,7 +202,8 @@ def bench_batched(fa, args):
     keys = list(fams)
     draws = [(keys[rng.integers(len(keys))], np.exp(rng.uniform(np.log(0.1), np.log(10.0), 4)))
              for _ in range(args.problems)]
-    B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol or 1e-2, precision=fa.FP32, use_graph=True)
+    B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol or 1e-2, precision=fa.FP32, use_graph=True,
+                 group_spmv=not args.no_group)
     ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
     run_batch(fa, B, fams, ids, draws[:min(8, len(draws))], args.chunk)      # warm-up
     lat, stats, wall = run_batch(fa, B, fams, ids, draws, args.chunk)
@@ -355,6 +356,7 @@ def bench_serving(fa, args):
         draws = [(keys[rng.integers(len(keys))], np.exp(rng.uniform(np.log(0.1), np.log(10.0), 4)))
                  for _ in range(nreq)]
         B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol or 1e-2, precision=fa.FP32, use_graph=True,
+                     group_spmv=not args.no_group,
                      hbm_budget=int(args.hbm_gib * 2**30))
         ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
         run_batch(fa, B, fams, ids, draws[:4], args.chunk)   # warm-up (graph capture, allocations)
@@ -399,6 +401,8 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="cubes per side of the Kuhn mesh")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--no-group", action="store_true",
+                    help="batched/serving: one SpMV per instance instead of one per family group")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="residual reduction between FP64 reliable updates of the FP32 solves (default 1e-2)")
     ap.add_argument("--forms", default="fp64,mixed")

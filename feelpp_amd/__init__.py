@@ -57,7 +57,7 @@ class SolveData(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("inner_rtol", C.c_double), ("maxit", C.c_int),
                 ("inner_maxit", C.c_int), ("max_outer", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("group_spmv", C.c_int)]
 
 
 # exported symbols of the two C-ABIs: name -> (restype, argtypes)
@@ -537,11 +537,13 @@ class Batch(_Handle):
     _destroy = "feel_hip_batch_destroy"
 
     def __init__(self, ctx, rtol=1e-8, inner_rtol=1e-2, precision=FP32, hbm_budget=0, use_graph=True,
-                 maxit=100000, max_outer=50):
+                 maxit=100000, max_outer=50, group_spmv=None):
         o = BatchOpts()
         lib().feel_hip_batch_opts_default(C.byref(o))
         o.rtol, o.inner_rtol, o.precision, o.hbm_budget, o.use_graph = rtol, inner_rtol, precision, hbm_budget, int(
             use_graph)
+        if group_spmv is not None:
+            o.group_spmv = int(group_spmv)
         o.maxit, o.max_outer = maxit, max_outer
         h = _vp()
         _check(lib().feel_hip_batch_create(ctx.h, C.byref(o), C.byref(h)))

@@ -49,8 +49,9 @@ struct BInst {
     void *valT, *dinvT, *xT, *rT, *zT, *pT, *wT, *bT;
     CgState* st;
     double* slots;             // beta[2], dpi[2]
-    double* part;              // per-instance partials: z.r | z.z | p.Ap, nitems each
+    double* part;              // per-instance partials: z.r | z.z | p.Ap (nitems each) | grouped p.Ap
     int nitems;
+    int apOff, nap;            // p.Ap partials of the current SpMV: part[apOff .. apOff + nap)
     int start;                 // global iteration at which the current inner solve began
     int ibeg, iend;            // item range of this instance in the work list
     double scale;              // FP32 residual = scale * r64 (scale = 1 / sigma, fixed per request)
@@ -284,6 +285,182 @@ __global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items
     if (threadIdx.x == 0) I.part[2 * I.nitems + blockIdx.x - I.ibeg] = dot;
 }
 
+constexpr int kGroupMax = 8;
+constexpr int kGroupUnroll = 4;
+constexpr int kGroupBlock = 256;   // 4 wavefronts
+constexpr int kXcds = 8;           // accelerator complex dies, each with its own L2
+constexpr int kGroupSlices = 16;   // slices per grouped-SpMV block (4 per wavefront)
+struct BGroup {
+    int ninst, nterms, compact, nitems;
+    int64_t n;
+    int inst[kGroupMax];                 // positions in the launch's instance array
+    float mu[kGroupMax][kMaxTerms];
+    const float* terms[kMaxTerms];       // family terms, FP32 SELL values
+    const uint8_t* mask;
+    SellCols cols;
+};
+struct BGItem {
+    int group, s0, s1, item;   // item: index of the block's p.Ap partial (s0 / kGroupSlices)
+};
+
+// U consecutive slots of one slice row: byte offset of the column in p, unit
+// diagonal of a Dirichlet row, and the term values (zero on Dirichlet slots)
+template <int NT, int U>
+struct GroupSlots {
+    int32_t off[U];
+    float e[U];
+    float a[U][NT];
+};
+
+template <int NT, int U, bool NARROW>
+__device__ inline void loadSlots(const BGroup& G, size_t slot0, const void* cp, int32_t cb, int k0,
+                                 GroupSlots<NT, U>& d) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t o = (size_t)(k0 + u) * 64;
+        const int32_t col = NARROW ? cb + (int32_t)__builtin_nontemporal_load(static_cast<const uint16_t*>(cp) + o)
+                                   : __builtin_nontemporal_load(static_cast<const int32_t*>(cp) + o);
+        d.off[u] = col * (int32_t)sizeof(float);
+        const uint8_t m = G.mask[slot0 + o];
+        d.e[u] = m == 2 ? 1.0f : 0.0f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const float v = __builtin_nontemporal_load(G.terms[t] + slot0 + o);
+            d.a[u][t] = m == 0 ? v : 0.0f;
+        }
+    }
+}
+
+// acc_b += A(mu_b) p_b on the U slots for every active instance b of the
+// group; p_b is read through a buffer resource so the gathers share the
+// per-slot offsets
+template <int NT, int U>
+__device__ inline void applySlots(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* R,
+                                  const GroupSlots<NT, U>& d, float* acc) {
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) {
+        if ((amask >> b) & 1u) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                float v = d.e[u];
+#pragma unroll
+                for (int t = 0; t < NT; ++t) v += G.mu[b][t] * d.a[u][t];
+                const float x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(R[b], d.off[u], 0, 0));
+                acc[b] += v * x;
+            }
+        }
+    }
+}
+
+// one slice row, the next U slots' matrix data loaded while the current U
+// slots' gathers and products run
+template <int NT, bool NARROW>
+__device__ inline void groupRow(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* P, size_t slot0,
+                                const void* cp, int32_t cb, int width, float* acc) {
+    constexpr int U = kGroupUnroll;
+    int k = 0;
+    if (width >= U) {
+        GroupSlots<NT, U> cur;
+        loadSlots<NT, U, NARROW>(G, slot0, cp, cb, 0, cur);
+        for (; k + 2 * U <= width; k += U) {
+            GroupSlots<NT, U> nxt;
+            loadSlots<NT, U, NARROW>(G, slot0, cp, cb, k + U, nxt);
+            applySlots<NT, U>(G, amask, P, cur, acc);
+            cur = nxt;
+        }
+        applySlots<NT, U>(G, amask, P, cur, acc);
+        k += U;
+    }
+    for (; k < width; ++k) {
+        GroupSlots<NT, 1> one;
+        loadSlots<NT, 1, NARROW>(G, slot0, cp, cb, k, one);
+        applySlots<NT, 1>(G, amask, P, one, acc);
+    }
+}
+
+// Grouped SpMV of an affine family (FP32 inner iterations): the in-flight
+// instances of one family share the sparsity pattern and the term matrices, so
+// a group streams each slot's FP32 term values, Dirichlet mask and column once
+// and every instance b combines a = sum_t mu_bt a_t on the fly (mask 1: zero,
+// mask 2: unit diagonal) before w_b += a * p_b[col].  Matrix traffic per
+// iteration drops from ninst x (value + column) to (T values + mask + column).
+// Each instance's arithmetic is independent of the others in the group, so a
+// result is the same alone and in any mix.
+template <int NT>
+__global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restrict__ gitems,
+                                                     const BGroup* __restrict__ groups,
+                                                     const BInst* __restrict__ insts) {
+    const BGItem it = gitems[blockIdx.x];
+    if (it.group < 0) return;
+    const BGroup& G = groups[it.group];
+    __shared__ unsigned actMask;
+    __shared__ double red[kGroupBlock / 64][kGroupMax];
+    if (threadIdx.x < 64) {
+        const int b = threadIdx.x;
+        const bool on = b < G.ninst && !insts[G.inst[b < G.ninst ? b : 0]].st->done;
+        const unsigned long long bal = __ballot(on);
+        if (threadIdx.x == 0) actMask = (unsigned)bal;
+    }
+    __syncthreads();
+    const unsigned amask = __builtin_amdgcn_readfirstlane(actMask);
+    if (!amask) return;
+    __amdgpu_buffer_rsrc_t R[kGroupMax];
+    const int32_t bytes = (int32_t)(G.n * (int64_t)sizeof(float));
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) {
+        const bool on = (amask >> b) & 1u;
+        // raw buffer resource over p_b: base, stride 0, byte extent, gfx9 data format word
+        R[b] = __builtin_amdgcn_make_buffer_rsrc(on ? insts[G.inst[b]].pT : nullptr, (short)0, bytes, 0x00020000);
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double dot[kGroupMax];
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) dot[b] = 0.0;
+    for (int sl = it.s0 + wv; sl < it.s1; sl += kGroupBlock / 64) {
+        const uint32_t base = G.cols.slicePtr[sl];
+        const int width = (int)((G.cols.slicePtr[sl + 1] - base) >> 6);
+        const size_t slot0 = base + lane;
+        float acc[kGroupMax];
+#pragma unroll
+        for (int b = 0; b < kGroupMax; ++b) acc[b] = 0.0f;
+        const int32_t cb = G.compact ? G.cols.colBase[sl] : -1;
+        if (cb >= 0)
+            groupRow<NT, true>(G, amask, R, slot0, G.cols.packed + G.cols.cptr[sl] + lane, cb, width, acc);
+        else if (G.compact)
+            groupRow<NT, false>(G, amask, R, slot0,
+                                reinterpret_cast<const int32_t*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb, width, acc);
+        else
+            groupRow<NT, false>(G, amask, R, slot0, G.cols.col + slot0, cb, width, acc);
+        const int64_t r = (int64_t)sl * 64 + lane;
+        if (r < G.n) {
+#pragma unroll
+            for (int b = 0; b < kGroupMax; ++b)
+                if ((amask >> b) & 1u) {
+                    tp<float>(insts[G.inst[b]].wT)[r] = acc[b];
+                    const float pr = __builtin_bit_cast(
+                        float, __builtin_amdgcn_raw_buffer_load_b32(R[b], (int32_t)(r * (int64_t)sizeof(float)), 0, 0));
+                    dot[b] += (double)pr * (double)acc[b];
+                }
+        }
+    }
+    // per-instance p.Ap partial of this item: wave sums, then the waves in order
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) {
+        if ((amask >> b) & 1u) {
+            const double v = waveReduceSum(dot[b]);
+            if (lane == 0) red[wv][b] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kGroupMax && ((amask >> threadIdx.x) & 1u)) {
+        const int b = threadIdx.x;
+        double v = 0.0;
+        for (int w = 0; w < kGroupBlock / 64; ++w) v += red[w][b];
+        const BInst& I = insts[G.inst[b]];
+        I.part[3 * I.nitems + it.item] = v;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ items, const BInst* __restrict__ insts,
                                                   const int* __restrict__ gcount) {
@@ -293,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ ite
     CgState* st = I.st;
     if (instDone(st)) return;
     const int it = *gcount - I.start;
-    const double dpi = instReduce(I.part + 2 * I.nitems, I.nitems);
+    const double dpi = instReduce(I.part + I.apOff, I.nap);
     const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
     if (bid == I.ibeg && threadIdx.x == 0) {
         I.slots[2 + (it & 1)] = dpi;
@@ -337,6 +514,7 @@ struct BatchFamily {
     feel_hip_graph* g = nullptr;
     int nterms = 0;
     std::vector<double*> terms;   // SELL FP64 values per term
+    std::vector<float*> termsF;   // the same rounded to FP32 (grouped FP32 SpMV)
     uint8_t* mask = nullptr;
     double* b64 = nullptr;        // permuted rhs with zero Dirichlet rows
     int64_t n = 0, nslices = 0, nslots = 0;
@@ -386,12 +564,15 @@ struct feel_hip_batch {
     std::vector<int> freeStates;
     BItem* dItems = nullptr;
     BInst* dInsts = nullptr;
+    BGroup* dGroups = nullptr;
+    BGItem* dGItems = nullptr;
+    int capGroups = 0, capGItems = 0, nGItems = 0;
     double* dSums = nullptr;
     int capItems = 0, capInsts = 0;
     int* gcount = nullptr;
     int hgcount = 0;
     hipGraphExec_t exec = nullptr;
-    int execItems = -1, execChunk = -1;
+    int execItems = -1, execChunk = -1, execGItems = -1, execTerms = -1;
     uint64_t execVersion = ~0ULL, version = 0;
 };
 
@@ -404,9 +585,14 @@ int alloc(feel_hip_ctx* c, T** p, size_t count) {
 
 size_t tsize(const feel_hip_batch* B) { return B->opts.precision == FEEL_HIP_FP32 ? 4 : 8; }
 
+// z.r, z.z and p.Ap partials per item, then one p.Ap partial per grouped-SpMV block
+size_t partCount(const BatchFamily& f) {
+    return 3 * (size_t)f.nitems + (size_t)((f.nslices + kGroupSlices - 1) / kGroupSlices);
+}
+
 size_t instanceBytes(const feel_hip_batch* B, const BatchFamily& f) {
     const size_t t = tsize(B);
-    return f.nslots * (8 + t) + f.n * (8 * 5 + t * 7) + sizeof(double) * 3 * f.nitems;
+    return f.nslots * (8 + t) + f.n * (8 * 5 + t * 7) + sizeof(double) * partCount(f);
 }
 
 int allocInstance(feel_hip_batch* B, int fam, int* out) {
@@ -422,7 +608,7 @@ int allocInstance(feel_hip_batch* B, int fam, int* out) {
     TRY_CALL(alloc(c, &ib.val64, f.nslots));
     TRY_CALL(c->alloc(&ib.valT, t * f.nslots));
     for (double** p : {&ib.dinv64, &ib.x64, &ib.r64, &ib.w64, &ib.out}) TRY_CALL(alloc(c, p, f.n));
-    TRY_CALL(alloc(c, &ib.part, 3 * (size_t)f.nitems));
+    TRY_CALL(alloc(c, &ib.part, partCount(f)));
     for (void** p : {&ib.dinvT, &ib.xT, &ib.rT, &ib.zT, &ib.pT, &ib.wT, &ib.bT}) TRY_CALL(c->alloc(p, t * f.n));
     B->bufs.push_back(ib);
     B->used += (int64_t)instanceBytes(B, f);
@@ -453,9 +639,15 @@ int ensureStates(feel_hip_batch* B, int need) {
 }
 
 // Work list (and per-instance descriptors) for a set of requests.
-int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
+// Work list of the requests rs: one BInst per request, its 64-slice items, and
+// with groups set the family groups of the grouped FP32 SpMV (instances of one
+// family in request order, kGroupMax per group; group items ordered by item
+// within a family so the groups of a family stream the same slices together).
+int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, bool groups = false) {
     std::vector<BItem> items;
     std::vector<BInst> insts;
+    std::vector<BGroup> grp;
+    std::vector<BGItem> gitems;
     for (Request* r : rs) {
         const BatchFamily& f = B->fams[r->family];
         const InstBuffers& ib = B->bufs[r->buf];
@@ -478,6 +670,8 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
         I.slots = B->slots + 4 * (size_t)r->state;
         I.part = ib.part;
         I.nitems = f.nitems;
+        I.apOff = groups ? 3 * f.nitems : 2 * f.nitems;
+        I.nap = groups ? (int)((f.nslices + kGroupSlices - 1) / kGroupSlices) : f.nitems;
         I.start = r->start;
         I.scale = r->sigma > 0 ? 1.0 / r->sigma : 0.0;
         I.xscale = r->xscale;
@@ -486,6 +680,47 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
             items.push_back({(int)insts.size(), (int)s, (int)std::min<int64_t>(f.nslices, s + kItemSlices)});
         I.iend = (int)items.size();
         insts.push_back(I);
+    }
+    if (groups) {
+        std::map<int, std::vector<int>> byFamily;
+        for (size_t i = 0; i < rs.size(); ++i) byFamily[rs[i]->family].push_back((int)i);
+        for (auto& kv : byFamily) {
+            const BatchFamily& f = B->fams[kv.first];
+            const std::vector<int>& members = kv.second;
+            const int g0 = (int)grp.size();
+            for (size_t m0 = 0; m0 < members.size(); m0 += kGroupMax) {
+                BGroup G{};
+                G.ninst = (int)std::min<size_t>(kGroupMax, members.size() - m0);
+                G.nterms = f.nterms;
+                G.compact = f.g->compact ? 1 : 0;
+                G.nitems = f.nitems;
+                G.n = f.n;
+                for (int b = 0; b < G.ninst; ++b) {
+                    const int i = members[m0 + b];
+                    G.inst[b] = i;
+                    for (int t = 0; t < f.nterms; ++t) G.mu[b][t] = (float)rs[i]->mu[t];
+                }
+                for (int t = 0; t < kMaxTerms; ++t) G.terms[t] = f.termsF[t < f.nterms ? t : 0];
+                G.mask = f.mask;
+                G.cols = SellCols{f.g->slicePtr, f.g->scol, f.g->colBase, f.g->ccptr, f.g->packed};
+                grp.push_back(G);
+            }
+            for (int g = g0; g < (int)grp.size(); ++g)
+                for (int64_t s = 0; s < f.nslices; s += kGroupSlices)
+                    gitems.push_back({g, (int)s, (int)std::min<int64_t>(f.nslices, s + kGroupSlices),
+                                      (int)(s / kGroupSlices)});
+        }
+        // Workgroups go to the XCDs round robin (block i to XCD i % 8): give each
+        // XCD one contiguous eighth of the (group, slice range) sequence, so the
+        // blocks sharing an L2 gather from neighbouring rows of the same
+        // instances.  Short eighths are padded with empty blocks (group -1).
+        const size_t N = gitems.size(), per = (N + kXcds - 1) / kXcds;
+        std::vector<BGItem> placed(per * kXcds, BGItem{-1, 0, 0, 0});
+        for (int x = 0; x < kXcds; ++x) {
+            const size_t b0 = N * x / kXcds, b1 = N * (x + 1) / kXcds;
+            for (size_t i = b0; i < b1; ++i) placed[(i - b0) * kXcds + x] = gitems[i];
+        }
+        gitems.swap(placed);
     }
     feel_hip_ctx* c = B->ctx;
     if ((int)items.size() > B->capItems) {
@@ -502,6 +737,24 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems) {
         TRY_CALL(alloc(c, &B->dSums, B->capInsts));
         ++B->version;
     }
+    if ((int)grp.size() > B->capGroups) {
+        c->release(B->dGroups, sizeof(BGroup) * B->capGroups);
+        B->capGroups = (int)grp.size() * 2;
+        TRY_CALL(alloc(c, &B->dGroups, B->capGroups));
+        ++B->version;
+    }
+    if ((int)gitems.size() > B->capGItems) {
+        c->release(B->dGItems, sizeof(BGItem) * B->capGItems);
+        B->capGItems = (int)gitems.size() * 2;
+        TRY_CALL(alloc(c, &B->dGItems, B->capGItems));
+        ++B->version;
+    }
+    if (!grp.empty()) {
+        FEEL_HIP_CHECK(hipMemcpyAsync(B->dGroups, grp.data(), sizeof(BGroup) * grp.size(), hipMemcpyHostToDevice, c->stream));
+        FEEL_HIP_CHECK(
+            hipMemcpyAsync(B->dGItems, gitems.data(), sizeof(BGItem) * gitems.size(), hipMemcpyHostToDevice, c->stream));
+    }
+    if (groups) B->nGItems = (int)gitems.size();
     if (!items.empty()) {
         FEEL_HIP_CHECK(hipMemcpyAsync(B->dItems, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice, c->stream));
         FEEL_HIP_CHECK(hipMemcpyAsync(B->dInsts, insts.data(), sizeof(BInst) * insts.size(), hipMemcpyHostToDevice, c->stream));
@@ -523,13 +776,30 @@ int residualNorms(feel_hip_batch* B, const std::vector<Request*>& rs) {
     return FEEL_HIP_OK;
 }
 
-template <typename T>
-int launchIterations(feel_hip_batch* B, int nItems, int chunk) {
+// grouped SpMV over every group of the current work list, instantiated for
+// the largest term count of the list: a family with fewer terms pads them with
+// its first term and mu = 0 (the repeated loads hit the cache lines just read)
+void launchGroupSpmv(feel_hip_batch* B, int nterms) {
     feel_hip_ctx* c = B->ctx;
+    const dim3 grid(B->nGItems), block(kGroupBlock);
+    switch (nterms) {
+#define FEEL_GROUP_CASE(NT) \
+    case NT: hipLaunchKernelGGL(kBSpmvGroup<NT>, grid, block, 0, c->stream, B->dGItems, B->dGroups, B->dInsts); break;
+        FEEL_GROUP_CASE(1) FEEL_GROUP_CASE(2) FEEL_GROUP_CASE(3) FEEL_GROUP_CASE(4)
+        FEEL_GROUP_CASE(5) FEEL_GROUP_CASE(6) FEEL_GROUP_CASE(7) FEEL_GROUP_CASE(8)
+#undef FEEL_GROUP_CASE
+    }
+}
+
+template <typename T>
+int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
+    feel_hip_ctx* c = B->ctx;
+    const bool grouped = sizeof(T) == 4 && nterms > 0;
     auto body = [&]() {
         for (int k = 0; k < chunk; ++k) {
             hipLaunchKernelGGL(kBDirection<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts, B->gcount);
-            hipLaunchKernelGGL(kBSpmv<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+            if (grouped) launchGroupSpmv(B, nterms);
+            else hipLaunchKernelGGL(kBSpmv<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
             hipLaunchKernelGGL(kBUpdate<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts, B->gcount);
             hipLaunchKernelGGL(kBBump, dim3(1), dim3(64), 0, c->stream, B->gcount);
         }
@@ -541,7 +811,9 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk) {
     }
     // the captured chunk reads the work list, descriptors and counter from device
     // memory, so it stays valid while those buffers keep their addresses
-    if (!B->exec || B->execItems != nItems || B->execChunk != chunk || B->execVersion != B->version) {
+    const int gItems = grouped ? B->nGItems : 0;
+    if (!B->exec || B->execItems != nItems || B->execGItems != gItems || B->execTerms != nterms ||
+        B->execChunk != chunk || B->execVersion != B->version) {
         if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
         hipGraph_t graph;
         FEEL_HIP_CHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -550,6 +822,8 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk) {
         FEEL_HIP_CHECK(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0));
         (void)hipGraphDestroy(graph);
         B->execItems = nItems;
+        B->execGItems = gItems;
+        B->execTerms = nterms;
         B->execChunk = chunk;
         B->execVersion = B->version;
     }
@@ -660,8 +934,11 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
     for (auto& kv : B->reqs)
         if (kv.second.phase == Phase::Inner) inner.push_back(&kv.second);
     if (!inner.empty() && iterations > 0) {
-        TRY_CALL(buildWork(B, inner, &nItems));
-        TRY_CALL(launchIterations<T>(B, nItems, iterations));
+        const bool grouped = sizeof(T) == 4 && B->opts.group_spmv;
+        int maxTerms = 0;
+        for (Request* r : inner) maxTerms = std::max(maxTerms, B->fams[r->family].nterms);
+        TRY_CALL(buildWork(B, inner, &nItems, grouped));
+        TRY_CALL(launchIterations<T>(B, nItems, iterations, grouped ? maxTerms : 0));
         B->hgcount += iterations;
         std::vector<CgState> hs(B->nStates);
         FEEL_HIP_CHECK(hipMemcpyAsync(hs.data(), B->states, sizeof(CgState) * B->nStates, hipMemcpyDeviceToHost, c->stream));
@@ -714,6 +991,7 @@ void feel_hip_batch_opts_default(feel_hip_batch_opts* o) {
     o->precision = FEEL_HIP_FP32;
     o->hbm_budget = 0;
     o->use_graph = 1;
+    o->group_spmv = 1;
 }
 
 int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out) {
@@ -747,6 +1025,7 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
     c->bytes -= B->used;   // instance buffers were counted through ctx->alloc
     for (auto& f : B->fams) {
         for (double* p : f.terms) c->release(p, sizeof(double) * f.nslots);
+        for (float* p : f.termsF) c->release(p, sizeof(float) * f.nslots);
         c->release(f.mask, f.nslots);
         c->release(f.b64, sizeof(double) * f.n);
         unref(f.g);
@@ -755,6 +1034,8 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
     c->release(B->slots, sizeof(double) * 4 * B->nStates);
     c->release(B->dItems, sizeof(BItem) * B->capItems);
     c->release(B->dInsts, sizeof(BInst) * B->capInsts);
+    c->release(B->dGroups, sizeof(BGroup) * B->capGroups);
+    c->release(B->dGItems, sizeof(BGItem) * B->capGItems);
     c->release(B->dSums, sizeof(double) * B->capInsts);
     c->release(B->gcount, sizeof(int));
     delete B;
@@ -773,6 +1054,8 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
     }
     if (rhs->n != g->n) return setError(FEEL_HIP_EINVAL, "rhs size mismatch");
     if (dirichlet && dirichlet->n != g->n) return setError(FEEL_HIP_EINVAL, "Dirichlet rows built for another size");
+    if (B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv && g->n > (int64_t)(INT32_MAX / sizeof(float)))
+        return setError(FEEL_HIP_EUNSUPPORTED, "grouped SpMV addresses at most 2^29 rows; set group_spmv = 0");
     feel_hip_ctx* c = B->ctx;
     BatchFamily f;
     f.g = g;
@@ -788,6 +1071,12 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
         hipLaunchKernelGGL(kCsrToSell<double>, dim3(grid), dim3(kBlock), 0, c->stream, f.nslots, g->slot2csr, terms[t]->v64,
                            (const float*)nullptr, v);
         f.terms.push_back(v);
+        if (B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv) {
+            float* vf = nullptr;
+            TRY_CALL(alloc(c, &vf, f.nslots));
+            hipLaunchKernelGGL(kCastF32, dim3(grid), dim3(kBlock), 0, c->stream, f.nslots, v, vf);
+            f.termsF.push_back(vf);
+        }
     }
     uint8_t* zeroFlags = nullptr;
     const uint8_t* flags = dirichlet ? dirichlet->flag : nullptr;

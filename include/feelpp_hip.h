@@ -233,6 +233,7 @@ typedef struct {
     int precision;            /* FEEL_HIP_FP32 (refined) or FEEL_HIP_FP64 */
     int64_t hbm_budget;       /* bytes of instance buffers allowed at once, 0 = unlimited */
     int use_graph;            /* capture each iteration chunk in a hipGraph */
+    int group_spmv;           /* FP32: one SpMV pass per family group, A(mu) formed on the fly */
 } feel_hip_batch_opts;
 void feel_hip_batch_opts_default(feel_hip_batch_opts* o);
 int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out);

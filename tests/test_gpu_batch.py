@@ -107,3 +107,46 @@ def test_admission_by_memory_budget(ctx):
     done = run_all(B, reqs)
     assert set(done) == set(reqs) and all(d[0]["converged"] for d in done.values())
     assert B.device_bytes() <= 2.5 * one
+
+
+def two_term(ctx, F):
+    """A(mu) = mu_0 A_{sub<2} + mu_1 A_{sub>=2} on the family's graph."""
+    terms = []
+    for m in (F["sub"] < 2, F["sub"] >= 2):
+        A = fa.Matrix(F["g"], fa.FP64)
+        A.assemble(fa.DIFFUSION, F["qorder"], kdiff=ctx.vector(F["ne"], m * 1.0))
+        terms.append(A)
+    return terms
+
+
+@pytest.mark.parametrize("group", [True, False])
+def test_groups_of_mixed_term_counts(ctx, group):
+    # eleven instances of a 4-term family (two groups of the grouped SpMV) and
+    # three of a 2-term family on the same graph, whose terms are padded to
+    # four in the grouped pass; every solution against the FP64 solve, and a
+    # 2-term instance bitwise equal to the same request solved alone
+    F = family(ctx, 5, 2)
+    t2 = two_term(ctx, F)
+    B = fa.Batch(ctx, rtol=1e-10, group_spmv=group)
+    f4 = B.add_family(F["terms"], F["b"], F["rows"])
+    f2 = B.add_family(t2, F["b"], F["rows"])
+    rng = np.random.default_rng(7)
+    reqs = {}
+    for k in range(14):
+        if k % 5 == 2:
+            mu = np.exp(rng.uniform(-2, 2, 2))
+            reqs[B.submit(f2, mu)] = np.repeat(mu, 2)
+        else:
+            mu = np.exp(rng.uniform(-2, 2, 4))
+            reqs[B.submit(f4, mu)] = mu
+    done = run_all(B, {r: F["ndof"] for r in reqs}, chunk=9)
+    for rid, mu4 in reqs.items():
+        sd, x = done[rid]
+        assert sd["converged"], sd
+        assert rel_fro(x, reference(ctx, F, mu4)) < 1e-8
+    rid2 = next(r for r, mu4 in reqs.items() if mu4[0] == mu4[1] and mu4[2] == mu4[3])
+    A = fa.Batch(ctx, rtol=1e-10, group_spmv=group)
+    a2 = A.add_family(t2, F["b"], F["rows"])
+    r = A.submit(a2, reqs[rid2][::2])
+    alone = run_all(A, {r: F["ndof"]}, chunk=9)[r][1]
+    assert np.array_equal(alone, done[rid2][1])
