@@ -96,10 +96,15 @@ _SIGS = {
     "feel_hip_ctx_reset_timing": (C.c_int, [_vp]),
     "feel_hip_space_create": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int64, _f64p, C.c_int64, _i32p, _i32p,
                                         C.c_int64, C.POINTER(_vp)]),
+    "feel_hip_space_create_from_mesh": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int64, _f64p, C.c_int64,
+                                                  _i32p, C.POINTER(_vp)]),
+    "feel_hip_space_elem2dof": (C.c_int, [_vp, _i32p]),
     "feel_hip_space_destroy": (C.c_int, [_vp]),
     "feel_hip_space_ndof": (C.c_int64, [_vp]),
     "feel_hip_space_nelements": (C.c_int64, [_vp]),
     "feel_hip_graph_create": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, C.POINTER(_vp)]),
+    "feel_hip_graph_create_from_space": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "feel_hip_graph_pattern": (C.c_int, [_vp, _i32p, _i32p]),
     "feel_hip_graph_destroy": (C.c_int, [_vp]),
     "feel_hip_graph_nnz": (C.c_int64, [_vp]),
     "feel_hip_mat_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
@@ -417,6 +422,27 @@ class Space(_Handle):
         self.coords, self.elems, self.elem2dof, self.ndof = coords, elems, elem2dof, ndof
         self.ne = elems.shape[0]
 
+    @classmethod
+    def from_mesh(cls, ctx, coords, elems, order, ncomp=1):
+        """The DofTable built on the device from the mesh (Feel++'s first-touch
+        numbering); elem2dof is copied back for host-side helpers."""
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        elems = np.ascontiguousarray(elems, dtype=np.int32)
+        dim = coords.shape[1]
+        h = _vp()
+        _check(lib().feel_hip_space_create_from_mesh(ctx.h, dim, order, ncomp, coords.shape[0], coords,
+                                                     elems.shape[0], elems, C.byref(h)))
+        self = cls.__new__(cls)
+        _Handle.__init__(self, h, keep=(ctx,))
+        nloc = (order + 1) * (order + 2) // 2 if dim == 2 else (order + 1) * (order + 2) * (order + 3) // 6
+        e2d = np.empty((elems.shape[0], nloc * ncomp), dtype=np.int32)
+        _check(lib().feel_hip_space_elem2dof(h, e2d))
+        self.ctx, self.dim, self.order, self.ncomp = ctx, dim, order, ncomp
+        self.coords, self.elems, self.elem2dof = coords, elems, e2d
+        self.ndof = int(lib().feel_hip_space_ndof(h))
+        self.ne = elems.shape[0]
+        return self
+
 
 class Graph(_Handle):
     _destroy = "feel_hip_graph_destroy"
@@ -431,6 +457,20 @@ class Graph(_Handle):
         super().__init__(h, keep=(space,))
         self.space, self.ia, self.ja = space, ia, ja
         self.nnz = int(ja.shape[0])
+
+    @classmethod
+    def from_space(cls, space):
+        """The space's COUPLED pattern built on the device; ia/ja copied back."""
+        h = _vp()
+        _check(lib().feel_hip_graph_create_from_space(space.h, C.byref(h)))
+        self = cls.__new__(cls)
+        _Handle.__init__(self, h, keep=(space,))
+        nnz = int(lib().feel_hip_graph_nnz(h))
+        ia = np.empty(space.ndof + 1, dtype=np.int32)
+        ja = np.empty(nnz, dtype=np.int32)
+        _check(lib().feel_hip_graph_pattern(h, ia, ja))
+        self.space, self.ia, self.ja, self.nnz = space, ia, ja, nnz
+        return self
 
 
 class Matrix(_Handle):

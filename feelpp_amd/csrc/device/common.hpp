@@ -36,15 +36,22 @@ constexpr int kBlock = 256;
 constexpr int kSpmvGrid = 2048;       // default grid of the solver SpMV (one p.Ap partial per block)
 constexpr int kSpmvPartCap = 4096;    // largest SpMV grid (partials buffer capacity)
 constexpr int kSpmvBlock = 1024;
-constexpr int kHistMax = 8192;
+constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
 constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
-constexpr int kSegTask = 256;         // scatter entries per task (whole slots)        // CG coefficient history kept for the condition estimate
+constexpr int kSegTask = 256;         // scatter entries per task (whole slots)
 
 }  // namespace feelpp_amd
 
 namespace feelpp_amd {
 // gather map of the scatter for (space, graph), built on the device (gathermap.hip)
 int buildGatherMapDevice(feel_hip_space* s, feel_hip_graph* g);
+// the space's COUPLED pattern (ia/ja) and the gather map from one sort (gathermap.hip)
+int buildGraphDevice(feel_hip_space* s, feel_hip_graph* g);
+// first-touch elem2dof and ndof of the space's mesh; dof incidence (doftable.hip)
+int buildDofTableDevice(feel_hip_space* s);
+int buildIncidenceDevice(feel_hip_space* s);
+// the solver's SELL-C-sigma image of g for the row order perm (sell.hip)
+int buildSellDevice(feel_hip_graph* g, const std::vector<int32_t>& perm);
 void retain(feel_hip_ctx* c);
 void retain(feel_hip_space* s);
 void retain(feel_hip_graph* g);

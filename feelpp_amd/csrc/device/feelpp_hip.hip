@@ -307,6 +307,50 @@ int feel_hip_space_create(feel_hip_ctx* ctx, int dim, int order, int ncomp, int6
     GUARD_END
 }
 
+int feel_hip_space_create_from_mesh(feel_hip_ctx* ctx, int dim, int order, int ncomp, int64_t nverts,
+                                    const double* coords, int64_t nelems, const int32_t* elems,
+                                    feel_hip_space** out) {
+    GUARD_BEGIN
+    if (!ctx || !out || !coords || !elems) return setError(FEEL_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    if (dim < 2 || dim > 3) return setError(FEEL_HIP_EUNSUPPORTED, "dim must be 2 or 3");
+    if (order < 1 || order > maxSupportedOrder(dim))
+        return setError(FEEL_HIP_EUNSUPPORTED, "order " + std::to_string(order) + " not supported in " +
+                                                   std::to_string(dim) + "D");
+    if (ncomp != 1 && ncomp != dim) return setError(FEEL_HIP_EINVAL, "ncomp must be 1 or dim");
+    if (nverts <= 0 || nelems <= 0) return setError(FEEL_HIP_EINVAL, "empty mesh");
+    if (nverts >= (1LL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "more than 2^31 vertices");
+    const int nloc = nLocalDof(dim, order), nldof = nloc * ncomp;
+    for (int64_t k = 0; k < nelems * (dim + 1); ++k)
+        if (elems[k] < 0 || elems[k] >= nverts) return setError(FEEL_HIP_EINVAL, "element vertex index out of range");
+    if ((uint64_t)nelems * nldof >= (1ULL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "space too large");
+    FEEL_HIP_CHECK(hipSetDevice(ctx->device));
+    auto* s = new feel_hip_space();
+    s->ctx = ctx;
+    retain(ctx);
+    s->dim = dim; s->order = order; s->ncomp = ncomp; s->nloc = nloc; s->nldof = nldof;
+    s->nv = nverts; s->ne = nelems;
+    int rc = upload(ctx, &s->coords, coords, (size_t)nverts * dim);
+    if (rc == FEEL_HIP_OK) rc = upload(ctx, &s->elems, elems, (size_t)nelems * (dim + 1));
+    if (rc == FEEL_HIP_OK) rc = buildDofTableDevice(s);
+    if (rc == FEEL_HIP_OK) rc = buildIncidenceDevice(s);
+    if (rc == FEEL_HIP_OK) rc = devAlloc(ctx, &s->elemvec, (size_t)nelems * nldof);
+    if (rc != FEEL_HIP_OK) {
+        feel_hip_space_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return FEEL_HIP_OK;
+    GUARD_END
+}
+
+int feel_hip_space_elem2dof(const feel_hip_space* s, int32_t* elem2dof) {
+    if (!s || !elem2dof) return setError(FEEL_HIP_EINVAL, "null argument");
+    FEEL_HIP_CHECK(hipStreamSynchronize(s->ctx->stream));
+    FEEL_HIP_CHECK(hipMemcpy(elem2dof, s->e2d, sizeof(int32_t) * s->ne * s->nldof, hipMemcpyDeviceToHost));
+    return FEEL_HIP_OK;
+}
+
 int feel_hip_space_destroy(feel_hip_space* s) {
     if (s) unref(s);
     return FEEL_HIP_OK;
@@ -429,9 +473,38 @@ int ensureGeo(feel_hip_space* s, size_t bytes) {
 
 }  // namespace
 
+// ============================================================== graph
+namespace {
+
+// diagonal positions (from the host copy of the pattern, rows with their
+// diagonal) and the solver's SELL-C-sigma image (row order on the host, the
+// image on the device)
+int finishGraph(feel_hip_graph* g) {
+    feel_hip_ctx* ctx = g->space->ctx;
+    const int64_t nrows = g->n;
+    const int32_t* ia = g->hia.data();
+    const int32_t* ja = g->hja.data();
+    std::vector<int32_t> diag(nrows, -1);
+    for (int64_t r = 0; r < nrows; ++r) {
+        const int32_t* b = ja + ia[r];
+        const int32_t* e = ja + ia[r + 1];
+        const int32_t* p = std::lower_bound(b, e, (int32_t)r);
+        if (p == e || *p != r) return setError(FEEL_HIP_EPATTERN, "graph row without its diagonal entry");
+        diag[r] = (int32_t)(p - ja);
+    }
+    TRY_CALL(upload(ctx, &g->diagpos, diag.data(), diag.size()));
+    GraphCSR gc;   // borrows the host pattern for the row ordering
+    gc.nrows = nrows; gc.nnz = g->nnz; gc.ia.swap(g->hia); gc.ja.swap(g->hja);
+    std::vector<int32_t> perm = sellPermutation(gc, 1024);
+    gc.ia.swap(g->hia); gc.ja.swap(g->hja);
+    TRY_CALL(buildSellDevice(g, perm));
+    return FEEL_HIP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
-// ============================================================== graph
 int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, const int32_t* ja,
                           feel_hip_graph** out) {
     GUARD_BEGIN
@@ -455,41 +528,11 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
     g->nnz = ia[nrows];
     g->hia.assign(ia, ia + nrows + 1);
     g->hja.assign(ja, ja + g->nnz);
-    std::vector<int32_t> diag(nrows, -1);
-    for (int64_t r = 0; r < nrows; ++r) {
-        const int32_t* b = ja + ia[r];
-        const int32_t* e = ja + ia[r + 1];
-        const int32_t* p = std::lower_bound(b, e, (int32_t)r);
-        if (p == e || *p != r) {
-            delete g;
-            return setError(FEEL_HIP_EPATTERN, "graph row without its diagonal entry");
-        }
-        diag[r] = (int32_t)(p - ja);
-    }
-    GraphCSR gc;
-    gc.nrows = nrows; gc.nnz = g->nnz; gc.ia = g->hia; gc.ja = g->hja;
     int rc = upload(ctx, &g->ia, ia, (size_t)nrows + 1);
     if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->ja, ja, (size_t)g->nnz);
-    if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->diagpos, diag.data(), diag.size());
     // gather map of the scatter, built on the device from elem2dof and ia/ja
     if (rc == FEEL_HIP_OK) rc = buildGatherMapDevice(s, g);
-    if (rc == FEEL_HIP_OK) {
-        SellLayout L = buildSell(gc, 1024);
-        g->nslices = L.nslices;
-        g->nslots = L.nslots;
-        rc = upload(ctx, &g->slicePtr, L.slicePtr.data(), L.slicePtr.size());
-        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->scol, L.col.data(), L.col.size());
-        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->slot2csr, L.slot2csr.data(), L.slot2csr.size());
-        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->perm, L.perm.data(), L.perm.size());
-        if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->sdiag, L.diagSlot.data(), L.diagSlot.size());
-        if (rc == FEEL_HIP_OK && L.compact) {
-            g->compact = true;
-            g->npacked = (int64_t)L.packed.size();
-            rc = upload(ctx, &g->colBase, L.colBase.data(), L.colBase.size());
-            if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->ccptr, L.cptr.data(), L.cptr.size());
-            if (rc == FEEL_HIP_OK) rc = upload(ctx, &g->packed, L.packed.data(), L.packed.size());
-        }
-    }
+    if (rc == FEEL_HIP_OK) rc = finishGraph(g);
     if (rc != FEEL_HIP_OK) {
         feel_hip_graph_destroy(g);
         return rc;
@@ -497,6 +540,40 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
     *out = g;
     return FEEL_HIP_OK;
     GUARD_END
+}
+
+int feel_hip_graph_create_from_space(feel_hip_space* s, feel_hip_graph** out) {
+    GUARD_BEGIN
+    if (!s || !out) return setError(FEEL_HIP_EINVAL, "null argument");
+    *out = nullptr;
+    feel_hip_ctx* ctx = s->ctx;
+    FEEL_HIP_CHECK(hipSetDevice(ctx->device));
+    auto* g = new feel_hip_graph();
+    g->space = s;
+    retain(s);
+    int rc = buildGraphDevice(s, g);
+    if (rc == FEEL_HIP_OK) {
+        g->hia.resize(g->n + 1);
+        g->hja.resize(g->nnz);
+        if (hipMemcpy(g->hia.data(), g->ia, sizeof(int32_t) * (g->n + 1), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(g->hja.data(), g->ja, sizeof(int32_t) * g->nnz, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = setError(FEEL_HIP_EDEVICE, "download of the device-built pattern failed");
+    }
+    if (rc == FEEL_HIP_OK) rc = finishGraph(g);
+    if (rc != FEEL_HIP_OK) {
+        feel_hip_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return FEEL_HIP_OK;
+    GUARD_END
+}
+
+int feel_hip_graph_pattern(const feel_hip_graph* g, int32_t* ia, int32_t* ja) {
+    if (!g || !ia || !ja) return setError(FEEL_HIP_EINVAL, "null argument");
+    std::copy(g->hia.begin(), g->hia.end(), ia);
+    std::copy(g->hja.begin(), g->hja.end(), ja);
+    return FEEL_HIP_OK;
 }
 
 int feel_hip_graph_destroy(feel_hip_graph* g) {

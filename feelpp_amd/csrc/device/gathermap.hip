@@ -90,59 +90,71 @@ __global__ __launch_bounds__(kGmBlock) void kTasks(int64_t nseg, const uint32_t*
     }
 }
 
+// (row << 32 | col) key and entry id of every element-matrix entry
+__global__ __launch_bounds__(kGmBlock) void kPairKeys(int64_t tot, int nl, const int32_t* __restrict__ e2d,
+                                                     uint64_t* __restrict__ key, uint32_t* __restrict__ val) {
+    const int64_t nn = (int64_t)nl * nl;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < tot; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = k / nn;
+        const int ij = (int)(k - e * nn);
+        const int i = ij / nl, j = ij - i * nl;
+        key[k] = ((uint64_t)(uint32_t)e2d[e * nl + i] << 32) | (uint32_t)e2d[e * nl + j];
+        val[k] = (uint32_t)k;
+    }
+}
+
+__global__ __launch_bounds__(kGmBlock) void kHeads(int64_t tot, const uint64_t* __restrict__ key,
+                                                  uint32_t* __restrict__ head) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x)
+        head[q] = (q == 0 || key[q] != key[q - 1]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kGmBlock) void kDecrement(int64_t n, uint32_t* __restrict__ a) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        a[q] -= 1u;
+}
+
+// ja[slot] = col of every distinct key; ia[row] = slot of the row's first key
+// (every row holds its diagonal, so no row is empty); ia[n] = nnz
+__global__ __launch_bounds__(kGmBlock) void kPattern(int64_t tot, const uint64_t* __restrict__ key,
+                                                    const uint32_t* __restrict__ head,
+                                                    const uint32_t* __restrict__ slot, int64_t n, int32_t nnz,
+                                                    int32_t* __restrict__ ia, int32_t* __restrict__ ja) {
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < tot; q += (int64_t)gridDim.x * blockDim.x) {
+        if (q == 0) ia[n] = nnz;
+        if (!head[q]) continue;
+        const uint64_t k = key[q];
+        ja[slot[q]] = (int32_t)(uint32_t)k;
+        if (q == 0 || (key[q - 1] >> 32) != (k >> 32)) ia[k >> 32] = (int32_t)slot[q];
+    }
+}
+
 #define GM_CHECK(expr)                                                                                      \
     do {                                                                                                    \
         hipError_t e_ = (expr);                                                                             \
         if (e_ != hipSuccess) { rc = setError(FEEL_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); goto done; } \
     } while (0)
 
-}  // namespace
 
-namespace feelpp_amd {
-
-// Fills g->centry, g->ncontrib, g->taskSeg, g->taskK, g->ntasks, g->segSlot,
-// g->nseg from the space's element-to-dof map and the graph's ia/ja (device).
-int buildGatherMapDevice(feel_hip_space* s, feel_hip_graph* g) {
+// Gather map tail: from the entries sorted by slot (key = slot, ascending,
+// entries in ascending element-entry order within a slot) to the end flags,
+// the segment -> slot map, the scatter tasks and g->centry.  spareA/spareB:
+// two scratch buffers of tot uint32; tmp/tmpBytes: the sort's temp storage,
+// grown here when a scan needs more.
+int gatherTail(feel_hip_space* s, feel_hip_graph* g, int64_t tot, const uint32_t* key, uint32_t* entry,
+               uint32_t* spareA, uint32_t* spareB, void*& tmp, size_t& tmpBytes) {
     feel_hip_ctx* c = s->ctx;
     hipStream_t st = c->stream;
-    const int nl = s->nldof;
-    const int64_t tot = s->ne * (int64_t)nl * nl;
-    if (tot >= (1LL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "space too large for 31-bit element-entry ids");
-    if (g->nnz >= (1LL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "graph too large");
     const uint32_t nnz = (uint32_t)g->nnz;
-    int bits = 1;
-    while (bits < 32 && (1ull << bits) <= nnz) ++bits;
     int rc = FEEL_HIP_OK;
-    uint32_t *keyA = nullptr, *keyB = nullptr, *valA = nullptr, *valB = nullptr, *flag = nullptr, *idx = nullptr;
     uint32_t *segEnd = nullptr, *cut = nullptr;
-    void* tmp = nullptr;
-    size_t tmpBytes = 0, need = 0;
-    uint32_t lastKey = 0, hv[2] = {0, 0};
+    uint32_t hv[2] = {0, 0};
     int64_t nseg = 0, ntasks = 0;
-    hipcub::DoubleBuffer<uint32_t> keys, vals;
-    GM_CHECK(hipMalloc(&keyA, sizeof(uint32_t) * tot));
-    GM_CHECK(hipMalloc(&keyB, sizeof(uint32_t) * tot));
-    GM_CHECK(hipMalloc(&valA, sizeof(uint32_t) * tot));
-    GM_CHECK(hipMalloc(&valB, sizeof(uint32_t) * tot));
-    hipLaunchKernelGGL(kEntrySlot, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, nl, s->e2d, g->ia, g->ja, nnz, keyA,
-                       valA);
+    size_t need = 0;
+    uint32_t* flag = spareA;
+    uint32_t* idx = spareB;
+    hipLaunchKernelGGL(kMarkEnds, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, key, entry, flag);
     GM_CHECK(hipGetLastError());
-    keys = hipcub::DoubleBuffer<uint32_t>(keyA, keyB);
-    vals = hipcub::DoubleBuffer<uint32_t>(valA, valB);
-    // radix sort is stable: equal slots keep ascending entry (element) order
-    GM_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys, vals, (int)tot, 0, bits, st));
-    tmpBytes = need;
-    GM_CHECK(hipMalloc(&tmp, tmpBytes));
-    GM_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, vals, (int)tot, 0, bits, st));
-    GM_CHECK(hipMemcpyAsync(&lastKey, keys.Current() + (tot - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    GM_CHECK(hipStreamSynchronize(st));
-    if (lastKey >= nnz) { rc = setError(FEEL_HIP_EPATTERN, "element entry outside the sparsity graph"); goto done; }
-    // the sorted values become the gather map entries; the other value buffer holds the end flags
-    flag = vals.Alternate();
-    hipLaunchKernelGGL(kMarkEnds, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, keys.Current(), vals.Current(), flag);
-    GM_CHECK(hipGetLastError());
-    idx = keys.Alternate();
-    need = 0;
     GM_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, flag, idx, (int)tot, st));
     if (need > tmpBytes) {
         GM_CHECK(hipFree(tmp));
@@ -161,8 +173,7 @@ int buildGatherMapDevice(feel_hip_space* s, feel_hip_graph* g) {
         rc = c->alloc((void**)&g->segSlot, sizeof(int32_t) * nseg);
         if (rc != FEEL_HIP_OK) goto done;
     }
-    hipLaunchKernelGGL(kSegments, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, keys.Current(), flag, idx, segEnd,
-                       g->segSlot);
+    hipLaunchKernelGGL(kSegments, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, key, flag, idx, segEnd, g->segSlot);
     GM_CHECK(hipGetLastError());
     // tasks: cut flags over segments, their exclusive scan, task arrays
     GM_CHECK(hipMalloc(&cut, sizeof(uint32_t) * nseg));
@@ -190,10 +201,131 @@ int buildGatherMapDevice(feel_hip_space* s, feel_hip_graph* g) {
     g->ncontrib = tot;
     rc = c->alloc((void**)&g->centry, sizeof(uint32_t) * tot);
     if (rc != FEEL_HIP_OK) goto done;
-    GM_CHECK(hipMemcpyAsync(g->centry, vals.Current(), sizeof(uint32_t) * tot, hipMemcpyDeviceToDevice, st));
+    GM_CHECK(hipMemcpyAsync(g->centry, entry, sizeof(uint32_t) * tot, hipMemcpyDeviceToDevice, st));
     GM_CHECK(hipStreamSynchronize(st));
 done:
-    for (void* p : {(void*)keyA, (void*)keyB, (void*)valA, (void*)valB, (void*)segEnd, (void*)cut, tmp})
+    for (void* p : {(void*)segEnd, (void*)cut})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+}  // namespace
+
+namespace feelpp_amd {
+
+// Fills g->centry, g->ncontrib, g->taskSeg, g->taskK, g->ntasks, g->segSlot,
+// g->nseg from the space's element-to-dof map and the graph's ia/ja (device).
+int buildGatherMapDevice(feel_hip_space* s, feel_hip_graph* g) {
+    feel_hip_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    const int nl = s->nldof;
+    const int64_t tot = s->ne * (int64_t)nl * nl;
+    if (tot >= (1LL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "space too large for 31-bit element-entry ids");
+    if (g->nnz >= (1LL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "graph too large");
+    const uint32_t nnz = (uint32_t)g->nnz;
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) <= nnz) ++bits;
+    int rc = FEEL_HIP_OK;
+    uint32_t *keyA = nullptr, *keyB = nullptr, *valA = nullptr, *valB = nullptr;
+    void* tmp = nullptr;
+    size_t tmpBytes = 0, need = 0;
+    uint32_t lastKey = 0;
+    hipcub::DoubleBuffer<uint32_t> keys, vals;
+    GM_CHECK(hipMalloc(&keyA, sizeof(uint32_t) * tot));
+    GM_CHECK(hipMalloc(&keyB, sizeof(uint32_t) * tot));
+    GM_CHECK(hipMalloc(&valA, sizeof(uint32_t) * tot));
+    GM_CHECK(hipMalloc(&valB, sizeof(uint32_t) * tot));
+    hipLaunchKernelGGL(kEntrySlot, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, nl, s->e2d, g->ia, g->ja, nnz, keyA,
+                       valA);
+    GM_CHECK(hipGetLastError());
+    keys = hipcub::DoubleBuffer<uint32_t>(keyA, keyB);
+    vals = hipcub::DoubleBuffer<uint32_t>(valA, valB);
+    // radix sort is stable: equal slots keep ascending entry (element) order
+    GM_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys, vals, (int)tot, 0, bits, st));
+    tmpBytes = need;
+    GM_CHECK(hipMalloc(&tmp, tmpBytes));
+    GM_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, vals, (int)tot, 0, bits, st));
+    GM_CHECK(hipMemcpyAsync(&lastKey, keys.Current() + (tot - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    GM_CHECK(hipStreamSynchronize(st));
+    if (lastKey >= nnz) { rc = setError(FEEL_HIP_EPATTERN, "element entry outside the sparsity graph"); goto done; }
+    rc = gatherTail(s, g, tot, keys.Current(), vals.Current(), vals.Alternate(), keys.Alternate(), tmp, tmpBytes);
+done:
+    for (void* p : {(void*)keyA, (void*)keyB, (void*)valA, (void*)valB, tmp})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+// The COUPLED sparsity pattern of the space (Stencil::computeGraph,
+// feeldiscr/stencil.hpp:1502-1692; GraphCSR::close, feelalg/graphcsr.cpp:663-1021):
+// row i holds the union of the dofs of the elements containing dof i, columns
+// ascending.  Every element-matrix entry k = e*nldof^2 + i*nldof + j becomes
+// the 64-bit key (row << 32 | col); one stable radix sort of (key, k) gives
+// the pattern (the distinct keys, in row-major order) and, with the slot of a
+// sorted entry = the number of distinct keys before it, the gather map of the
+// scatter as buildGatherMapDevice would build it from that pattern.
+// Fills g->n, g->nnz, g->ia, g->ja (device) and the gather map fields.
+int buildGraphDevice(feel_hip_space* s, feel_hip_graph* g) {
+    feel_hip_ctx* c = s->ctx;
+    hipStream_t st = c->stream;
+    const int nl = s->nldof;
+    const int64_t tot = s->ne * (int64_t)nl * nl;
+    if (tot >= (1LL << 31)) return setError(FEEL_HIP_EUNSUPPORTED, "space too large for 31-bit element-entry ids");
+    int rbits = 1;
+    while ((1LL << rbits) < s->ndof) ++rbits;
+    g->n = s->ndof;
+    int rc = FEEL_HIP_OK;
+    uint64_t *keyA = nullptr, *keyB = nullptr;
+    uint32_t *valA = nullptr, *valB = nullptr, *head = nullptr, *slot = nullptr;
+    void* tmp = nullptr;
+    size_t tmpBytes = 0, need = 0;
+    uint32_t hv[2] = {0, 0};
+    hipcub::DoubleBuffer<uint64_t> keys;
+    hipcub::DoubleBuffer<uint32_t> vals;
+    GM_CHECK(hipMalloc(&keyA, sizeof(uint64_t) * tot));
+    GM_CHECK(hipMalloc(&keyB, sizeof(uint64_t) * tot));
+    GM_CHECK(hipMalloc(&valA, sizeof(uint32_t) * tot));
+    GM_CHECK(hipMalloc(&valB, sizeof(uint32_t) * tot));
+    hipLaunchKernelGGL(kPairKeys, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, nl, s->e2d, keyA, valA);
+    GM_CHECK(hipGetLastError());
+    keys = hipcub::DoubleBuffer<uint64_t>(keyA, keyB);
+    vals = hipcub::DoubleBuffer<uint32_t>(valA, valB);
+    GM_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys, vals, (int)tot, 0, 32 + rbits, st));
+    tmpBytes = need;
+    GM_CHECK(hipMalloc(&tmp, tmpBytes));
+    GM_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, vals, (int)tot, 0, 32 + rbits, st));
+    // distinct keys: head flags; slot of a sorted entry = (heads up to and including it) - 1
+    head = vals.Alternate();
+    slot = reinterpret_cast<uint32_t*>(keys.Alternate());
+    hipLaunchKernelGGL(kHeads, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, keys.Current(), head);
+    GM_CHECK(hipGetLastError());
+    need = 0;
+    GM_CHECK(hipcub::DeviceScan::InclusiveSum(nullptr, need, head, slot, (int)tot, st));
+    if (need > tmpBytes) {
+        GM_CHECK(hipFree(tmp));
+        tmp = nullptr;
+        tmpBytes = need;
+        GM_CHECK(hipMalloc(&tmp, tmpBytes));
+    }
+    GM_CHECK(hipcub::DeviceScan::InclusiveSum(tmp, tmpBytes, head, slot, (int)tot, st));
+    GM_CHECK(hipMemcpyAsync(&hv[0], slot + (tot - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(kDecrement, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, slot);
+    GM_CHECK(hipGetLastError());
+    GM_CHECK(hipStreamSynchronize(st));
+    g->nnz = (int64_t)hv[0];
+    rc = c->alloc((void**)&g->ia, sizeof(int32_t) * (g->n + 1));
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->ja, sizeof(int32_t) * g->nnz);
+    if (rc != FEEL_HIP_OK) goto done;
+    hipLaunchKernelGGL(kPattern, dim3(gmGrid(tot)), dim3(kGmBlock), 0, st, tot, keys.Current(), head, slot, g->n,
+                       (int32_t)g->nnz, g->ia, g->ja);
+    GM_CHECK(hipGetLastError());
+    // the head buffer is free again: the gather map tail needs two scratch buffers
+    {
+        // the sorted 64-bit keys are no longer needed: their buffer holds the slot keys' scratch
+        uint32_t* spare = reinterpret_cast<uint32_t*>(keys.Current());
+        rc = gatherTail(s, g, tot, slot, vals.Current(), head, spare, tmp, tmpBytes);
+    }
+done:
+    for (void* p : {(void*)keyA, (void*)keyB, (void*)valA, (void*)valB, tmp})
         if (p) (void)hipFree(p);
     return rc;
 }

@@ -1,0 +1,241 @@
+// Device-side construction of the solver's SELL-C-sigma image (C = 64 rows
+// per slice) from the graph's device ia/ja and a row order computed on the
+// host (reverse Cuthill-McKee, then rows sorted by length inside windows of
+// sigma rows).  Per slice: width = longest row; slot (s, k, lane) holds the
+// lane's k-th entry (permuted column, CSR position) or, past the row's end, a
+// padding entry pointing at the row itself with no CSR position.  The packed
+// column stream stores a slice whose columns span fewer than 2^16 as 16-bit
+// offsets from its smallest column (colBase), any other slice as int32 pairs
+// of uint16 (colBase = -1); it is kept when at least a quarter of the slots
+// are narrow.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+using namespace feelpp_amd;
+
+namespace {
+
+constexpr int kSlBlock = 256;
+constexpr int kC = 64;
+
+int slGrid(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kSlBlock - 1) / kSlBlock, 65536)); }
+
+__global__ __launch_bounds__(kSlBlock) void kInversePerm(int64_t n, const int32_t* __restrict__ perm,
+                                                        int32_t* __restrict__ iperm) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        iperm[perm[r]] = (int32_t)r;
+}
+
+// slots of every slice (width x 64), one wavefront per slice
+__global__ __launch_bounds__(kSlBlock) void kSliceSlots(int64_t n, int64_t nslices, const int32_t* __restrict__ ia,
+                                                       const int32_t* __restrict__ perm, uint64_t* __restrict__ slots) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
+        const int64_t r = s * kC + lane;
+        int len = 0;
+        if (r < n) len = ia[perm[r] + 1] - ia[perm[r]];
+        for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+        if (lane == 0) slots[s] = (uint64_t)len * kC;
+    }
+}
+
+__global__ __launch_bounds__(kSlBlock) void kSlicePtr(int64_t m, const uint64_t* __restrict__ off,
+                                                     uint32_t* __restrict__ ptr) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < m; s += (int64_t)gridDim.x * blockDim.x)
+        ptr[s] = (uint32_t)off[s];
+}
+
+// one thread per (slice, lane) row of the image
+__global__ __launch_bounds__(kSlBlock) void kSellFill(int64_t n, int64_t nrowsPadded, const int32_t* __restrict__ ia,
+                                                     const int32_t* __restrict__ ja, const int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ iperm,
+                                                     const uint32_t* __restrict__ slicePtr, int32_t* __restrict__ col,
+                                                     int32_t* __restrict__ slot2csr, int32_t* __restrict__ diagSlot) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrowsPadded;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = r / kC;
+        const int lane = (int)(r - s * kC);
+        const uint32_t b = slicePtr[s];
+        const int width = (int)((slicePtr[s + 1] - b) / kC);
+        const int64_t self = r < n ? r : n - 1;   // padding rows point inside the slice's range
+        const int32_t old = r < n ? perm[r] : 0;
+        const int32_t p0 = r < n ? ia[old] : 0;
+        const int len = r < n ? ia[old + 1] - p0 : 0;
+        for (int k = 0; k < width; ++k) {
+            const int64_t slot = b + (int64_t)k * kC + lane;
+            if (k < len) {
+                const int32_t c = ja[p0 + k];
+                col[slot] = iperm[c];
+                slot2csr[slot] = p0 + k;
+                if (c == old) diagSlot[r] = (int32_t)slot;
+            } else {
+                col[slot] = (int32_t)self;
+                slot2csr[slot] = -1;
+            }
+        }
+    }
+}
+
+// per slice: colBase (smallest column, or -1 when the span needs 32 bits),
+// packed units and narrow slot count; one wavefront per slice
+__global__ __launch_bounds__(kSlBlock) void kSliceSpan(int64_t nslices, const uint32_t* __restrict__ slicePtr,
+                                                      const int32_t* __restrict__ col, int32_t* __restrict__ colBase,
+                                                      int64_t* __restrict__ units, int64_t* __restrict__ narrow) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
+        const uint32_t b = slicePtr[s], e = slicePtr[s + 1];
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (uint32_t q = b + lane; q < e; q += kC) { lo = min(lo, col[q]); hi = max(hi, col[q]); }
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+        }
+        if (lane == 0) {
+            const int64_t w = (int64_t)e - b;
+            const bool nar = w == 0 || (int64_t)hi - lo < 65536;
+            colBase[s] = w == 0 ? 0 : (nar ? lo : -1);
+            units[s] = nar ? w : 2 * w;
+            narrow[s] = nar ? w : 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSlBlock) void kPackFill(int64_t nslices, const uint32_t* __restrict__ slicePtr,
+                                                     const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ colBase,
+                                                     const int64_t* __restrict__ cptr, uint16_t* __restrict__ packed) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
+        const uint32_t b = slicePtr[s], e = slicePtr[s + 1];
+        uint16_t* out = packed + cptr[s];
+        const int32_t cb = colBase[s];
+        if (cb >= 0) {
+            for (uint32_t q = b + lane; q < e; q += kC) out[q - b] = (uint16_t)(col[q] - cb);
+        } else {
+            int32_t* out32 = reinterpret_cast<int32_t*>(out);   // cptr is a multiple of 64 units
+            for (uint32_t q = b + lane; q < e; q += kC) out32[q - b] = col[q];
+        }
+    }
+}
+
+#define SL_CHECK(expr)                                                                                      \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess) { rc = setError(FEEL_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); goto done; } \
+    } while (0)
+
+}  // namespace
+
+namespace feelpp_amd {
+
+// Fills g->perm, slicePtr, scol, slot2csr, sdiag, nslices, nslots and, when
+// worth it, the packed column stream, from g->ia/g->ja (device) and the host
+// row order perm (new -> old).
+int buildSellDevice(feel_hip_graph* g, const std::vector<int32_t>& perm) {
+    feel_hip_ctx* c = g->space->ctx;
+    hipStream_t st = c->stream;
+    const int64_t n = g->n;
+    const int64_t nslices = (n + kC - 1) / kC;
+    int rc = FEEL_HIP_OK;
+    int32_t* iperm = nullptr;
+    uint64_t *cnt = nullptr, *off = nullptr;
+    int64_t *units = nullptr, *uoff = nullptr, *narrow = nullptr, *sums = nullptr;
+    void* tmp = nullptr;
+    size_t need = 0, tmpBytes = 0;
+    uint64_t total = 0;
+    int64_t hs[2] = {0, 0};
+    g->nslices = nslices;
+    rc = c->alloc((void**)&g->perm, sizeof(int32_t) * n);
+    if (rc != FEEL_HIP_OK) return rc;
+    SL_CHECK(hipMemcpyAsync(g->perm, perm.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, st));
+    SL_CHECK(hipMalloc(&iperm, sizeof(int32_t) * n));
+    hipLaunchKernelGGL(kInversePerm, dim3(slGrid(n)), dim3(kSlBlock), 0, st, n, g->perm, iperm);
+    // slice offsets: exclusive scan of the slice slot counts (64-bit, checked below 2^32)
+    SL_CHECK(hipMalloc(&cnt, sizeof(uint64_t) * (nslices + 1)));
+    SL_CHECK(hipMalloc(&off, sizeof(uint64_t) * (nslices + 1)));
+    SL_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint64_t) * (nslices + 1), st));
+    hipLaunchKernelGGL(kSliceSlots, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, n, nslices, g->ia, g->perm, cnt);
+    SL_CHECK(hipGetLastError());
+    SL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, (int)(nslices + 1), st));
+    tmpBytes = need;
+    SL_CHECK(hipMalloc(&tmp, tmpBytes));
+    SL_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmpBytes, cnt, off, (int)(nslices + 1), st));
+    SL_CHECK(hipMemcpyAsync(&total, off + nslices, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SL_CHECK(hipStreamSynchronize(st));
+    if (total >= (1ULL << 32)) { rc = setError(FEEL_HIP_EUNSUPPORTED, "SELL image exceeds 2^32 slots"); goto done; }
+    g->nslots = (int64_t)total;
+    rc = c->alloc((void**)&g->slicePtr, sizeof(uint32_t) * (nslices + 1));
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->scol, sizeof(int32_t) * g->nslots);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->slot2csr, sizeof(int32_t) * g->nslots);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->sdiag, sizeof(int32_t) * n);
+    if (rc != FEEL_HIP_OK) goto done;
+    hipLaunchKernelGGL(kSlicePtr, dim3(slGrid(nslices + 1)), dim3(kSlBlock), 0, st, nslices + 1, off, g->slicePtr);
+    hipLaunchKernelGGL(kSellFill, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, n, nslices * kC, g->ia, g->ja,
+                       g->perm, iperm, g->slicePtr, g->scol, g->slot2csr, g->sdiag);
+    SL_CHECK(hipGetLastError());
+    // packed column stream
+    SL_CHECK(hipMalloc(&units, sizeof(int64_t) * (nslices + 1)));
+    SL_CHECK(hipMalloc(&uoff, sizeof(int64_t) * (nslices + 1)));
+    SL_CHECK(hipMalloc(&narrow, sizeof(int64_t) * nslices));
+    SL_CHECK(hipMalloc(&sums, sizeof(int64_t)));
+    SL_CHECK(hipMemsetAsync(units + nslices, 0, sizeof(int64_t), st));
+    rc = c->alloc((void**)&g->colBase, sizeof(int32_t) * nslices);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->ccptr, sizeof(int64_t) * nslices);
+    if (rc != FEEL_HIP_OK) goto done;
+    hipLaunchKernelGGL(kSliceSpan, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nslices, g->slicePtr, g->scol,
+                       g->colBase, units, narrow);
+    SL_CHECK(hipGetLastError());
+    need = 0;
+    SL_CHECK(hipcub::DeviceReduce::Sum(nullptr, need, narrow, sums, (int)nslices, st));
+    if (need > tmpBytes) {
+        SL_CHECK(hipFree(tmp));
+        tmp = nullptr;
+        tmpBytes = need;
+        SL_CHECK(hipMalloc(&tmp, tmpBytes));
+    }
+    SL_CHECK(hipcub::DeviceReduce::Sum(tmp, tmpBytes, narrow, sums, (int)nslices, st));
+    need = 0;
+    SL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, units, uoff, (int)(nslices + 1), st));
+    if (need > tmpBytes) {
+        SL_CHECK(hipFree(tmp));
+        tmp = nullptr;
+        tmpBytes = need;
+        SL_CHECK(hipMalloc(&tmp, tmpBytes));
+    }
+    SL_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tmpBytes, units, uoff, (int)(nslices + 1), st));
+    SL_CHECK(hipMemcpyAsync(&hs[0], sums, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SL_CHECK(hipMemcpyAsync(&hs[1], uoff + nslices, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    SL_CHECK(hipMemcpyAsync(g->ccptr, uoff, sizeof(int64_t) * nslices, hipMemcpyDeviceToDevice, st));
+    SL_CHECK(hipStreamSynchronize(st));
+    // below a quarter of narrow slots the packed stream saves too little to pay
+    // for the second copy of the columns
+    if (g->nslots > 0 && (double)hs[0] / (double)g->nslots >= 0.25) {
+        g->npacked = hs[1];
+        rc = c->alloc((void**)&g->packed, sizeof(uint16_t) * g->npacked);
+        if (rc != FEEL_HIP_OK) goto done;
+        g->compact = true;
+        hipLaunchKernelGGL(kPackFill, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nslices, g->slicePtr, g->scol,
+                           g->colBase, g->ccptr, g->packed);
+        SL_CHECK(hipGetLastError());
+    } else {
+        c->release(g->colBase, sizeof(int32_t) * nslices);
+        c->release(g->ccptr, sizeof(int64_t) * nslices);
+        g->colBase = nullptr;
+        g->ccptr = nullptr;
+    }
+    SL_CHECK(hipStreamSynchronize(st));
+done:
+    for (void* p : {(void*)iperm, (void*)cnt, (void*)off, (void*)units, (void*)uoff, (void*)narrow, (void*)sums, tmp})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+}  // namespace feelpp_amd

@@ -502,98 +502,19 @@ std::vector<int32_t> reverseCuthillMcKee(const GraphCSR& g) {
     return order;
 }
 
-SellLayout buildSell(const GraphCSR& g, int window, bool reorder) {
-    SellLayout L;
+std::vector<int32_t> sellPermutation(const GraphCSR& g, int window, bool reorder) {
     const int64_t n = g.nrows;
-    const int C = SellLayout::C;
-    L.nrows = n;
-    L.perm.resize(n);
-    L.iperm.resize(n);
-    std::vector<int32_t> base(n);
-    if (reorder) base = reverseCuthillMcKee(g);
+    std::vector<int32_t> perm(n);
+    if (reorder) perm = reverseCuthillMcKee(g);
     else
-        for (int64_t r = 0; r < n; ++r) base[r] = (int32_t)r;
+        for (int64_t r = 0; r < n; ++r) perm[r] = (int32_t)r;
     for (int64_t w0 = 0; w0 < n; w0 += window) {
         const int64_t w1 = std::min<int64_t>(n, w0 + window);
-        for (int64_t r = w0; r < w1; ++r) L.perm[r] = base[r];
-        std::stable_sort(L.perm.begin() + w0, L.perm.begin() + w1, [&](int32_t a, int32_t b) {
+        std::stable_sort(perm.begin() + w0, perm.begin() + w1, [&](int32_t a, int32_t b) {
             return g.ia[a + 1] - g.ia[a] > g.ia[b + 1] - g.ia[b];
         });
     }
-    for (int64_t r = 0; r < n; ++r) L.iperm[L.perm[r]] = (int32_t)r;
-    L.nslices = (n + C - 1) / C;
-    L.slicePtr.assign(L.nslices + 1, 0);
-    uint64_t acc = 0;
-    for (int64_t s = 0; s < L.nslices; ++s) {
-        int width = 0;
-        for (int64_t r = s * C; r < std::min<int64_t>(n, (s + 1) * C); ++r)
-            width = std::max(width, g.ia[L.perm[r] + 1] - g.ia[L.perm[r]]);
-        L.slicePtr[s] = (uint32_t)acc;
-        acc += (uint64_t)width * C;
-        if (acc >= (1ULL << 32)) throw std::overflow_error("SELL image exceeds 2^32 slots");
-    }
-    L.slicePtr[L.nslices] = (uint32_t)acc;
-    L.nslots = (int64_t)acc;
-    L.col.assign(acc, 0);
-    L.slot2csr.assign(acc, -1);
-    L.diagSlot.assign(n, -1);
-    for (int64_t s = 0; s < L.nslices; ++s) {
-        const int width = (int)((L.slicePtr[s + 1] - L.slicePtr[s]) / C);
-        for (int lane = 0; lane < C; ++lane) {
-            const int64_t r = s * C + lane;
-            const int64_t self = r < n ? r : n - 1;   // padding rows point inside the slice's range
-            const int32_t old = r < n ? L.perm[r] : 0;
-            const int len = r < n ? g.ia[old + 1] - g.ia[old] : 0;
-            for (int k = 0; k < width; ++k) {
-                const int64_t slot = L.slicePtr[s] + (int64_t)k * C + lane;
-                if (k < len) {
-                    const int64_t p = g.ia[old] + k;
-                    L.col[slot] = L.iperm[g.ja[p]];
-                    L.slot2csr[slot] = (int32_t)p;
-                    if (g.ja[p] == old) L.diagSlot[r] = (int32_t)slot;
-                } else {
-                    L.col[slot] = (int32_t)self;
-                }
-            }
-        }
-    }
-    // packed columns: 16-bit offsets for slices spanning fewer than 2^16 columns
-    L.colBase.assign(L.nslices, -1);
-    L.cptr.assign(L.nslices, 0);
-    int64_t narrow = 0, units = 0;
-    for (int64_t s = 0; s < L.nslices; ++s) {
-        int32_t lo = INT32_MAX, hi = INT32_MIN;
-        for (uint32_t q = L.slicePtr[s]; q < L.slicePtr[s + 1]; ++q) { lo = std::min(lo, L.col[q]); hi = std::max(hi, L.col[q]); }
-        const int64_t w = L.slicePtr[s + 1] - L.slicePtr[s];
-        L.cptr[s] = units;
-        if (w == 0 || (int64_t)hi - lo < 65536) {
-            L.colBase[s] = w == 0 ? 0 : lo;
-            narrow += w;
-            units += w;
-        } else {
-            units += 2 * w;
-        }
-    }
-    L.compactFraction = L.nslots ? (double)narrow / (double)L.nslots : 1.0;
-    // below a quarter of narrow slots the packed stream saves too little to pay
-    // for the second copy of the columns
-    if (L.compactFraction >= 0.25) {
-        L.compact = true;
-        L.packed.resize(units);
-        for (int64_t s = 0; s < L.nslices; ++s) {
-            uint16_t* out = L.packed.data() + L.cptr[s];
-            const uint32_t b = L.slicePtr[s];
-            if (L.colBase[s] >= 0) {
-                for (uint32_t q = b; q < L.slicePtr[s + 1]; ++q) out[q - b] = (uint16_t)(L.col[q] - L.colBase[s]);
-            } else {
-                std::memcpy(out, L.col.data() + b, sizeof(int32_t) * (L.slicePtr[s + 1] - b));
-            }
-        }
-    } else {
-        L.colBase.clear();
-        L.cptr.clear();
-    }
-    return L;
+    return perm;
 }
 
 }  // namespace feelpp_amd

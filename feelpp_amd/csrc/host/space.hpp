@@ -73,27 +73,12 @@ Incidence buildIncidence(const DofTable& dt);
 // slot = slicePtr[s] + k*64 + lane.  Columns are renumbered with iperm so the
 // solver works entirely in the permuted numbering; padding slots point at the
 // row itself with a zero value (slot2csr = -1).
-struct SellLayout {
-    static constexpr int C = 64;
-    int64_t nrows = 0, nslices = 0, nslots = 0;
-    std::vector<uint32_t> slicePtr;   // nslices + 1
-    std::vector<int32_t> col;         // nslots, permuted column
-    // Packed column stream: slice s starts at packed[cptr[s]] (uint16 units).
-    // A slice whose columns span fewer than 2^16 stores 16-bit offsets from
-    // colBase[s] >= 0; any other slice (colBase[s] = -1) stores its int32
-    // columns as pairs of uint16 (cptr is always even, so they stay aligned).
-    bool compact = false;             // packed stream built (enough slices are narrow)
-    double compactFraction = 0.0;     // fraction of slots in narrow slices
-    std::vector<int32_t> colBase;     // nslices
-    std::vector<int64_t> cptr;        // nslices
-    std::vector<uint16_t> packed;
-    std::vector<int32_t> slot2csr;    // nslots, CSR position or -1
-    std::vector<int32_t> perm, iperm; // nrows
-    std::vector<int32_t> diagSlot;    // nrows (new numbering)
-};
-// reorder: apply reverse Cuthill-McKee before the window sort (bandwidth
-// reduction, so slices span few columns and x gathers stay local)
-SellLayout buildSell(const GraphCSR& g, int window, bool reorder = true);
+// Row order of the solver's SELL-C-sigma image (new -> old): reverse
+// Cuthill-McKee when reorder is set (bandwidth reduction, so slices span few
+// columns and x gathers stay local), then a stable sort by decreasing row
+// length inside windows of `window` rows.  The image itself is built on the
+// device.
+std::vector<int32_t> sellPermutation(const GraphCSR& g, int window, bool reorder = true);
 
 // Reverse Cuthill-McKee ordering of a symmetric CSR graph (new -> old)
 std::vector<int32_t> reverseCuthillMcKee(const GraphCSR& g);

@@ -138,6 +138,16 @@ int feel_hip_ctx_reset_timing(feel_hip_ctx* ctx);
 int feel_hip_space_create(feel_hip_ctx* ctx, int dim, int order, int ncomp, int64_t nverts,
                           const double* coords, int64_t nelems, const int32_t* elems,
                           const int32_t* elem2dof, int64_t ndof, feel_hip_space** out);
+/* The same space with the DofTable built on the device from the mesh alone:
+ * Feel++'s first-touch numbering (DofTable::build/buildDofMap,
+ * feeldiscr/doftable.hpp:1802-1890,2364-2455; DofFromElement::add,
+ * feeldiscr/doffromelement.hpp:492-530), identical to the host restatement. */
+int feel_hip_space_create_from_mesh(feel_hip_ctx* ctx, int dim, int order, int ncomp, int64_t nverts,
+                                    const double* coords, int64_t nelems, const int32_t* elems,
+                                    feel_hip_space** out);
+/* copies the space's elem2dof (nelements x nloc*ncomp) to host memory
+ * (DofTable::localToGlobalIndices, doftable.hpp:732-750) */
+int feel_hip_space_elem2dof(const feel_hip_space* s, int32_t* elem2dof);
 int feel_hip_space_destroy(feel_hip_space* s);
 int64_t feel_hip_space_ndof(const feel_hip_space* s);
 int64_t feel_hip_space_nelements(const feel_hip_space* s);
@@ -148,6 +158,12 @@ int64_t feel_hip_space_nelements(const feel_hip_space* s);
  * and derives the owner-computes scatter map of the space's element entries. */
 int feel_hip_graph_create(feel_hip_space* space, int64_t nrows, const int32_t* ia, const int32_t* ja,
                           feel_hip_graph** out);
+/* The space's COUPLED pattern built on the device (Stencil::computeGraph,
+ * feeldiscr/stencil.hpp:1502-1692; GraphCSR::close, feelalg/graphcsr.cpp:663-1021),
+ * identical to the graph feel_hip_graph_create takes from the host. */
+int feel_hip_graph_create_from_space(feel_hip_space* space, feel_hip_graph** out);
+/* copies ia (nrows+1) and ja (nnz) to host memory (GraphCSR::ia/ja, graphcsr.hpp:61-81) */
+int feel_hip_graph_pattern(const feel_hip_graph* g, int32_t* ia, int32_t* ja);
 int feel_hip_graph_destroy(feel_hip_graph* g);
 int64_t feel_hip_graph_nnz(const feel_hip_graph* g);
 
