@@ -36,24 +36,30 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_problem(fa, n, order, jitter, seed):
+def build_problem(fa, ctx, n, order, jitter, seed):
+    """Mesh on the host; DofTable, sparsity pattern and SELL image on the device.
+    host: the arrays the CPU baseline works on (elem2dof and ia/ja copied back)."""
     t0 = time.perf_counter()
     coords, elems = fa.mesh_kuhn_box(n, jitter=jitter, seed=seed)
-    e2d, ndof = fa.doftable(3, order, 1, elems, coords.shape[0])
-    ia, ja = fa.graph(e2d, ndof)
+    t1 = time.perf_counter()
+    space = fa.Space.from_mesh(ctx, coords, elems, order)
+    graph = fa.Graph.from_space(space)
+    ctx.synchronize()
+    t2 = time.perf_counter()
     k = fa.lognormal(len(elems), 0.0, 1.0, seed + 1)
-    isb = fa.boundary_dofs(3, order, 1, coords, elems, e2d, ndof)
-    log(f"host setup {time.perf_counter() - t0:.1f}s: {len(elems)} tets, {ndof} dofs, {len(ja)} nnz")
-    return coords, elems, e2d, ndof, ia, ja, k, isb
+    isb = fa.boundary_dofs(3, order, 1, coords, elems, space.elem2dof, space.ndof)
+    log(f"setup: mesh {t1 - t0:.2f}s, device DofTable + pattern + SELL {t2 - t1:.2f}s: {len(elems)} tets, "
+        f"{space.ndof} dofs, {graph.nnz} nnz")
+    host = (coords, elems, space.elem2dof, space.ndof, graph.ia, graph.ja, k, isb)
+    return space, graph, host
 
 
 class Problem:
-    def __init__(self, fa, ctx, host, order, qorder):
-        coords, elems, e2d, ndof, ia, ja, k, isb = host
+    def __init__(self, fa, ctx, built, order, qorder):
+        self.space, self.graph, self.host = built
+        coords, elems, e2d, ndof, ia, ja, k, isb = self.host
         t0 = time.perf_counter()
         self.fa, self.ctx = fa, ctx
-        self.space = fa.Space(ctx, coords, elems, order, 1, e2d, ndof)
-        self.graph = fa.Graph(self.space, ia, ja)
         self.A = fa.Matrix(self.graph, fa.FP64_FP32)
         self.k = ctx.vector(len(elems), k)
         self.b = ctx.vector(ndof)
@@ -61,7 +67,7 @@ class Problem:
         self.rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
         self.order, self.qorder, self.ndof, self.ne, self.nnz = order, qorder, ndof, len(elems), len(ja)
         ctx.synchronize()
-        log(f"device setup {time.perf_counter() - t0:.1f}s, resident {ctx.device_bytes() / 2**30:.2f} GiB")
+        log(f"device buffers {time.perf_counter() - t0:.1f}s, resident {ctx.device_bytes() / 2**30:.2f} GiB")
 
     def step(self, precision, rtol, inner_rtol=1e-2):
         fa = self.fa
@@ -252,12 +258,11 @@ def bench_beam(fa, args):
         ne = len(elems)
         E = fa.uniform(ne, 1.0, 100.0, args.seed + 1, log=True)
         nu = fa.uniform(ne, 0.2, 0.35, args.seed + 2)
-        e2d, ndof = fa.doftable(3, k, 3, elems, coords.shape[0])
-        ia, ja = fa.graph(e2d, ndof)
+        space = fa.Space.from_mesh(ctx, coords, elems, k, 3)
+        g = fa.Graph.from_space(space)
+        e2d, ndof, ja = space.elem2dof, space.ndof, g.ja
         isb = fa.boundary_dofs(3, k, 3, coords, elems, e2d, ndof, sel=1)
-        log(f"beam P{k}: host setup {time.perf_counter() - t0:.1f}s, {ne} tets, {ndof} dofs, {len(ja)} nnz")
-        space = fa.Space(ctx, coords, elems, k, 3, e2d, ndof)
-        g = fa.Graph(space, ia, ja)
+        log(f"beam P{k}: setup {time.perf_counter() - t0:.1f}s, {ne} tets, {ndof} dofs, {len(ja)} nnz")
         A = fa.Matrix(g, fa.FP64_FP32)
         young, poisson = ctx.vector(ne, E), ctx.vector(ne, nu)
         b, x = ctx.vector(ndof), ctx.vector(ndof)
@@ -325,9 +330,9 @@ def bench_kernels(fa, args):
     log(f"copy kernel: {nbytes} bytes read + {nbytes} bytes written per launch, 5 launches")
     del a, b
     order, qorder = args.order, 2 * args.order
-    host = build_problem(fa, args.n, order, 0.15, 2024)
     ctx = fa.Context(0)
-    prob = Problem(fa, ctx, host, order, qorder)
+    prob = Problem(fa, ctx, build_problem(fa, ctx, args.n, order, 0.15, 2024), order, qorder)
+    host = prob.host
     for _ in range(3):
         prob.A.assemble(fa.DIFFUSION, qorder, kdiff=prob.k, precision=fa.FP64_FP32)
     fa.assemble_linear(prob.space, prob.b, fa.SRC_CONST, order, (1.0,))
@@ -439,9 +444,9 @@ def main():
             args.rtol = 1e-9
         return bench_serving(fa, args)
     order, qorder = args.order, 2 * args.order
-    host = build_problem(fa, args.n, order, 0.15, 2024)
     ctx = fa.Context(0)
-    prob = Problem(fa, ctx, host, order, qorder)
+    prob = Problem(fa, ctx, build_problem(fa, ctx, args.n, order, 0.15, 2024), order, qorder)
+    host = prob.host
 
     # FP64 reference solution (tight) for the error figure
     st = prob.step(fa.FP64, 1e-13)

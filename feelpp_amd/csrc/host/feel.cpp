@@ -208,10 +208,29 @@ mesh_ptr loadMesh(std::string const& path, int dim) {
 
 FunctionSpace::FunctionSpace(mesh_ptr mesh, int order, int ncomp, std::shared_ptr<Backend> be)
     : M_mesh(std::move(mesh)), M_backend(std::move(be)) {
-    M_dt = feelpp_amd::buildDofTable(M_mesh->m, order, ncomp);
+    // DofTable::build on the device; the host copy serves boundary dof lists and export
     auto const& m = M_mesh->m;
-    check(feel_hip_space_create(M_backend->ctx(), m.dim, order, ncomp, m.nv, m.coords.data(), m.ne, m.elems.data(),
-                                M_dt.elem2dof.data(), M_dt.ndof, &M_space));
+    check(feel_hip_space_create_from_mesh(M_backend->ctx(), m.dim, order, ncomp, m.nv, m.coords.data(), m.ne,
+                                          m.elems.data(), &M_space));
+    M_dt.dim = m.dim;
+    M_dt.order = order;
+    M_dt.ncomp = ncomp;
+    M_dt.nloc = feelpp_amd::nLocalDof(m.dim, order);
+    M_dt.nldof = M_dt.nloc * ncomp;
+    M_dt.ne = m.ne;
+    M_dt.ndof = feel_hip_space_ndof(M_space);
+    M_dt.elem2dof.resize((size_t)m.ne * M_dt.nldof);
+    check(feel_hip_space_elem2dof(M_space, M_dt.elem2dof.data()));
+}
+
+std::shared_ptr<feel_hip_graph> FunctionSpace::graph() {
+    // one pattern per space, shared by its forms (StencilManager cache, backend.hpp:423)
+    if (!M_graph) {
+        feel_hip_graph* g = nullptr;
+        check(feel_hip_graph_create_from_space(M_space, &g));
+        M_graph = std::shared_ptr<feel_hip_graph>(g, [](feel_hip_graph* p) { feel_hip_graph_destroy(p); });
+    }
+    return M_graph;
 }
 
 Element FunctionSpace::element() {
@@ -292,12 +311,9 @@ void LinearForm::assemble(IntegralExpr const& ie, bool add) {
 BilinearForm::BilinearForm(space_ptr trial, space_ptr test, int precision) : M_trial(std::move(trial)), M_test(std::move(test)) {
     if (M_trial != M_test) throw std::invalid_argument("trial and test spaces must be the same space");
     // Backend::newMatrix -> Stencil::computeGraph -> MatrixSparse::init (backend.hpp:349-440)
-    feelpp_amd::GraphCSR gc = feelpp_amd::buildGraph(M_test->dofTable());
-    feel_hip_graph* g = nullptr;
-    check(feel_hip_graph_create(M_test->handle(), gc.nrows, gc.ia.data(), gc.ja.data(), &g));
-    M_graph = std::shared_ptr<feel_hip_graph>(g, [](feel_hip_graph* p) { feel_hip_graph_destroy(p); });
+    M_graph = M_test->graph();
     feel_hip_mat* A = nullptr;
-    check(feel_hip_mat_create(g, precision, &A));
+    check(feel_hip_mat_create(M_graph.get(), precision, &A));
     auto keepGraph = M_graph;
     M_mat = std::shared_ptr<feel_hip_mat>(A, [keepGraph](feel_hip_mat* p) { feel_hip_mat_destroy(p); });
 }

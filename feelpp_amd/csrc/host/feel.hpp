@@ -154,12 +154,15 @@ class FunctionSpace : public std::enable_shared_from_this<FunctionSpace> {
     feelpp_amd::DofTable const& dofTable() const { return M_dt; }
     feel_hip_space* handle() const { return M_space; }
     std::shared_ptr<Backend> backendPtr() const { return M_backend; }
+    // the COUPLED sparsity pattern of the space, built on first use
+    std::shared_ptr<feel_hip_graph> graph();
 
    private:
     mesh_ptr M_mesh;
     feelpp_amd::DofTable M_dt;
     std::shared_ptr<Backend> M_backend;
     feel_hip_space* M_space = nullptr;
+    std::shared_ptr<feel_hip_graph> M_graph;
     int M_next_id = 0;
 };
 // Reference functions of the error norms (per component)

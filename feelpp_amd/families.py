@@ -12,8 +12,7 @@ import time
 
 import numpy as np
 
-from . import (DIFFUSION, FP64, SRC_CONST, Graph, Matrix, Rows, Space, assemble_linear, boundary_dofs, doftable,
-               graph, mesh_kuhn_box)
+from . import DIFFUSION, FP64, SRC_CONST, Graph, Matrix, Rows, Space, assemble_linear, boundary_dofs, mesh_kuhn_box
 
 
 class Family:
@@ -21,10 +20,9 @@ class Family:
         t0 = time.perf_counter()
         self.order, self.n = order, n
         coords, elems = mesh_kuhn_box(n, jitter=jitter, seed=seed)
-        e2d, ndof = doftable(3, order, 1, elems, coords.shape[0])
-        ia, ja = graph(e2d, ndof)
-        self.space = Space(ctx, coords, elems, order, 1, e2d, ndof)
-        self.graph = Graph(self.space, ia, ja)
+        self.space = Space.from_mesh(ctx, coords, elems, order)
+        self.graph = Graph.from_space(self.space)
+        e2d, ndof, ja = self.space.elem2dof, self.space.ndof, self.graph.ja
         cen = coords[elems].mean(axis=1)
         self.sub = (cen[:, 0] > 0.5).astype(np.int64) + 2 * (cen[:, 1] > 0.5).astype(np.int64)
         self.qorder = max(2 * order - 2, 1)   # exact for inner(gradt, grad) on affine P_k
