@@ -343,8 +343,9 @@ class Context(_Handle):
     def device_bytes(self):
         return lib().feel_hip_ctx_device_bytes(self.h)
 
-    def set_timing(self, on):
-        _check(lib().feel_hip_ctx_set_timing(self.h, int(on)))
+    def set_timing(self, period):
+        """0: off; N: time every N-th launch of each timed kernel (True = every launch)"""
+        _check(lib().feel_hip_ctx_set_timing(self.h, int(period)))
 
     def reset_timing(self):
         _check(lib().feel_hip_ctx_reset_timing(self.h))

@@ -73,6 +73,9 @@ struct feel_hip_ctx {
     int64_t bytes = 0;
     // timing
     bool timing = false;
+    int period = 1;                           // time every period-th launch of each timer
+    bool pending = false;                     // the last tic recorded its start event
+    std::map<std::string, int64_t> timerCalls;
     struct Ev { hipEvent_t a, b; std::string name; };
     std::vector<Ev> events;
     size_t evUsed = 0;

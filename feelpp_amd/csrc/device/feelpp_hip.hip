@@ -87,7 +87,9 @@ void feel_hip_ctx::release(void* p, size_t n) {
 }
 
 void feel_hip_ctx::tic(const char* name) {
+    pending = false;
     if (!timing) return;
+    if (period > 1 && timerCalls[name]++ % period != 0) return;
     if (evUsed == events.size()) {
         Ev ev;
         if (hipEventCreate(&ev.a) != hipSuccess || hipEventCreate(&ev.b) != hipSuccess) return;
@@ -95,12 +97,14 @@ void feel_hip_ctx::tic(const char* name) {
     }
     events[evUsed].name = name;
     (void)hipEventRecord(events[evUsed].a, stream);
+    pending = true;
 }
 
 void feel_hip_ctx::toc() {
-    if (!timing || evUsed >= events.size()) return;
+    if (!timing || !pending || evUsed >= events.size()) return;
     (void)hipEventRecord(events[evUsed].b, stream);
     ++evUsed;
+    pending = false;
 }
 
 void feel_hip_ctx::collectTimers() {
@@ -234,10 +238,13 @@ int feel_hip_ctx_synchronize(feel_hip_ctx* c) {
 void* feel_hip_ctx_stream(feel_hip_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int64_t feel_hip_ctx_device_bytes(feel_hip_ctx* c) { return c ? c->bytes : -1; }
 
-int feel_hip_ctx_set_timing(feel_hip_ctx* c, int enable) {
+int feel_hip_ctx_set_timing(feel_hip_ctx* c, int period) {
     if (!c) return setError(FEEL_HIP_EINVAL, "null context");
+    if (period < 0) return setError(FEEL_HIP_EINVAL, "timing period must be >= 0");
     c->collectTimers();
-    c->timing = enable != 0;
+    c->timing = period > 0;
+    c->period = period;
+    c->timerCalls.clear();
     return FEEL_HIP_OK;
 }
 

@@ -123,10 +123,11 @@ void* feel_hip_ctx_stream(feel_hip_ctx* ctx);
 /* Device memory held by the context's handles, in bytes */
 int64_t feel_hip_ctx_device_bytes(feel_hip_ctx* ctx);
 
-/* Kernel timing with HIP events on the context stream: when enabled, every
- * SpMV launch of a solve is bracketed by events.  name: "spmv_fp64",
- * "spmv_fp32", "assemble_gather", "assemble_geo". */
-int feel_hip_ctx_set_timing(feel_hip_ctx* ctx, int enable);
+/* Kernel timing with HIP events on the context stream.  period 0: off; N > 0:
+ * every N-th launch of each timed kernel is bracketed by events (1 = every
+ * launch; a larger period keeps the events' own cost out of a timed loop).
+ * name: "spmv_fp64", "spmv_fp32", "assemble_gather", "assemble_geo". */
+int feel_hip_ctx_set_timing(feel_hip_ctx* ctx, int period);
 int feel_hip_ctx_timing(feel_hip_ctx* ctx, const char* name, int64_t* launches, double* total_ms);
 int feel_hip_ctx_reset_timing(feel_hip_ctx* ctx);
 
