@@ -11,8 +11,8 @@ namespace feelpp_amd {
 // off-diagonal sqrt(b_j)/a_j with a_j = gamma_j / pAp_j, b_j = gamma_{j+1} /
 // gamma_j) has extreme eigenvalues that approach those of D^{-1} A from
 // inside, so the ratio is a lower bound that grows with k.  The eigenvalues
-// come from Sturm-sequence bisection.
-double cgConditionEstimate(const double* hist, int k) {
+// come from Sturm-sequence bisection, each to the relative precision rel.
+double cgConditionEstimate(const double* hist, int k, double rel) {
     if (k < 2) return 1.0;
     std::vector<double> d(k), e(k, 0.0);
     for (int j = 0; j < k; ++j) {
@@ -45,7 +45,7 @@ double cgConditionEstimate(const double* hist, int k) {
     };
     auto bisect = [&](int target) {   // smallest x with below(x) >= target
         double a = lo, b = hi;
-        for (int i = 0; i < 200 && b - a > 1e-14 * std::max(std::fabs(a), std::fabs(b)); ++i) {
+        for (int i = 0; i < 200 && b - a > rel * std::max(std::fabs(a), std::fabs(b)); ++i) {
             const double m = 0.5 * (a + b);
             if (below(m) >= target) b = m; else a = m;
         }
