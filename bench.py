@@ -479,7 +479,7 @@ def main():
         bytes_spmv = 8 * nnz + 12 * n + 4
     avg_ms = ms / max(nl, 1)
     achieved = bytes_spmv / (avg_ms * 1e-3) / 1e9 if nl else 0.0
-    spmv_kernel = "kSellSpmv<double,1024,4,true>" if spmv_name == "spmv_fp64" else "kSellSpmv<float,1024,8,true>"
+    spmv_kernel = "kSellSpmv<double,1024,2,true>" if spmv_name == "spmv_fp64" else "kSellSpmv<float,1024,2,true>"
     # HBM bytes per launch from the committed counter passes (calibrated FETCH_SIZE + WRITE_SIZE),
     # valid for the workload they were taken on
     traffic = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_p2_kuhn64.txt"))

@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void kBSpmv64(const BItem* __restrict__ ite
         const int width = (int)((I.slicePtr[s + 1] - base) >> 6);
         double acc = 0.0;
         for (int k = 0; k < width; ++k) {
-            const size_t slot = base + (size_t)k * 64 + lane;
+            const size_t slot = sellSlot(base, k, lane);
             acc += I.val64[slot] * I.x64[I.col[slot]];
         }
         const int64_t r = (int64_t)s * 64 + lane;
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void kBMask(int64_t n, int64_t nslices, con
         const int width = (int)((slicePtr[s + 1] - base) >> 6);
         const int64_t r = s * 64 + lane;
         for (int k = 0; k < width; ++k) {
-            const size_t slot = base + (size_t)k * 64 + lane;
+            const size_t slot = sellSlot(base, k, lane);
             uint8_t m = 1;
             if (r < n && slot2csr[slot] >= 0) {
                 const int32_t c = col[slot];
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ 
 // of the single solver), partial p.Ap
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
-    constexpr int U = sizeof(T) == 8 ? 4 : 8;
+    constexpr int U = 2;   // quads per step
     const BItem item = items[blockIdx.x];
     const BInst& I = insts[item.inst];
     if (instDone(I.st)) return;
@@ -286,7 +286,6 @@ __global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items
 }
 
 constexpr int kGroupMax = 8;
-constexpr int kGroupUnroll = 4;
 constexpr int kGroupBlock = 256;   // 4 wavefronts
 constexpr int kXcds = 8;           // accelerator complex dies, each with its own L2
 constexpr int kGroupSlices = 16;   // slices per grouped-SpMV block (4 per wavefront)
@@ -303,45 +302,55 @@ struct BGItem {
     int group, s0, s1, item;   // item: index of the block's p.Ap partial (s0 / kGroupSlices)
 };
 
-// U consecutive slots of one slice row: byte offset of the column in p, unit
-// diagonal of a Dirichlet row, and the term values (zero on Dirichlet slots)
-template <int NT, int U>
-struct GroupSlots {
-    int32_t off[U];
-    float e[U];
-    float a[U][NT];
+// One quad (4 consecutive entries) of one slice row: byte offsets of the
+// columns in p, unit diagonal of a Dirichlet row, and the term values (zero on
+// Dirichlet slots)
+template <int NT>
+struct GroupQuad {
+    int32_t off[4];
+    float e[4];
+    float a[4][NT];
 };
 
-template <int NT, int U, bool NARROW>
-__device__ inline void loadSlots(const BGroup& G, size_t slot0, const void* cp, int32_t cb, int k0,
-                                 GroupSlots<NT, U>& d) {
+template <int NT, bool NARROW>
+__device__ inline void loadQuad(const BGroup& G, size_t q0, const void* cp, int32_t cb, int kq, GroupQuad<NT>& d) {
+    // q0: slot of the lane's first quad; quad kq starts 256 kq slots further
+    const size_t q = q0 + ((size_t)kq << 8);
+    int32_t col[4];
+    if (NARROW) {
+        const u16x4 c = static_cast<const u16x4*>(cp)[(size_t)kq * 64];
+        col[0] = cb + c.x; col[1] = cb + c.y; col[2] = cb + c.z; col[3] = cb + c.w;
+    } else {
+        const i32x4 c = static_cast<const i32x4*>(cp)[(size_t)kq * 64];
+        col[0] = c.x; col[1] = c.y; col[2] = c.z; col[3] = c.w;
+    }
+    const u8x4 m = *reinterpret_cast<const u8x4*>(G.mask + q);
+    const uint8_t mm[4] = {m.x, m.y, m.z, m.w};
+    float tv[NT][4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const size_t o = (size_t)(k0 + u) * 64;
-        const int32_t col = NARROW ? cb + (int32_t)__builtin_nontemporal_load(static_cast<const uint16_t*>(cp) + o)
-                                   : __builtin_nontemporal_load(static_cast<const int32_t*>(cp) + o);
-        d.off[u] = col * (int32_t)sizeof(float);
-        const uint8_t m = G.mask[slot0 + o];
-        d.e[u] = m == 2 ? 1.0f : 0.0f;
+    for (int t = 0; t < NT; ++t) {
+        const Quad<float>::type v = __builtin_nontemporal_load(reinterpret_cast<const Quad<float>::type*>(G.terms[t] + q));
+        tv[t][0] = v.x; tv[t][1] = v.y; tv[t][2] = v.z; tv[t][3] = v.w;
+    }
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const float v = __builtin_nontemporal_load(G.terms[t] + slot0 + o);
-            d.a[u][t] = m == 0 ? v : 0.0f;
-        }
+    for (int u = 0; u < 4; ++u) {
+        d.off[u] = col[u] * (int32_t)sizeof(float);
+        d.e[u] = mm[u] == 2 ? 1.0f : 0.0f;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) d.a[u][t] = mm[u] == 0 ? tv[t][u] : 0.0f;
     }
 }
 
-// acc_b += A(mu_b) p_b on the U slots for every active instance b of the
-// group; p_b is read through a buffer resource so the gathers share the
-// per-slot offsets
-template <int NT, int U>
-__device__ inline void applySlots(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* R,
-                                  const GroupSlots<NT, U>& d, float* acc) {
+// acc_b += A(mu_b) p_b on the quad for every active instance b of the group;
+// p_b is read through a buffer resource so the gathers share the offsets
+template <int NT>
+__device__ inline void applyQuad(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* R,
+                                 const GroupQuad<NT>& d, float* acc) {
 #pragma unroll
     for (int b = 0; b < kGroupMax; ++b) {
         if ((amask >> b) & 1u) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
+            for (int u = 0; u < 4; ++u) {
                 float v = d.e[u];
 #pragma unroll
                 for (int t = 0; t < NT; ++t) v += G.mu[b][t] * d.a[u][t];
@@ -352,30 +361,21 @@ __device__ inline void applySlots(const BGroup& G, unsigned amask, const __amdgp
     }
 }
 
-// one slice row, the next U slots' matrix data loaded while the current U
-// slots' gathers and products run
+// one slice row, the next quad's matrix data loaded while the current quad's
+// gathers and products run
 template <int NT, bool NARROW>
-__device__ inline void groupRow(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* P, size_t slot0,
-                                const void* cp, int32_t cb, int width, float* acc) {
-    constexpr int U = kGroupUnroll;
-    int k = 0;
-    if (width >= U) {
-        GroupSlots<NT, U> cur;
-        loadSlots<NT, U, NARROW>(G, slot0, cp, cb, 0, cur);
-        for (; k + 2 * U <= width; k += U) {
-            GroupSlots<NT, U> nxt;
-            loadSlots<NT, U, NARROW>(G, slot0, cp, cb, k + U, nxt);
-            applySlots<NT, U>(G, amask, P, cur, acc);
-            cur = nxt;
-        }
-        applySlots<NT, U>(G, amask, P, cur, acc);
-        k += U;
+__device__ inline void groupRow(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* P, size_t q0,
+                                const void* cp, int32_t cb, int nq, float* acc) {
+    if (nq <= 0) return;
+    GroupQuad<NT> cur;
+    loadQuad<NT, NARROW>(G, q0, cp, cb, 0, cur);
+    for (int k = 1; k < nq; ++k) {
+        GroupQuad<NT> nxt;
+        loadQuad<NT, NARROW>(G, q0, cp, cb, k, nxt);
+        applyQuad<NT>(G, amask, P, cur, acc);
+        cur = nxt;
     }
-    for (; k < width; ++k) {
-        GroupSlots<NT, 1> one;
-        loadSlots<NT, 1, NARROW>(G, slot0, cp, cb, k, one);
-        applySlots<NT, 1>(G, amask, P, one, acc);
-    }
+    applyQuad<NT>(G, amask, P, cur, acc);
 }
 
 // Grouped SpMV of an affine family (FP32 inner iterations): the in-flight
@@ -418,19 +418,20 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
     for (int b = 0; b < kGroupMax; ++b) dot[b] = 0.0;
     for (int sl = it.s0 + wv; sl < it.s1; sl += kGroupBlock / 64) {
         const uint32_t base = G.cols.slicePtr[sl];
-        const int width = (int)((G.cols.slicePtr[sl + 1] - base) >> 6);
-        const size_t slot0 = base + lane;
+        const int nq = (int)((G.cols.slicePtr[sl + 1] - base) >> 8);
+        const size_t q0 = (size_t)base + ((size_t)lane << 2);
         float acc[kGroupMax];
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b) acc[b] = 0.0f;
         const int32_t cb = G.compact ? G.cols.colBase[sl] : -1;
         if (cb >= 0)
-            groupRow<NT, true>(G, amask, R, slot0, G.cols.packed + G.cols.cptr[sl] + lane, cb, width, acc);
+            groupRow<NT, true>(G, amask, R, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
+                               cb, nq, acc);
         else if (G.compact)
-            groupRow<NT, false>(G, amask, R, slot0,
-                                reinterpret_cast<const int32_t*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb, width, acc);
+            groupRow<NT, false>(G, amask, R, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
+                                cb, nq, acc);
         else
-            groupRow<NT, false>(G, amask, R, slot0, G.cols.col + slot0, cb, width, acc);
+            groupRow<NT, false>(G, amask, R, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
         const int64_t r = (int64_t)sl * 64 + lane;
         if (r < G.n) {
 #pragma unroll

@@ -745,50 +745,108 @@ struct SellCols {
     const uint16_t* __restrict__ packed;
 };
 
-// Row (s*64 + lane) of A x for slice s, executed by one wavefront: coalesced
-// column-major value/column streams read with non-temporal loads (streamed
-// once; keeps L2 for the x gathers), U independent partial sums per lane for
-// memory-level parallelism.
-template <typename T, int U, bool COMPACT>
+// Slot order inside a slice of the SELL image: the slice's width (padded to a
+// multiple of 4) is cut into quads of 4 consecutive entries of each row, and
+// quad q holds lane l's entries 4q..4q+3 at 256q + 4l + (0..3).  A lane then
+// reads each quad of values and columns with one 16-byte (8 for 16-bit
+// columns) load, and a wavefront's load covers 1 KiB of values.
+constexpr int kSellQuad = 4;
+__device__ inline size_t sellSlot(uint32_t base, int k, int lane) {
+    return (size_t)base + ((size_t)(k >> 2) << 8) + ((size_t)lane << 2) + (size_t)(k & 3);
+}
+
+template <typename T>
+struct Quad;
+template <>
+struct Quad<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Quad<double> {
+    typedef double type __attribute__((ext_vector_type(4)));
+};
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+typedef uint8_t u8x4 __attribute__((ext_vector_type(4)));
+
+// Row (s*64 + lane) of A x for slice s, executed by one wavefront: the
+// lane's quads of values and columns are streamed, UQ quads per step with
+// 4 x UQ independent partial sums for memory-level parallelism.  FP32 values
+// (one 16-byte load per quad) are read non-temporally, keeping L2 for the x
+// gathers; FP64 values take two 16-byte loads per quad, which need the line
+// to stay in L2 between them, and columns (8 or 16 bytes) likewise.
+template <typename T, int UQ, bool COMPACT>
 __device__ inline T sellRow(const SellCols& S, int64_t s, int lane, const T* __restrict__ val,
                             const T* __restrict__ x) {
+    typedef typename Quad<T>::type V;
     const uint32_t base = S.slicePtr[s];
-    const int width = (int)((S.slicePtr[s + 1] - base) >> 6);
-    const T* v = val + base + lane;
-    T acc[U];
+    const int nq = (int)((S.slicePtr[s + 1] - base) >> 8);
+    const V* v = reinterpret_cast<const V*>(val + base) + lane;
+    T acc[UQ][4];
 #pragma unroll
-    for (int u = 0; u < U; ++u) acc[u] = 0;
+    for (int u = 0; u < UQ; ++u) acc[u][0] = acc[u][1] = acc[u][2] = acc[u][3] = 0;
     // one wave per slice: the narrow/wide choice is uniform across the wave
     const int32_t cb = COMPACT ? S.colBase[s] : -1;
     if (cb >= 0) {
-        const uint16_t* c = S.packed + S.cptr[s] + lane;
+        const u16x4* c = reinterpret_cast<const u16x4*>(S.packed + S.cptr[s]) + lane;
         const T* xs = x + cb;
         int k = 0;
-        for (; k + U <= width; k += U) {
+        for (; k + UQ <= nq; k += UQ) {
+            V vv[UQ];
+            u16x4 cc[UQ];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
-                const uint16_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
-                acc[u] += vv * xs[cc];
+            for (int u = 0; u < UQ; ++u) {
+                vv[u] = sizeof(T) == 4 ? __builtin_nontemporal_load(v + (size_t)(k + u) * 64) : v[(size_t)(k + u) * 64];
+                cc[u] = c[(size_t)(k + u) * 64];
+            }
+#pragma unroll
+            for (int u = 0; u < UQ; ++u) {
+                acc[u][0] += vv[u].x * xs[cc[u].x];
+                acc[u][1] += vv[u].y * xs[cc[u].y];
+                acc[u][2] += vv[u].z * xs[cc[u].z];
+                acc[u][3] += vv[u].w * xs[cc[u].w];
             }
         }
-        for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * xs[c[(size_t)k * 64]];
+        for (; k < nq; ++k) {
+            const V vv = v[(size_t)k * 64];
+            const u16x4 cc = c[(size_t)k * 64];
+            acc[0][0] += vv.x * xs[cc.x];
+            acc[0][1] += vv.y * xs[cc.y];
+            acc[0][2] += vv.z * xs[cc.z];
+            acc[0][3] += vv.w * xs[cc.w];
+        }
     } else {
-        const int32_t* c = COMPACT ? reinterpret_cast<const int32_t*>(S.packed + S.cptr[s]) + lane : S.col + base + lane;
+        const i32x4* c = (COMPACT ? reinterpret_cast<const i32x4*>(S.packed + S.cptr[s])
+                                  : reinterpret_cast<const i32x4*>(S.col + base)) + lane;
         int k = 0;
-        for (; k + U <= width; k += U) {
+        for (; k + UQ <= nq; k += UQ) {
+            V vv[UQ];
+            i32x4 cc[UQ];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const T vv = __builtin_nontemporal_load(v + (size_t)(k + u) * 64);
-                const int32_t cc = __builtin_nontemporal_load(c + (size_t)(k + u) * 64);
-                acc[u] += vv * x[cc];
+            for (int u = 0; u < UQ; ++u) {
+                vv[u] = sizeof(T) == 4 ? __builtin_nontemporal_load(v + (size_t)(k + u) * 64) : v[(size_t)(k + u) * 64];
+                cc[u] = c[(size_t)(k + u) * 64];
+            }
+#pragma unroll
+            for (int u = 0; u < UQ; ++u) {
+                acc[u][0] += vv[u].x * x[cc[u].x];
+                acc[u][1] += vv[u].y * x[cc[u].y];
+                acc[u][2] += vv[u].z * x[cc[u].z];
+                acc[u][3] += vv[u].w * x[cc[u].w];
             }
         }
-        for (; k < width; ++k) acc[0] += v[(size_t)k * 64] * x[c[(size_t)k * 64]];
+        for (; k < nq; ++k) {
+            const V vv = v[(size_t)k * 64];
+            const i32x4 cc = c[(size_t)k * 64];
+            acc[0][0] += vv.x * x[cc.x];
+            acc[0][1] += vv.y * x[cc.y];
+            acc[0][2] += vv.z * x[cc.z];
+            acc[0][3] += vv.w * x[cc.w];
+        }
     }
     T a = 0;
 #pragma unroll
-    for (int u = 0; u < U; ++u) a += acc[u];
+    for (int u = 0; u < UQ; ++u) a += (acc[u][0] + acc[u][1]) + (acc[u][2] + acc[u][3]);
     return a;
 }
 

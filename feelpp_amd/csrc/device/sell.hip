@@ -1,9 +1,10 @@
 // Device-side construction of the solver's SELL-C-sigma image (C = 64 rows
 // per slice) from the graph's device ia/ja and a row order computed on the
 // host (reverse Cuthill-McKee, then rows sorted by length inside windows of
-// sigma rows).  Per slice: width = longest row; slot (s, k, lane) holds the
-// lane's k-th entry (permuted column, CSR position) or, past the row's end, a
-// padding entry pointing at the row itself with no CSR position.  The packed
+// sigma rows).  Per slice: width = longest row rounded up to a multiple of 4;
+// the lane's k-th entry (permuted column, CSR position) sits at sellSlot(base,
+// k, lane) (quads of 4 consecutive entries per lane) or, past the row's end,
+// a padding entry pointing at the row itself with no CSR position.  The packed
 // column stream stores a slice whose columns span fewer than 2^16 as 16-bit
 // offsets from its smallest column (colBase), any other slice as int32 pairs
 // of uint16 (colBase = -1); it is kept when at least a quarter of the slots
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "kernels.hpp"
 
 using namespace feelpp_amd;
 
@@ -41,6 +43,7 @@ __global__ __launch_bounds__(kSlBlock) void kSliceSlots(int64_t n, int64_t nslic
         int len = 0;
         if (r < n) len = ia[perm[r] + 1] - ia[perm[r]];
         for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+        len = (len + kSellQuad - 1) / kSellQuad * kSellQuad;
         if (lane == 0) slots[s] = (uint64_t)len * kC;
     }
 }
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(kSlBlock) void kSellFill(int64_t n, int64_t nrowsPa
         const int32_t p0 = r < n ? ia[old] : 0;
         const int len = r < n ? ia[old + 1] - p0 : 0;
         for (int k = 0; k < width; ++k) {
-            const int64_t slot = b + (int64_t)k * kC + lane;
+            const int64_t slot = (int64_t)sellSlot(b, k, lane);
             if (k < len) {
                 const int32_t c = ja[p0 + k];
                 col[slot] = iperm[c];

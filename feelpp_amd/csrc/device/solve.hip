@@ -29,8 +29,8 @@ SellCols sellCols(const feel_hip_graph* g) { return SellCols{g->slicePtr, g->sco
 // with the same operations in the same order; shapes differ only in how rows
 // map to waves and in the number of p.Ap block partials.
 struct SpmvShape { int block, grid; };
-constexpr SpmvShape kSpmvShapes[] = {{1024, 2048}, {1024, 1024}, {512, 2048}, {512, 1024}, {256, 1024},
-                                     {256, 2048}, {256, 4096}};
+constexpr SpmvShape kSpmvShapes[] = {{1024, 2048}, {1024, 1024}, {1024, 4096}, {512, 2048}, {512, 1024},
+                                     {256, 1024}, {256, 2048}, {256, 4096}};
 constexpr int kNumSpmvShapes = sizeof(kSpmvShapes) / sizeof(kSpmvShapes[0]);
 // a shape replaces the default only when it is this much faster, so that the
 // pick is stable from run to run where shapes are close
@@ -39,7 +39,7 @@ constexpr double kShapeMargin = 0.95;
 template <typename T, int BS, bool COMPACT>
 void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, int grid, const T* val, const T* x, T* y, double* part,
                 const CgState* st) {
-    constexpr int U = sizeof(T) == 8 ? 4 : 8;
+    constexpr int U = 2;   // quads per step
     hipLaunchKernelGGL((kSellSpmv<T, BS, U, COMPACT>), dim3(grid), dim3(BS), 0, c->stream, g->n, g->nslices,
                        sellCols(g), val, x, y, part, st);
 }
