@@ -53,12 +53,12 @@ struct BInst {
     int nitems;
     int apOff, nap;            // p.Ap partials of the current SpMV: part[apOff .. apOff + nap)
     int start;                 // global iteration at which the current inner solve began
-    int ibeg, iend;            // item range of this instance in the work list
+    int pstride;               // p of the FP32 iterations: element r at pT[r * pstride] (8 in a family group)
     double scale;              // FP32 residual = scale * r64 (scale = 1 / sigma, fixed per request)
     double xscale;             // x64 += xscale * xT
 };
 struct BItem {
-    int inst, s0, s1;
+    int inst, s0, s1, item;    // item: index of the item within its instance (its partial slot)
 };
 
 template <typename T>
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void kBResidual(const BItem* __restrict__ i
         zz += zi * zi;
     }
     zz = blockReduceSum(zz);
-    if (threadIdx.x == 0) zzPart[blockIdx.x - I.ibeg] = zz;
+    if (threadIdx.x == 0) zzPart[it.item] = zz;
 }
 
 // one block per instance of the work list: out[i] = sum of its z.z partials
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void kBInnerInit(const BItem* __restrict__ 
     }
     zr = blockReduceSum(zr);
     zz = blockReduceSum(zz);
-    if (threadIdx.x == 0) { I.part[blockIdx.x - I.ibeg] = zr; I.part[I.nitems + blockIdx.x - I.ibeg] = zz; }
+    if (threadIdx.x == 0) { I.part[it.item] = zr; I.part[I.nitems + it.item] = zz; }
 }
 
 // CG direction with the convergence test (as kCgDirection, per instance); the
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ 
     else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
     else if (it >= st->maxit) reason = -3;
     else if (beta == 0.0) reason = 3;
-    if (bid == I.ibeg && threadIdx.x == 0) {
+    if (item.item == 0 && threadIdx.x == 0) {
         st->dp = dp;
         if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
         if (reason) { st->done = 1; st->reason = reason; st->its = it; }
@@ -256,7 +256,9 @@ __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ 
     itemRows(I, item, r0, r1);
     T* p = tp<T>(I.pT);
     const T* z = tp<T>(I.zT);
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) p[r] = first ? z[r] : (T)((double)z[r] + bb * (double)p[r]);
+    const int ps = I.pstride;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock)
+        p[r * ps] = first ? z[r] : (T)((double)z[r] + bb * (double)p[r * ps]);
 }
 
 // w = A p on the item's slices (one wavefront per slice, the SELL row kernel
@@ -282,17 +284,21 @@ __global__ __launch_bounds__(kBlock) void kBSpmv(const BItem* __restrict__ items
         }
     }
     dot = blockReduceSum(dot);
-    if (threadIdx.x == 0) I.part[2 * I.nitems + blockIdx.x - I.ibeg] = dot;
+    if (threadIdx.x == 0) I.part[2 * I.nitems + item.item] = dot;
 }
 
 constexpr int kGroupMax = 8;
 constexpr int kGroupBlock = 256;   // 4 wavefronts
 constexpr int kXcds = 8;           // accelerator complex dies, each with its own L2
 constexpr int kGroupSlices = 16;   // slices per grouped-SpMV block (4 per wavefront)
+// A family group: up to kGroupMax in-flight FP32 instances of one family whose
+// search directions are stored interleaved, P[r * 8 + slot], so that one
+// 32-byte load brings every instance's p at a column.
 struct BGroup {
-    int ninst, nterms, compact, nitems;
+    int nterms, compact, nitems;
     int64_t n;
-    int inst[kGroupMax];                 // positions in the launch's instance array
+    const float* P;                      // interleaved search directions of the group
+    int inst[kGroupMax];                 // per slot: position in the launch's instance array, -1 if none
     float mu[kGroupMax][kMaxTerms];
     const float* terms[kMaxTerms];       // family terms, FP32 SELL values
     const uint8_t* mask;
@@ -302,12 +308,13 @@ struct BGItem {
     int group, s0, s1, item;   // item: index of the block's p.Ap partial (s0 / kGroupSlices)
 };
 
-// One quad (4 consecutive entries) of one slice row: byte offsets of the
-// columns in p, unit diagonal of a Dirichlet row, and the term values (zero on
-// Dirichlet slots)
+typedef Quad<float>::type f32x4;
+
+// One quad (4 consecutive entries) of one slice row: columns, unit diagonal
+// of a Dirichlet row, and the term values (zero on Dirichlet slots)
 template <int NT>
 struct GroupQuad {
-    int32_t off[4];
+    int32_t col[4];
     float e[4];
     float a[4][NT];
 };
@@ -316,46 +323,51 @@ template <int NT, bool NARROW>
 __device__ inline void loadQuad(const BGroup& G, size_t q0, const void* cp, int32_t cb, int kq, GroupQuad<NT>& d) {
     // q0: slot of the lane's first quad; quad kq starts 256 kq slots further
     const size_t q = q0 + ((size_t)kq << 8);
-    int32_t col[4];
     if (NARROW) {
         const u16x4 c = static_cast<const u16x4*>(cp)[(size_t)kq * 64];
-        col[0] = cb + c.x; col[1] = cb + c.y; col[2] = cb + c.z; col[3] = cb + c.w;
+        d.col[0] = cb + c.x; d.col[1] = cb + c.y; d.col[2] = cb + c.z; d.col[3] = cb + c.w;
     } else {
         const i32x4 c = static_cast<const i32x4*>(cp)[(size_t)kq * 64];
-        col[0] = c.x; col[1] = c.y; col[2] = c.z; col[3] = c.w;
+        d.col[0] = c.x; d.col[1] = c.y; d.col[2] = c.z; d.col[3] = c.w;
     }
     const u8x4 m = *reinterpret_cast<const u8x4*>(G.mask + q);
     const uint8_t mm[4] = {m.x, m.y, m.z, m.w};
     float tv[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const Quad<float>::type v = __builtin_nontemporal_load(reinterpret_cast<const Quad<float>::type*>(G.terms[t] + q));
+        const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(G.terms[t] + q));
         tv[t][0] = v.x; tv[t][1] = v.y; tv[t][2] = v.z; tv[t][3] = v.w;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        d.off[u] = col[u] * (int32_t)sizeof(float);
         d.e[u] = mm[u] == 2 ? 1.0f : 0.0f;
 #pragma unroll
         for (int t = 0; t < NT; ++t) d.a[u][t] = mm[u] == 0 ? tv[t][u] : 0.0f;
     }
 }
 
-// acc_b += A(mu_b) p_b on the quad for every active instance b of the group;
-// p_b is read through a buffer resource so the gathers share the offsets
+// the 8 interleaved p values of the group at row (or column) r
+__device__ inline void groupP(const float* __restrict__ P, int64_t r, float* pv) {
+    const f32x4* q = reinterpret_cast<const f32x4*>(P) + 2 * r;
+    const f32x4 lo = q[0], hi = q[1];
+    pv[0] = lo.x; pv[1] = lo.y; pv[2] = lo.z; pv[3] = lo.w;
+    pv[4] = hi.x; pv[5] = hi.y; pv[6] = hi.z; pv[7] = hi.w;
+}
+
+// acc_b += A(mu_b) p_b on the quad for every active instance b of the group
 template <int NT>
-__device__ inline void applyQuad(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* R,
-                                 const GroupQuad<NT>& d, float* acc) {
+__device__ inline void applyQuad(const BGroup& G, unsigned amask, const GroupQuad<NT>& d, float* acc) {
 #pragma unroll
-    for (int b = 0; b < kGroupMax; ++b) {
-        if ((amask >> b) & 1u) {
+    for (int u = 0; u < 4; ++u) {
+        float pv[kGroupMax];
+        groupP(G.P, d.col[u], pv);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+        for (int b = 0; b < kGroupMax; ++b) {
+            if ((amask >> b) & 1u) {
                 float v = d.e[u];
 #pragma unroll
                 for (int t = 0; t < NT; ++t) v += G.mu[b][t] * d.a[u][t];
-                const float x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(R[b], d.off[u], 0, 0));
-                acc[b] += v * x;
+                acc[b] += v * pv[b];
             }
         }
     }
@@ -364,32 +376,32 @@ __device__ inline void applyQuad(const BGroup& G, unsigned amask, const __amdgpu
 // one slice row, the next quad's matrix data loaded while the current quad's
 // gathers and products run
 template <int NT, bool NARROW>
-__device__ inline void groupRow(const BGroup& G, unsigned amask, const __amdgpu_buffer_rsrc_t* P, size_t q0,
-                                const void* cp, int32_t cb, int nq, float* acc) {
+__device__ inline void groupRow(const BGroup& G, unsigned amask, size_t q0, const void* cp, int32_t cb, int nq,
+                                float* acc) {
     if (nq <= 0) return;
     GroupQuad<NT> cur;
     loadQuad<NT, NARROW>(G, q0, cp, cb, 0, cur);
     for (int k = 1; k < nq; ++k) {
         GroupQuad<NT> nxt;
         loadQuad<NT, NARROW>(G, q0, cp, cb, k, nxt);
-        applyQuad<NT>(G, amask, P, cur, acc);
+        applyQuad<NT>(G, amask, cur, acc);
         cur = nxt;
     }
-    applyQuad<NT>(G, amask, P, cur, acc);
+    applyQuad<NT>(G, amask, cur, acc);
 }
 
 // Grouped SpMV of an affine family (FP32 inner iterations): the in-flight
 // instances of one family share the sparsity pattern and the term matrices, so
 // a group streams each slot's FP32 term values, Dirichlet mask and column once
 // and every instance b combines a = sum_t mu_bt a_t on the fly (mask 1: zero,
-// mask 2: unit diagonal) before w_b += a * p_b[col].  Matrix traffic per
-// iteration drops from ninst x (value + column) to (T values + mask + column).
-// Each instance's arithmetic is independent of the others in the group, so a
-// result is the same alone and in any mix.
+// mask 2: unit diagonal) before w_b += a * p_b[col], the p of all instances at
+// a column coming from one 32-byte load of the interleaved directions.  Each
+// instance's arithmetic is independent of the others in the group, so a result
+// is the same alone and in any mix.
 template <int NT>
 __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restrict__ gitems,
-                                                     const BGroup* __restrict__ groups,
-                                                     const BInst* __restrict__ insts) {
+                                                          const BGroup* __restrict__ groups,
+                                                          const BInst* __restrict__ insts) {
     const BGItem it = gitems[blockIdx.x];
     if (it.group < 0) return;
     const BGroup& G = groups[it.group];
@@ -397,21 +409,14 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
     __shared__ double red[kGroupBlock / 64][kGroupMax];
     if (threadIdx.x < 64) {
         const int b = threadIdx.x;
-        const bool on = b < G.ninst && !insts[G.inst[b < G.ninst ? b : 0]].st->done;
+        const int ii = b < kGroupMax ? G.inst[b] : -1;
+        const bool on = ii >= 0 && !insts[ii].st->done;
         const unsigned long long bal = __ballot(on);
         if (threadIdx.x == 0) actMask = (unsigned)bal;
     }
     __syncthreads();
     const unsigned amask = __builtin_amdgcn_readfirstlane(actMask);
     if (!amask) return;
-    __amdgpu_buffer_rsrc_t R[kGroupMax];
-    const int32_t bytes = (int32_t)(G.n * (int64_t)sizeof(float));
-#pragma unroll
-    for (int b = 0; b < kGroupMax; ++b) {
-        const bool on = (amask >> b) & 1u;
-        // raw buffer resource over p_b: base, stride 0, byte extent, gfx9 data format word
-        R[b] = __builtin_amdgcn_make_buffer_rsrc(on ? insts[G.inst[b]].pT : nullptr, (short)0, bytes, 0x00020000);
-    }
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double dot[kGroupMax];
 #pragma unroll
@@ -425,22 +430,22 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
         for (int b = 0; b < kGroupMax; ++b) acc[b] = 0.0f;
         const int32_t cb = G.compact ? G.cols.colBase[sl] : -1;
         if (cb >= 0)
-            groupRow<NT, true>(G, amask, R, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
-                               cb, nq, acc);
+            groupRow<NT, true>(G, amask, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb,
+                               nq, acc);
         else if (G.compact)
-            groupRow<NT, false>(G, amask, R, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
+            groupRow<NT, false>(G, amask, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
                                 cb, nq, acc);
         else
-            groupRow<NT, false>(G, amask, R, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
+            groupRow<NT, false>(G, amask, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
         const int64_t r = (int64_t)sl * 64 + lane;
         if (r < G.n) {
+            float pr[kGroupMax];
+            groupP(G.P, r, pr);
 #pragma unroll
             for (int b = 0; b < kGroupMax; ++b)
                 if ((amask >> b) & 1u) {
                     tp<float>(insts[G.inst[b]].wT)[r] = acc[b];
-                    const float pr = __builtin_bit_cast(
-                        float, __builtin_amdgcn_raw_buffer_load_b32(R[b], (int32_t)(r * (int64_t)sizeof(float)), 0, 0));
-                    dot[b] += (double)pr * (double)acc[b];
+                    dot[b] += (double)pr[b] * (double)acc[b];
                 }
         }
     }
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ ite
     const int it = *gcount - I.start;
     const double dpi = instReduce(I.part + I.apOff, I.nap);
     const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
-    if (bid == I.ibeg && threadIdx.x == 0) {
+    if (item.item == 0 && threadIdx.x == 0) {
         I.slots[2 + (it & 1)] = dpi;
         if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
     }
@@ -483,9 +488,10 @@ __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ ite
     itemRows(I, item, r0, r1);
     T *x = tp<T>(I.xT), *r = tp<T>(I.rT), *z = tp<T>(I.zT);
     const T *p = tp<T>(I.pT), *w = tp<T>(I.wT), *dinv = tp<T>(I.dinvT);
+    const int ps = I.pstride;
     double zr = 0.0, zz = 0.0;
     for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-        x[i] = (T)((double)x[i] + a * (double)p[i]);
+        x[i] = (T)((double)x[i] + a * (double)p[i * ps]);
         const T ri = (T)((double)r[i] - a * (double)w[i]);
         r[i] = ri;
         const T zi = dinv[i] * ri;
@@ -495,7 +501,7 @@ __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ ite
     }
     zr = blockReduceSum(zr);
     zz = blockReduceSum(zz);
-    if (threadIdx.x == 0) { I.part[bid - I.ibeg] = zr; I.part[I.nitems + bid - I.ibeg] = zz; }
+    if (threadIdx.x == 0) { I.part[item.item] = zr; I.part[I.nitems + item.item] = zz; }
 }
 
 __global__ void kBBump(int* gcount) {
@@ -522,6 +528,14 @@ struct BatchFamily {
     int nitems = 0;
     size_t instBytes = 0;
     std::vector<int> freeBufs;    // reusable instance buffer sets
+    // family groups of the grouped FP32 SpMV: interleaved search directions
+    // P[n x 8] and the request holding each slot (-1 = free); an instance keeps
+    // its slot from admission until it finishes
+    struct PGroup {
+        float* P = nullptr;
+        int64_t req[kGroupMax];
+    };
+    std::vector<PGroup> pgroups;
 };
 
 struct InstBuffers {
@@ -547,6 +561,7 @@ struct Request {
     int fresh = 0;         // iteration at which the search direction last (re)started
     double devRnorm0 = 0;  // the device's initial residual norm (FP32 units), for the dtol test
     bool unpermuted = false;   // solution copied to Feel++ numbering
+    int pgroup = -1, pslot = -1;   // family group slot of the grouped FP32 SpMV
 };
 
 struct feel_hip_batch {
@@ -667,6 +682,11 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
         I.val64 = ib.val64; I.dinv64 = ib.dinv64; I.x64 = ib.x64; I.r64 = ib.r64; I.w64 = ib.w64;
         I.valT = ib.valT; I.dinvT = ib.dinvT; I.xT = ib.xT; I.rT = ib.rT; I.zT = ib.zT; I.pT = ib.pT;
         I.wT = ib.wT; I.bT = ib.bT;
+        I.pstride = 1;
+        if (r->pgroup >= 0) {
+            I.pT = f.pgroups[r->pgroup].P + r->pslot;
+            I.pstride = kGroupMax;
+        }
         I.st = B->states + r->state;
         I.slots = B->slots + 4 * (size_t)r->state;
         I.part = ib.part;
@@ -676,40 +696,55 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
         I.start = r->start;
         I.scale = r->sigma > 0 ? 1.0 / r->sigma : 0.0;
         I.xscale = r->xscale;
-        I.ibeg = (int)items.size();
-        for (int64_t s = 0; s < f.nslices; s += kItemSlices)
-            items.push_back({(int)insts.size(), (int)s, (int)std::min<int64_t>(f.nslices, s + kItemSlices)});
-        I.iend = (int)items.size();
+        // grouped instances get their items below, interleaved by group
+        if (r->pgroup < 0)
+            for (int64_t s = 0, j = 0; s < f.nslices; s += kItemSlices, ++j)
+                items.push_back({(int)insts.size(), (int)s, (int)std::min<int64_t>(f.nslices, s + kItemSlices), (int)j});
         insts.push_back(I);
     }
+    // Items of the instances of one family group: for each slice range, the
+    // group's instances side by side, so the strided accesses of the direction
+    // and update kernels to the interleaved P meet in L2.
+    {
+        std::map<std::pair<int, int>, std::vector<int>> byGroup;   // (family, group) -> positions in rs
+        for (size_t i = 0; i < rs.size(); ++i)
+            if (rs[i]->pgroup >= 0) byGroup[{rs[i]->family, rs[i]->pgroup}].push_back((int)i);
+        for (auto& kv : byGroup) {
+            const BatchFamily& f = B->fams[kv.first.first];
+            for (int64_t s = 0, j = 0; s < f.nslices; s += kItemSlices, ++j)
+                for (int i : kv.second)
+                    items.push_back({i, (int)s, (int)std::min<int64_t>(f.nslices, s + kItemSlices), (int)j});
+        }
+    }
     if (groups) {
-        std::map<int, std::vector<int>> byFamily;
-        for (size_t i = 0; i < rs.size(); ++i) byFamily[rs[i]->family].push_back((int)i);
-        for (auto& kv : byFamily) {
-            const BatchFamily& f = B->fams[kv.first];
-            const std::vector<int>& members = kv.second;
-            const int g0 = (int)grp.size();
-            for (size_t m0 = 0; m0 < members.size(); m0 += kGroupMax) {
-                BGroup G{};
-                G.ninst = (int)std::min<size_t>(kGroupMax, members.size() - m0);
-                G.nterms = f.nterms;
-                G.compact = f.g->compact ? 1 : 0;
-                G.nitems = f.nitems;
-                G.n = f.n;
-                for (int b = 0; b < G.ninst; ++b) {
-                    const int i = members[m0 + b];
-                    G.inst[b] = i;
-                    for (int t = 0; t < f.nterms; ++t) G.mu[b][t] = (float)rs[i]->mu[t];
-                }
-                for (int t = 0; t < kMaxTerms; ++t) G.terms[t] = f.termsF[t < f.nterms ? t : 0];
-                G.mask = f.mask;
-                G.cols = SellCols{f.g->slicePtr, f.g->scol, f.g->colBase, f.g->ccptr, f.g->packed};
-                grp.push_back(G);
+        // one BGroup per family group with members in this launch
+        std::map<int64_t, int> pos;   // request id -> position in rs
+        for (size_t i = 0; i < rs.size(); ++i) pos[rs[i]->id] = (int)i;
+        std::map<std::pair<int, int>, bool> seen;
+        for (Request* r : rs) {
+            if (r->pgroup < 0 || seen.count({r->family, r->pgroup})) continue;
+            seen[{r->family, r->pgroup}] = true;
+            const BatchFamily& f = B->fams[r->family];
+            const BatchFamily::PGroup& pg = f.pgroups[r->pgroup];
+            BGroup G{};
+            G.nterms = f.nterms;
+            G.compact = f.g->compact ? 1 : 0;
+            G.nitems = f.nitems;
+            G.n = f.n;
+            G.P = pg.P;
+            for (int b = 0; b < kGroupMax; ++b) {
+                auto it = pg.req[b] >= 0 ? pos.find(pg.req[b]) : pos.end();
+                G.inst[b] = it == pos.end() ? -1 : it->second;
+                if (G.inst[b] >= 0)
+                    for (int t = 0; t < f.nterms; ++t) G.mu[b][t] = (float)rs[G.inst[b]]->mu[t];
             }
-            for (int g = g0; g < (int)grp.size(); ++g)
-                for (int64_t s = 0; s < f.nslices; s += kGroupSlices)
-                    gitems.push_back({g, (int)s, (int)std::min<int64_t>(f.nslices, s + kGroupSlices),
-                                      (int)(s / kGroupSlices)});
+            for (int t = 0; t < kMaxTerms; ++t) G.terms[t] = f.termsF[t < f.nterms ? t : 0];
+            G.mask = f.mask;
+            G.cols = SellCols{f.g->slicePtr, f.g->scol, f.g->colBase, f.g->ccptr, f.g->packed};
+            const int g = (int)grp.size();
+            grp.push_back(G);
+            for (int64_t s = 0; s < f.nslices; s += kGroupSlices)
+                gitems.push_back({g, (int)s, (int)std::min<int64_t>(f.nslices, s + kGroupSlices), (int)(s / kGroupSlices)});
         }
         // Workgroups go to the XCDs round robin (block i to XCD i % 8): give each
         // XCD one contiguous eighth of the (group, slice range) sequence, so the
@@ -832,10 +867,48 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
     return FEEL_HIP_OK;
 }
 
+// a family group slot for a request of the grouped FP32 SpMV (a new group
+// when every slot of the family's groups is taken)
+bool hasFreeSlot(const BatchFamily& f) {
+    for (const auto& pg : f.pgroups)
+        for (int b = 0; b < kGroupMax; ++b)
+            if (pg.req[b] < 0) return true;
+    return false;
+}
+
+size_t groupBytes(const BatchFamily& f) { return sizeof(float) * kGroupMax * (size_t)f.n; }
+
+int assignSlot(feel_hip_batch* B, Request& r) {
+    BatchFamily& f = B->fams[r.family];
+    for (size_t g = 0; g < f.pgroups.size(); ++g)
+        for (int b = 0; b < kGroupMax; ++b)
+            if (f.pgroups[g].req[b] < 0) {
+                f.pgroups[g].req[b] = r.id;
+                r.pgroup = (int)g;
+                r.pslot = b;
+                return FEEL_HIP_OK;
+            }
+    BatchFamily::PGroup pg;
+    for (int b = 0; b < kGroupMax; ++b) pg.req[b] = -1;
+    TRY_CALL(B->ctx->alloc((void**)&pg.P, groupBytes(f)));
+    // slots without an instance are read (one 32-byte load serves all eight) but never used
+    FEEL_HIP_CHECK(hipMemsetAsync(pg.P, 0, groupBytes(f), B->ctx->stream));
+    B->used += (int64_t)groupBytes(f);
+    pg.req[0] = r.id;
+    f.pgroups.push_back(pg);
+    r.pgroup = (int)f.pgroups.size() - 1;
+    r.pslot = 0;
+    return FEEL_HIP_OK;
+}
+
 void finish(feel_hip_batch* B, Request& r, int reason) {
     r.reason = reason;
     r.phase = Phase::Done;
     B->finished.push_back(r.id);
+    if (r.pgroup >= 0) {   // the search direction is no longer needed
+        B->fams[r.family].pgroups[r.pgroup].req[r.pslot] = -1;
+        r.pgroup = r.pslot = -1;
+    }
 }
 
 template <typename T>
@@ -848,12 +921,16 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         Request& r = B->reqs[B->pending.front()];
         BatchFamily& f = B->fams[r.family];
         const bool reuse = !f.freeBufs.empty();
-        if (!reuse && B->opts.hbm_budget > 0 && B->used + (int64_t)instanceBytes(B, f) > B->opts.hbm_budget) {
+        const bool grouped = sizeof(T) == 4 && B->opts.group_spmv;
+        const int64_t need = (reuse ? 0 : (int64_t)instanceBytes(B, f)) +
+                             (grouped && !hasFreeSlot(f) ? (int64_t)groupBytes(f) : 0);
+        if (need > 0 && B->opts.hbm_budget > 0 && B->used + need > B->opts.hbm_budget) {
             if (B->used == 0) return setError(FEEL_HIP_ENOMEM, "one instance exceeds the batch HBM budget");
             break;
         }
         TRY_CALL(ensureStates(B, B->nStates - (int)B->freeStates.size() + 1));
         TRY_CALL(allocInstance(B, r.family, &r.buf));
+        if (grouped) TRY_CALL(assignSlot(B, r));
         r.state = B->freeStates.back();
         B->freeStates.pop_back();
         admitted.push_back(&r);
@@ -1023,7 +1100,9 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
         for (double* p : {ib.val64, ib.dinv64, ib.x64, ib.r64, ib.w64, ib.out, ib.part}) if (p) (void)hipFree(p);
         for (void* p : {ib.valT, ib.dinvT, ib.xT, ib.rT, ib.zT, ib.pT, ib.wT, ib.bT}) if (p) (void)hipFree(p);
     }
-    c->bytes -= B->used;   // instance buffers were counted through ctx->alloc
+    for (auto& f : B->fams)
+        for (auto& pg : f.pgroups) (void)hipFree(pg.P);
+    c->bytes -= B->used;   // instance and group buffers were counted through ctx->alloc
     for (auto& f : B->fams) {
         for (double* p : f.terms) c->release(p, sizeof(double) * f.nslots);
         for (float* p : f.termsF) c->release(p, sizeof(float) * f.nslots);
