@@ -467,6 +467,148 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
     }
 }
 
+// Direction and update kernels of a family group: one block covers one item
+// (64 slices) of every instance of the group, so the interleaved P is read
+// and written in whole 32-byte rows.  Per instance the operations, their
+// order and the rows each thread handles are those of kBDirection and
+// kBUpdate, so the results are the same as theirs, bit for bit.
+__global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restrict__ gitems,
+                                                          const BGroup* __restrict__ groups,
+                                                          const BInst* __restrict__ insts, const int* __restrict__ gcount) {
+    const BGItem item = gitems[blockIdx.x];
+    if (item.group < 0) return;
+    const BGroup& G = groups[item.group];
+    __shared__ int sdone[kGroupMax];
+    __shared__ int smode[kGroupMax];      // 0 skip, 1 p = z + bb p, 2 p = z
+    __shared__ double sbb[kGroupMax];
+    if (threadIdx.x < kGroupMax) {
+        const int ii = G.inst[threadIdx.x];
+        sdone[threadIdx.x] = ii < 0 ? 1 : insts[ii].st->done;
+        smode[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int b = 0; b < kGroupMax; ++b) {
+        if (sdone[b]) continue;
+        const BInst& I = insts[G.inst[b]];
+        CgState* st = I.st;
+        const int it = *gcount - I.start;
+        const double beta = instReduce(I.part, I.nitems);
+        const double dp = sqrt(instReduce(I.part + I.nitems, I.nitems));
+        const double ttol = it == 0 ? fmax(st->rtol * dp, st->atol) : st->ttol;
+        const double rnorm0 = it == 0 ? dp : st->rnorm0;
+        int reason = 0;
+        if (dp != dp) reason = -9;
+        else if (dp <= ttol) reason = dp < st->atol ? 3 : 2;
+        else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
+        else if (it >= st->maxit) reason = -3;
+        else if (beta == 0.0) reason = 3;
+        if (item.item == 0 && threadIdx.x == 0) {
+            st->dp = dp;
+            if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
+            if (reason) { st->done = 1; st->reason = reason; st->its = it; }
+            I.slots[it & 1] = beta;
+        }
+        if (!reason && threadIdx.x == 0) {
+            const bool first = it == st->fresh;
+            smode[b] = first ? 2 : 1;
+            sbb[b] = first ? 0.0 : beta / I.slots[(it - 1) & 1];
+        }
+    }
+    __syncthreads();
+    const float* z[kGroupMax];
+    int mode[kGroupMax];
+    double bb[kGroupMax];
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) {
+        mode[b] = smode[b];
+        bb[b] = sbb[b];
+        z[b] = mode[b] ? tp<float>(insts[G.inst[b]].zT) : nullptr;
+    }
+    const int64_t r0 = (int64_t)item.s0 * 64, r1 = std::min<int64_t>((int64_t)item.s1 * 64, G.n);
+    f32x4* P = reinterpret_cast<f32x4*>(const_cast<float*>(G.P));
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        f32x4 lo = P[2 * r], hi = P[2 * r + 1];
+        float pv[kGroupMax] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int b = 0; b < kGroupMax; ++b)
+            if (mode[b]) pv[b] = mode[b] == 2 ? z[b][r] : (float)((double)z[b][r] + bb[b] * (double)pv[b]);
+        lo = f32x4{pv[0], pv[1], pv[2], pv[3]};
+        hi = f32x4{pv[4], pv[5], pv[6], pv[7]};
+        P[2 * r] = lo;
+        P[2 * r + 1] = hi;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict__ gitems,
+                                                       const BGroup* __restrict__ groups,
+                                                       const BInst* __restrict__ insts, const int* __restrict__ gcount) {
+    const BGItem item = gitems[blockIdx.x];
+    if (item.group < 0) return;
+    const BGroup& G = groups[item.group];
+    __shared__ int sdone[kGroupMax];
+    __shared__ int sact[kGroupMax];
+    __shared__ double sa[kGroupMax];
+    if (threadIdx.x < kGroupMax) {
+        const int ii = G.inst[threadIdx.x];
+        sdone[threadIdx.x] = ii < 0 ? 1 : insts[ii].st->done;
+        sact[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int b = 0; b < kGroupMax; ++b) {
+        if (sdone[b]) continue;
+        const BInst& I = insts[G.inst[b]];
+        CgState* st = I.st;
+        const int it = *gcount - I.start;
+        const double dpi = instReduce(I.part + I.apOff, I.nap);
+        const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
+        if (item.item == 0 && threadIdx.x == 0) {
+            I.slots[2 + (it & 1)] = dpi;
+            if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
+        }
+        if (!indefinite && threadIdx.x == 0) {
+            sact[b] = 1;
+            sa[b] = I.slots[it & 1] / dpi;
+        }
+    }
+    __syncthreads();
+    int act[kGroupMax];
+    double a[kGroupMax], zr[kGroupMax], zz[kGroupMax];
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) {
+        act[b] = sact[b];
+        a[b] = sa[b];
+        zr[b] = 0.0;
+        zz[b] = 0.0;
+    }
+    const int64_t r0 = (int64_t)item.s0 * 64, r1 = std::min<int64_t>((int64_t)item.s1 * 64, G.n);
+    const f32x4* P = reinterpret_cast<const f32x4*>(G.P);
+    for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+        const f32x4 lo = P[2 * i], hi = P[2 * i + 1];
+        const float pv[kGroupMax] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int b = 0; b < kGroupMax; ++b) {
+            if (!act[b]) continue;
+            const BInst& I = insts[G.inst[b]];
+            float *x = tp<float>(I.xT), *r = tp<float>(I.rT), *z = tp<float>(I.zT);
+            const float *w = tp<float>(I.wT), *dinv = tp<float>(I.dinvT);
+            x[i] = (float)((double)x[i] + a[b] * (double)pv[b]);
+            const float ri = (float)((double)r[i] - a[b] * (double)w[i]);
+            r[i] = ri;
+            const float zi = dinv[i] * ri;
+            z[i] = zi;
+            zr[b] += (double)zi * (double)ri;
+            zz[b] += (double)zi * (double)zi;
+        }
+    }
+    for (int b = 0; b < kGroupMax; ++b) {
+        if (!act[b]) continue;
+        const double vzr = blockReduceSum(zr[b]);
+        const double vzz = blockReduceSum(zz[b]);
+        const BInst& I = insts[G.inst[b]];
+        if (threadIdx.x == 0) { I.part[item.item] = vzr; I.part[I.nitems + item.item] = vzz; }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ items, const BInst* __restrict__ insts,
                                                   const int* __restrict__ gcount) {
@@ -583,12 +725,14 @@ struct feel_hip_batch {
     BGroup* dGroups = nullptr;
     BGItem* dGItems = nullptr;
     int capGroups = 0, capGItems = 0, nGItems = 0;
+    BGItem* dGDItems = nullptr;   // group items of the direction / update kernels (64 slices each)
+    int capGDItems = 0, nGDItems = 0;
     double* dSums = nullptr;
     int capItems = 0, capInsts = 0;
     int* gcount = nullptr;
     int hgcount = 0;
     hipGraphExec_t exec = nullptr;
-    int execItems = -1, execChunk = -1, execGItems = -1, execTerms = -1;
+    int execItems = -1, execChunk = -1, execGItems = -1, execGDItems = -1, execTerms = -1;
     uint64_t execVersion = ~0ULL, version = 0;
 };
 
@@ -663,7 +807,7 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
     std::vector<BItem> items;
     std::vector<BInst> insts;
     std::vector<BGroup> grp;
-    std::vector<BGItem> gitems;
+    std::vector<BGItem> gitems, gditems;
     for (Request* r : rs) {
         const BatchFamily& f = B->fams[r->family];
         const InstBuffers& ib = B->bufs[r->buf];
@@ -745,6 +889,8 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
             grp.push_back(G);
             for (int64_t s = 0; s < f.nslices; s += kGroupSlices)
                 gitems.push_back({g, (int)s, (int)std::min<int64_t>(f.nslices, s + kGroupSlices), (int)(s / kGroupSlices)});
+            for (int64_t s = 0, j = 0; s < f.nslices; s += kItemSlices, ++j)
+                gditems.push_back({g, (int)s, (int)std::min<int64_t>(f.nslices, s + kItemSlices), (int)j});
         }
         // Workgroups go to the XCDs round robin (block i to XCD i % 8): give each
         // XCD one contiguous eighth of the (group, slice range) sequence, so the
@@ -785,12 +931,24 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
         TRY_CALL(alloc(c, &B->dGItems, B->capGItems));
         ++B->version;
     }
+    if ((int)gditems.size() > B->capGDItems) {
+        c->release(B->dGDItems, sizeof(BGItem) * B->capGDItems);
+        B->capGDItems = (int)gditems.size() * 2;
+        TRY_CALL(alloc(c, &B->dGDItems, B->capGDItems));
+        ++B->version;
+    }
+    if (!gditems.empty())
+        FEEL_HIP_CHECK(hipMemcpyAsync(B->dGDItems, gditems.data(), sizeof(BGItem) * gditems.size(), hipMemcpyHostToDevice,
+                                      c->stream));
     if (!grp.empty()) {
         FEEL_HIP_CHECK(hipMemcpyAsync(B->dGroups, grp.data(), sizeof(BGroup) * grp.size(), hipMemcpyHostToDevice, c->stream));
         FEEL_HIP_CHECK(
             hipMemcpyAsync(B->dGItems, gitems.data(), sizeof(BGItem) * gitems.size(), hipMemcpyHostToDevice, c->stream));
     }
-    if (groups) B->nGItems = (int)gitems.size();
+    if (groups) {
+        B->nGItems = (int)gitems.size();
+        B->nGDItems = (int)gditems.size();
+    }
     if (!items.empty()) {
         FEEL_HIP_CHECK(hipMemcpyAsync(B->dItems, items.data(), sizeof(BItem) * items.size(), hipMemcpyHostToDevice, c->stream));
         FEEL_HIP_CHECK(hipMemcpyAsync(B->dInsts, insts.data(), sizeof(BInst) * insts.size(), hipMemcpyHostToDevice, c->stream));
@@ -833,10 +991,19 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
     const bool grouped = sizeof(T) == 4 && nterms > 0;
     auto body = [&]() {
         for (int k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL(kBDirection<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts, B->gcount);
-            if (grouped) launchGroupSpmv(B, nterms);
-            else hipLaunchKernelGGL(kBSpmv<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
-            hipLaunchKernelGGL(kBUpdate<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts, B->gcount);
+            if (grouped) {
+                hipLaunchKernelGGL(kBDirectionGroup, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems,
+                                   B->dGroups, B->dInsts, B->gcount);
+                launchGroupSpmv(B, nterms);
+                hipLaunchKernelGGL(kBUpdateGroup, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems, B->dGroups,
+                                   B->dInsts, B->gcount);
+            } else {
+                hipLaunchKernelGGL(kBDirection<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts,
+                                   B->gcount);
+                hipLaunchKernelGGL(kBSpmv<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
+                hipLaunchKernelGGL(kBUpdate<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts,
+                                   B->gcount);
+            }
             hipLaunchKernelGGL(kBBump, dim3(1), dim3(64), 0, c->stream, B->gcount);
         }
     };
@@ -847,8 +1014,9 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
     }
     // the captured chunk reads the work list, descriptors and counter from device
     // memory, so it stays valid while those buffers keep their addresses
-    const int gItems = grouped ? B->nGItems : 0;
-    if (!B->exec || B->execItems != nItems || B->execGItems != gItems || B->execTerms != nterms ||
+    const int gItems = grouped ? B->nGItems : 0, gdItems = grouped ? B->nGDItems : 0;
+    if (!B->exec || B->execItems != nItems || B->execGItems != gItems || B->execGDItems != gdItems ||
+        B->execTerms != nterms ||
         B->execChunk != chunk || B->execVersion != B->version) {
         if (B->exec) { (void)hipGraphExecDestroy(B->exec); B->exec = nullptr; }
         hipGraph_t graph;
@@ -859,6 +1027,7 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
         (void)hipGraphDestroy(graph);
         B->execItems = nItems;
         B->execGItems = gItems;
+        B->execGDItems = gdItems;
         B->execTerms = nterms;
         B->execChunk = chunk;
         B->execVersion = B->version;
@@ -1116,6 +1285,7 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
     c->release(B->dInsts, sizeof(BInst) * B->capInsts);
     c->release(B->dGroups, sizeof(BGroup) * B->capGroups);
     c->release(B->dGItems, sizeof(BGItem) * B->capGItems);
+    c->release(B->dGDItems, sizeof(BGItem) * B->capGDItems);
     c->release(B->dSums, sizeof(double) * B->capInsts);
     c->release(B->gcount, sizeof(int));
     delete B;
