@@ -527,11 +527,15 @@ __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restr
     const int64_t r0 = (int64_t)item.s0 * 64, r1 = std::min<int64_t>((int64_t)item.s1 * 64, G.n);
     f32x4* P = reinterpret_cast<f32x4*>(const_cast<float*>(G.P));
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+        // every load of the row before any store: the instances' loads overlap
+        float zv[kGroupMax];
+#pragma unroll
+        for (int b = 0; b < kGroupMax; ++b) zv[b] = mode[b] ? z[b][r] : 0.0f;
         f32x4 lo = P[2 * r], hi = P[2 * r + 1];
         float pv[kGroupMax] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b)
-            if (mode[b]) pv[b] = mode[b] == 2 ? z[b][r] : (float)((double)z[b][r] + bb[b] * (double)pv[b]);
+            if (mode[b]) pv[b] = mode[b] == 2 ? zv[b] : (float)((double)zv[b] + bb[b] * (double)pv[b]);
         lo = f32x4{pv[0], pv[1], pv[2], pv[3]};
         hi = f32x4{pv[4], pv[5], pv[6], pv[7]};
         P[2 * r] = lo;
@@ -571,31 +575,53 @@ __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict
         }
     }
     __syncthreads();
+    // active set as a wave-uniform mask, so the per-instance pointers stay scalar
+    unsigned m = 0;
+    for (int b = 0; b < kGroupMax; ++b) m |= (unsigned)(sact[b] != 0) << b;
+    const unsigned amask = __builtin_amdgcn_readfirstlane(m);
     int act[kGroupMax];
     double a[kGroupMax], zr[kGroupMax], zz[kGroupMax];
 #pragma unroll
     for (int b = 0; b < kGroupMax; ++b) {
-        act[b] = sact[b];
+        act[b] = (amask >> b) & 1u;
         a[b] = sa[b];
         zr[b] = 0.0;
         zz[b] = 0.0;
     }
     const int64_t r0 = (int64_t)item.s0 * 64, r1 = std::min<int64_t>((int64_t)item.s1 * 64, G.n);
     const f32x4* P = reinterpret_cast<const f32x4*>(G.P);
+    float *X[kGroupMax], *Rv[kGroupMax], *Z[kGroupMax];
+    const float *W[kGroupMax], *D[kGroupMax];
+#pragma unroll
+    for (int b = 0; b < kGroupMax; ++b) {
+        const BInst* I = act[b] ? &insts[G.inst[b]] : nullptr;
+        X[b] = I ? tp<float>(I->xT) : nullptr;
+        Rv[b] = I ? tp<float>(I->rT) : nullptr;
+        Z[b] = I ? tp<float>(I->zT) : nullptr;
+        W[b] = I ? tp<float>(I->wT) : nullptr;
+        D[b] = I ? tp<float>(I->dinvT) : nullptr;
+    }
     for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
         const f32x4 lo = P[2 * i], hi = P[2 * i + 1];
         const float pv[kGroupMax] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        // every load of the row before any store: the instances' loads overlap
+        float xv[kGroupMax], rv[kGroupMax], wv[kGroupMax], dv[kGroupMax];
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b) {
             if (!act[b]) continue;
-            const BInst& I = insts[G.inst[b]];
-            float *x = tp<float>(I.xT), *r = tp<float>(I.rT), *z = tp<float>(I.zT);
-            const float *w = tp<float>(I.wT), *dinv = tp<float>(I.dinvT);
-            x[i] = (float)((double)x[i] + a[b] * (double)pv[b]);
-            const float ri = (float)((double)r[i] - a[b] * (double)w[i]);
-            r[i] = ri;
-            const float zi = dinv[i] * ri;
-            z[i] = zi;
+            xv[b] = X[b][i];
+            rv[b] = Rv[b][i];
+            wv[b] = W[b][i];
+            dv[b] = D[b][i];
+        }
+#pragma unroll
+        for (int b = 0; b < kGroupMax; ++b) {
+            if (!act[b]) continue;
+            X[b][i] = (float)((double)xv[b] + a[b] * (double)pv[b]);
+            const float ri = (float)((double)rv[b] - a[b] * (double)wv[b]);
+            Rv[b][i] = ri;
+            const float zi = dv[b] * ri;
+            Z[b][i] = zi;
             zr[b] += (double)zi * (double)ri;
             zz[b] += (double)zi * (double)zi;
         }
