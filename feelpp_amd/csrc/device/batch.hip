@@ -1330,8 +1330,6 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
     }
     if (rhs->n != g->n) return setError(FEEL_HIP_EINVAL, "rhs size mismatch");
     if (dirichlet && dirichlet->n != g->n) return setError(FEEL_HIP_EINVAL, "Dirichlet rows built for another size");
-    if (B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv && g->n > (int64_t)(INT32_MAX / sizeof(float)))
-        return setError(FEEL_HIP_EUNSUPPORTED, "grouped SpMV addresses at most 2^29 rows; set group_spmv = 0");
     feel_hip_ctx* c = B->ctx;
     BatchFamily f;
     f.g = g;
