@@ -1329,6 +1329,7 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
         if (!terms[t]->v64) return setError(FEEL_HIP_EINVAL, "terms need FP64 values");
     }
     if (rhs->n != g->n) return setError(FEEL_HIP_EINVAL, "rhs size mismatch");
+    if (g->block) return setError(FEEL_HIP_EUNSUPPORTED, "batched families need a scalar space");
     if (dirichlet && dirichlet->n != g->n) return setError(FEEL_HIP_EINVAL, "Dirichlet rows built for another size");
     feel_hip_ctx* c = B->ctx;
     BatchFamily f;

@@ -52,6 +52,8 @@ int buildDofTableDevice(feel_hip_space* s);
 int buildIncidenceDevice(feel_hip_space* s);
 // the solver's SELL-C-sigma image of g for the row order perm (sell.hip)
 int buildSellDevice(feel_hip_graph* g, const std::vector<int32_t>& perm);
+// the 3 x 3 block image of a vector-space graph for the node order nperm (sell.hip)
+int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm);
 void retain(feel_hip_ctx* c);
 void retain(feel_hip_space* s);
 void retain(feel_hip_graph* g);
@@ -148,6 +150,8 @@ struct feel_hip_graph {
     int32_t* perm = nullptr;      // new -> old row
     int32_t* sdiag = nullptr;     // diagonal slot per new row
     bool compact = false;         // packed 16/32-bit column stream available
+    bool block = false;           // 3 x 3 block image (vector spaces): slicePtr/scol in block slots
+    int64_t nbslots = 0;          // block slots of the block image (nslots = 9 nbslots value slots)
     int32_t* colBase = nullptr;   // nslices, -1 = slice stores int32 columns
     int64_t* ccptr = nullptr;     // nslices, slice start in packed (uint16 units)
     uint16_t* packed = nullptr;   // npacked

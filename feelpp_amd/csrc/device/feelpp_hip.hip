@@ -174,7 +174,7 @@ void unref(feel_hip_graph* g) {
     c->release(g->taskK, sizeof(uint32_t) * (g->ntasks + 1));
     if (g->segSlot) c->release(g->segSlot, sizeof(int32_t) * g->nseg);
     c->release(g->slicePtr, sizeof(uint32_t) * (g->nslices + 1));
-    c->release(g->scol, sizeof(int32_t) * g->nslots);
+    c->release(g->scol, sizeof(int32_t) * (g->block ? g->nbslots : g->nslots));
     c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
     c->release(g->perm, sizeof(int32_t) * g->n);
     c->release(g->sdiag, sizeof(int32_t) * g->n);
@@ -483,6 +483,26 @@ int ensureGeo(feel_hip_space* s, size_t bytes) {
 // ============================================================== graph
 namespace {
 
+// A vector-space pattern is 3 x 3 blocked when the three rows of every node
+// have the same length, hold columns in whole node triples 3j, 3j+1, 3j+2 and
+// list the same nodes in the same order.
+bool blockedPattern(const std::vector<int32_t>& ia, const std::vector<int32_t>& ja) {
+    const int64_t n = (int64_t)ia.size() - 1;
+    if (n % 3 != 0) return false;
+    for (int64_t i = 0; i < n; i += 3) {
+        const int32_t len = ia[i + 1] - ia[i];
+        if (len % 3 != 0) return false;
+        for (int c = 0; c < 3; ++c) {
+            if (ia[i + c + 1] - ia[i + c] != len) return false;
+            for (int32_t k = 0; k < len; ++k) {
+                const int32_t col = ja[ia[i + c] + k];
+                if (col != ja[ia[i] + k] || (k % 3 == 0 ? col % 3 != 0 : col != ja[ia[i] + k - 1] + 1)) return false;
+            }
+        }
+    }
+    return true;
+}
+
 // diagonal positions (from the host copy of the pattern, rows with their
 // diagonal) and the solver's SELL-C-sigma image (row order on the host, the
 // image on the device)
@@ -500,6 +520,21 @@ int finishGraph(feel_hip_graph* g) {
         diag[r] = (int32_t)(p - ja);
     }
     TRY_CALL(upload(ctx, &g->diagpos, diag.data(), diag.size()));
+    if (g->space->ncomp == 3 && blockedPattern(g->hia, g->hja)) {
+        // vector space: node graph (the first row of each node, every third column)
+        const int64_t nb = nrows / 3;
+        GraphCSR ng;
+        ng.nrows = nb;
+        ng.ia.resize(nb + 1);
+        ng.ia[0] = 0;
+        for (int64_t i = 0; i < nb; ++i) ng.ia[i + 1] = ng.ia[i] + (ia[3 * i + 1] - ia[3 * i]) / 3;
+        ng.nnz = ng.ia[nb];
+        ng.ja.resize(ng.nnz);
+        for (int64_t i = 0; i < nb; ++i)
+            for (int32_t p = ia[3 * i], q = ng.ia[i]; p < ia[3 * i + 1]; p += 3, ++q) ng.ja[q] = ja[p] / 3;
+        TRY_CALL(buildBlockSellDevice(g, sellPermutation(ng, 1024)));
+        return FEEL_HIP_OK;
+    }
     GraphCSR gc;   // borrows the host pattern for the row ordering
     gc.nrows = nrows; gc.nnz = g->nnz; gc.ia.swap(g->hia); gc.ja.swap(g->hja);
     std::vector<int32_t> perm = sellPermutation(gc, 1024);

@@ -876,6 +876,72 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
     }
 }
 
+// y = A x on the 3 x 3 block image of a vector space: one wavefront per slice
+// of 64 node rows, a lane computes its node's three rows; per block slot one
+// column index, nine values (planes of 64) and three consecutive x entries.
+// Two block slots per step keep independent partial sums.  With part, also the
+// block partial of x.y (p.Ap).
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
+                                                     const int32_t* __restrict__ bcol, const T* __restrict__ val,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     double* __restrict__ part, const CgState* __restrict__ st) {
+    if (st && blockDone(st)) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * BS) >> 6;
+    const int64_t nb = nrows / 3;
+    double dot = 0.0;
+    for (int64_t s = wave; s < nslices; s += nwaves) {
+        const uint32_t base = slicePtr[s];
+        const int width = (int)((slicePtr[s + 1] - base) >> 6);
+        const int32_t* c = bcol + base + lane;
+        const T* v = val + 9 * (size_t)base + lane;
+        T acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        int k = 0;
+        for (; k + 2 <= width; k += 2) {
+            int32_t j[2];
+            T vv[2][9];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                j[u] = c[(size_t)(k + u) * 64];
+#pragma unroll
+                for (int q = 0; q < 9; ++q)
+                    vv[u][q] = sizeof(T) == 4 ? __builtin_nontemporal_load(v + ((size_t)(k + u) * 9 + q) * 64)
+                                              : v[((size_t)(k + u) * 9 + q) * 64];
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const T x0 = x[3 * (int64_t)j[u]], x1 = x[3 * (int64_t)j[u] + 1], x2 = x[3 * (int64_t)j[u] + 2];
+                acc[u][0] += vv[u][0] * x0 + vv[u][1] * x1 + vv[u][2] * x2;
+                acc[u][1] += vv[u][3] * x0 + vv[u][4] * x1 + vv[u][5] * x2;
+                acc[u][2] += vv[u][6] * x0 + vv[u][7] * x1 + vv[u][8] * x2;
+            }
+        }
+        for (; k < width; ++k) {
+            const int32_t jj = c[(size_t)k * 64];
+            const T x0 = x[3 * (int64_t)jj], x1 = x[3 * (int64_t)jj + 1], x2 = x[3 * (int64_t)jj + 2];
+            const T* w = v + (size_t)k * 9 * 64;
+            acc[0][0] += w[0] * x0 + w[64] * x1 + w[128] * x2;
+            acc[0][1] += w[192] * x0 + w[256] * x1 + w[320] * x2;
+            acc[0][2] += w[384] * x0 + w[448] * x1 + w[512] * x2;
+        }
+        const int64_t r = s * 64 + lane;
+        if (r < nb) {
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const T a = acc[0][cc] + acc[1][cc];
+                y[3 * r + cc] = a;
+                if (part) dot += (double)x[3 * r + cc] * (double)a;
+            }
+        }
+    }
+    if (part) {
+        const double t = blockReduceSumN<BS>(dot);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
 // Reliable update, device half: r32 = r64 / sigma, z32 = D^{-1} r32, xs = 0,
 // partials of r.z and z.z for the next direction kernel.
 static __global__ __launch_bounds__(kBlock) void kCgStepReset(int64_t n, const double* __restrict__ r64, double invSigma,

@@ -242,3 +242,130 @@ done:
 }
 
 }  // namespace feelpp_amd
+
+// ------------------------------------------------------------------------
+// 3 x 3 block image for vector spaces (ncomp = 3, dofs interleaved node * 3 +
+// c, COUPLED pattern: the rows of a node share their columns, which come in
+// triples).  One lane per node row, 64 node rows per slice; block slot
+// (s, k, lane) holds the permuted node column of the lane's k-th block, and its
+// nine values sit in planes, value (c, d) of block slot k at
+// 9 base + (9 k + 3 c + d) 64 + lane.  One column index serves nine values and
+// the x gathers read three consecutive entries.
+
+namespace {
+
+__global__ __launch_bounds__(kSlBlock) void kBlockSliceSlots(int64_t nb, int64_t nslices, const int32_t* __restrict__ ia,
+                                                            const int32_t* __restrict__ nperm,
+                                                            uint64_t* __restrict__ slots) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
+        const int64_t r = s * kC + lane;
+        int len = 0;
+        if (r < nb) len = (ia[3 * (int64_t)nperm[r] + 1] - ia[3 * (int64_t)nperm[r]]) / 3;
+        for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+        if (lane == 0) slots[s] = (uint64_t)len * kC;
+    }
+}
+
+__global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nb, int64_t nrowsPadded, const int32_t* __restrict__ ia,
+                                                          const int32_t* __restrict__ ja,
+                                                          const int32_t* __restrict__ nperm,
+                                                          const int32_t* __restrict__ niperm,
+                                                          const uint32_t* __restrict__ slicePtr,
+                                                          int32_t* __restrict__ bcol, int32_t* __restrict__ slot2csr,
+                                                          int32_t* __restrict__ diagSlot) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrowsPadded;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = r / kC;
+        const int lane = (int)(r - s * kC);
+        const uint32_t b = slicePtr[s];
+        const int width = (int)((slicePtr[s + 1] - b) / kC);
+        const int32_t self = (int32_t)(r < nb ? r : nb - 1);
+        const int32_t old = r < nb ? nperm[r] : 0;
+        const int len = r < nb ? (ia[3 * (int64_t)old + 1] - ia[3 * (int64_t)old]) / 3 : 0;
+        for (int k = 0; k < width; ++k) {
+            const int64_t bs = b + (int64_t)k * kC + lane;
+            const size_t v0 = 9 * (size_t)b + (size_t)k * 9 * kC + lane;
+            if (k < len) {
+                const int32_t bj = ja[ia[3 * (int64_t)old] + 3 * k] / 3;
+                bcol[bs] = niperm[bj];
+                for (int c = 0; c < 3; ++c)
+                    for (int d = 0; d < 3; ++d) {
+                        const int32_t p = ia[3 * (int64_t)old + c] + 3 * k + d;
+                        const size_t vs = v0 + (size_t)(3 * c + d) * kC;
+                        slot2csr[vs] = p;
+                        if (bj == old && c == d) diagSlot[3 * r + c] = (int32_t)vs;
+                    }
+            } else {
+                bcol[bs] = self;
+                for (int q = 0; q < 9; ++q) slot2csr[v0 + (size_t)q * kC] = -1;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSlBlock) void kExpandPerm(int64_t nb, const int32_t* __restrict__ nperm,
+                                                       int32_t* __restrict__ perm) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nb; r += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) perm[3 * r + c] = 3 * nperm[r] + c;
+}
+
+}  // namespace
+
+namespace feelpp_amd {
+
+// Fills the block image (g->block, perm, slicePtr in block slots, scol = block
+// columns, slot2csr and sdiag over value slots, nslots = value slots) from the
+// device ia/ja and the host node order nperm (new -> old node).
+int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
+    feel_hip_ctx* c = g->space->ctx;
+    hipStream_t st = c->stream;
+    const int64_t nb = g->n / 3;
+    const int64_t nslices = (nb + kC - 1) / kC;
+    int rc = FEEL_HIP_OK;
+    int32_t *dnperm = nullptr, *niperm = nullptr;
+    uint64_t *cnt = nullptr, *off = nullptr;
+    void* tmp = nullptr;
+    size_t need = 0;
+    uint64_t total = 0;
+    g->block = true;
+    g->nslices = nslices;
+    rc = c->alloc((void**)&g->perm, sizeof(int32_t) * g->n);
+    if (rc != FEEL_HIP_OK) return rc;
+    SL_CHECK(hipMalloc(&dnperm, sizeof(int32_t) * nb));
+    SL_CHECK(hipMemcpyAsync(dnperm, nperm.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
+    SL_CHECK(hipMalloc(&niperm, sizeof(int32_t) * nb));
+    hipLaunchKernelGGL(kInversePerm, dim3(slGrid(nb)), dim3(kSlBlock), 0, st, nb, dnperm, niperm);
+    hipLaunchKernelGGL(kExpandPerm, dim3(slGrid(nb)), dim3(kSlBlock), 0, st, nb, dnperm, g->perm);
+    SL_CHECK(hipMalloc(&cnt, sizeof(uint64_t) * (nslices + 1)));
+    SL_CHECK(hipMalloc(&off, sizeof(uint64_t) * (nslices + 1)));
+    SL_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint64_t) * (nslices + 1), st));
+    hipLaunchKernelGGL(kBlockSliceSlots, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nb, nslices, g->ia, dnperm,
+                       cnt);
+    SL_CHECK(hipGetLastError());
+    SL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, (int)(nslices + 1), st));
+    SL_CHECK(hipMalloc(&tmp, need));
+    SL_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, need, cnt, off, (int)(nslices + 1), st));
+    SL_CHECK(hipMemcpyAsync(&total, off + nslices, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    SL_CHECK(hipStreamSynchronize(st));
+    if (9 * total >= (1ULL << 31)) { rc = setError(FEEL_HIP_EUNSUPPORTED, "block SELL image exceeds 2^31 values"); goto done; }
+    g->nbslots = (int64_t)total;
+    g->nslots = 9 * (int64_t)total;
+    rc = c->alloc((void**)&g->slicePtr, sizeof(uint32_t) * (nslices + 1));
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->scol, sizeof(int32_t) * g->nbslots);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->slot2csr, sizeof(int32_t) * g->nslots);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->sdiag, sizeof(int32_t) * g->n);
+    if (rc != FEEL_HIP_OK) goto done;
+    hipLaunchKernelGGL(kSlicePtr, dim3(slGrid(nslices + 1)), dim3(kSlBlock), 0, st, nslices + 1, off, g->slicePtr);
+    hipLaunchKernelGGL(kBlockSellFill, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nb, nslices * kC, g->ia,
+                       g->ja, dnperm, niperm, g->slicePtr, g->scol, g->slot2csr, g->sdiag);
+    SL_CHECK(hipGetLastError());
+    SL_CHECK(hipStreamSynchronize(st));
+done:
+    for (void* p : {(void*)dnperm, (void*)niperm, (void*)cnt, (void*)off, tmp})
+        if (p) (void)hipFree(p);
+    return rc;
+}
+
+}  // namespace feelpp_amd

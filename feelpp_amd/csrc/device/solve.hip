@@ -47,6 +47,18 @@ void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, int grid, const T* val
 template <typename T>
 void launchShape(feel_hip_ctx* c, const feel_hip_graph* g, const SpmvShape& sh, const T* val, const T* x, T* y,
                  double* part, const CgState* st) {
+    if (g->block) {
+        if (sh.block == 1024)
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 1024>), dim3(sh.grid), dim3(1024), 0, c->stream, g->n, g->nslices,
+                               g->slicePtr, g->scol, val, x, y, part, st);
+        else if (sh.block == 512)
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 512>), dim3(sh.grid), dim3(512), 0, c->stream, g->n, g->nslices,
+                               g->slicePtr, g->scol, val, x, y, part, st);
+        else
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 256>), dim3(sh.grid), dim3(256), 0, c->stream, g->n, g->nslices,
+                               g->slicePtr, g->scol, val, x, y, part, st);
+        return;
+    }
     if (g->compact) {
         if (sh.block == 1024) launchSpmv<T, 1024, true>(c, g, sh.grid, val, x, y, part, st);
         else if (sh.block == 512) launchSpmv<T, 512, true>(c, g, sh.grid, val, x, y, part, st);
