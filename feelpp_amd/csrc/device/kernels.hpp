@@ -878,14 +878,16 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
 
 // y = A x on the 3 x 3 block image of a vector space: one wavefront per slice
 // of 64 node rows, a lane computes its node's three rows; per block slot one
-// column index, nine values (planes of 64) and three consecutive x entries.
-// Two block slots per step keep independent partial sums.  With part, also the
-// block partial of x.y (p.Ap).
+// column index, nine values (planes of 64, read non-temporally) and three
+// consecutive x entries.  UB block slots per step (4 for FP32, 2 for FP64:
+// measured) keep independent partial sums.  With part, also the block partial
+// of x.y (p.Ap).
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
                                                      const int32_t* __restrict__ bcol, const T* __restrict__ val,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      double* __restrict__ part, const CgState* __restrict__ st) {
+    constexpr int UB = sizeof(T) == 4 ? 4 : 2;
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
@@ -897,21 +899,21 @@ __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nsli
         const int width = (int)((slicePtr[s + 1] - base) >> 6);
         const int32_t* c = bcol + base + lane;
         const T* v = val + 9 * (size_t)base + lane;
-        T acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
-        int k = 0;
-        for (; k + 2 <= width; k += 2) {
-            int32_t j[2];
-            T vv[2][9];
+        T acc[UB][3];
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < UB; ++u) acc[u][0] = acc[u][1] = acc[u][2] = 0;
+        int k = 0;
+        for (; k + UB <= width; k += UB) {
+            int32_t j[UB];
+            T vv[UB][9];
+#pragma unroll
+            for (int u = 0; u < UB; ++u) {
                 j[u] = c[(size_t)(k + u) * 64];
 #pragma unroll
-                for (int q = 0; q < 9; ++q)
-                    vv[u][q] = sizeof(T) == 4 ? __builtin_nontemporal_load(v + ((size_t)(k + u) * 9 + q) * 64)
-                                              : v[((size_t)(k + u) * 9 + q) * 64];
+                for (int q = 0; q < 9; ++q) vv[u][q] = __builtin_nontemporal_load(v + ((size_t)(k + u) * 9 + q) * 64);
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
+            for (int u = 0; u < UB; ++u) {
                 const T x0 = x[3 * (int64_t)j[u]], x1 = x[3 * (int64_t)j[u] + 1], x2 = x[3 * (int64_t)j[u] + 2];
                 acc[u][0] += vv[u][0] * x0 + vv[u][1] * x1 + vv[u][2] * x2;
                 acc[u][1] += vv[u][3] * x0 + vv[u][4] * x1 + vv[u][5] * x2;
@@ -930,7 +932,9 @@ __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nsli
         if (r < nb) {
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                const T a = acc[0][cc] + acc[1][cc];
+                T a = 0;
+#pragma unroll
+                for (int u = 0; u < UB; ++u) a += acc[u][cc];
                 y[3 * r + cc] = a;
                 if (part) dot += (double)x[3 * r + cc] * (double)a;
             }
