@@ -123,19 +123,15 @@ __global__ __launch_bounds__(kBlock) void kBAxpy(const BItem* __restrict__ items
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) I.x64[r] += I.xscale * (double)tp<T>(I.xT)[r];
 }
 
-// w64 = A64 x64 on the item's slices (one wavefront per slice)
+// w64 = A64 x64 on the item's slices (one wavefront per slice, the SELL row
+// kernel of the single solver)
 __global__ __launch_bounds__(kBlock) void kBSpmv64(const BItem* __restrict__ items, const BInst* __restrict__ insts) {
     const BItem it = items[blockIdx.x];
     const BInst& I = insts[it.inst];
     const int lane = threadIdx.x & 63;
     for (int s = it.s0 + (int)(threadIdx.x >> 6); s < it.s1; s += kBlock / 64) {
-        const uint32_t base = I.slicePtr[s];
-        const int width = (int)((I.slicePtr[s + 1] - base) >> 6);
-        double acc = 0.0;
-        for (int k = 0; k < width; ++k) {
-            const size_t slot = sellSlot(base, k, lane);
-            acc += I.val64[slot] * I.x64[I.col[slot]];
-        }
+        const double acc = I.compact ? sellRow<double, 2, true>(I.cols, s, lane, I.val64, I.x64)
+                                     : sellRow<double, 2, false>(I.cols, s, lane, I.val64, I.x64);
         const int64_t r = (int64_t)s * 64 + lane;
         if (r < I.n) I.w64[r] = acc;
     }
