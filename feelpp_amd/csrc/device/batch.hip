@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void kBCombine(const BItem* __restrict__ it
         else if (m == 2)
             v = 1.0;
         I.val64[s] = v;
-        tp<T>(I.valT)[s] = (T)v;
+        if (I.valT) tp<T>(I.valT)[s] = (T)v;   // the grouped engine reads the terms instead
     }
 }
 
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void kBDiag(const BItem* __restrict__ items
         const double d = I.val64[I.sdiag[r]];
         const double inv = d != 0.0 ? 1.0 / d : 1.0;
         I.dinv64[r] = inv;
-        const double dT = (double)tp<T>(I.valT)[I.sdiag[r]];
+        const double dT = (double)(T)d;   // the diagonal of the T copy of A(mu)
         tp<T>(I.dinvT)[r] = (T)(dT != 0.0 ? 1.0 / dT : 1.0);
     }
 }
@@ -772,8 +772,13 @@ size_t partCount(const BatchFamily& f) {
     return 3 * (size_t)f.nitems + (size_t)((f.nslices + kGroupSlices - 1) / kGroupSlices);
 }
 
+// the grouped FP32 engine reads the family terms and keeps p in the family
+// group: its instances hold no T copy of A(mu) and no p of their own
+bool groupedEngine(const feel_hip_batch* B) { return B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv; }
+
 size_t instanceBytes(const feel_hip_batch* B, const BatchFamily& f) {
     const size_t t = tsize(B);
+    if (groupedEngine(B)) return f.nslots * 8 + f.n * (8 * 5 + t * 6) + sizeof(double) * partCount(f);
     return f.nslots * (8 + t) + f.n * (8 * 5 + t * 7) + sizeof(double) * partCount(f);
 }
 
@@ -788,10 +793,13 @@ int allocInstance(feel_hip_batch* B, int fam, int* out) {
     const size_t t = tsize(B);
     InstBuffers ib;
     TRY_CALL(alloc(c, &ib.val64, f.nslots));
-    TRY_CALL(c->alloc(&ib.valT, t * f.nslots));
+    if (!groupedEngine(B)) {
+        TRY_CALL(c->alloc(&ib.valT, t * f.nslots));
+        TRY_CALL(c->alloc(&ib.pT, t * f.n));
+    }
     for (double** p : {&ib.dinv64, &ib.x64, &ib.r64, &ib.w64, &ib.out}) TRY_CALL(alloc(c, p, f.n));
     TRY_CALL(alloc(c, &ib.part, partCount(f)));
-    for (void** p : {&ib.dinvT, &ib.xT, &ib.rT, &ib.zT, &ib.pT, &ib.wT, &ib.bT}) TRY_CALL(c->alloc(p, t * f.n));
+    for (void** p : {&ib.dinvT, &ib.xT, &ib.rT, &ib.zT, &ib.wT, &ib.bT}) TRY_CALL(c->alloc(p, t * f.n));
     B->bufs.push_back(ib);
     B->used += (int64_t)instanceBytes(B, f);
     *out = (int)B->bufs.size() - 1;
