@@ -54,6 +54,24 @@ __device__ inline double reducePartials(const double* __restrict__ part, int cou
     return blockReduceSum(v);
 }
 
+// 16-byte vector of T (four floats or two doubles) for the streaming CG
+// vector kernels: one dwordx4 load or store per lane instead of four or two.
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename... P>
+__device__ inline bool aligned16(const P*... ptr) {
+    return ((reinterpret_cast<uintptr_t>(ptr) | ...) & 15) == 0;
+}
+
 // ---------------------------------------------------------------- geometry
 // Affine map of a P1 simplex on the [-1,1] reference element
 // (feelpoly/geomap.hpp:2258-2355): K[:,a] = (x_{a+1} - x_0)/2, J = |det K|
@@ -569,8 +587,28 @@ __global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const 
     // mixed-precision solve restarted it)
     const bool first = it == st->fresh;
     const double bb = first ? 0.0 : beta / betaSlots[(it - 1) & 1];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        p[i] = first ? z[i] : (T)((double)z[i] + bb * (double)p[i]);
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t tail = 0;
+    if (aligned16(z, p)) {
+        typedef typename Vec16<T>::type VT;
+        constexpr int V = 16 / sizeof(T);
+        const int64_t nv = n / V;
+        const VT* zv = reinterpret_cast<const VT*>(z);
+        VT* pv = reinterpret_cast<VT*>(p);
+        for (int64_t i = gid; i < nv; i += stride) {
+            const VT zi = zv[i];
+            if (first) {
+                pv[i] = zi;
+            } else {
+                VT pi = pv[i];
+#pragma unroll
+                for (int l = 0; l < V; ++l) pi[l] = (T)((double)zi[l] + bb * (double)pi[l]);
+                pv[i] = pi;
+            }
+        }
+        tail = nv * V;
+    }
+    for (int64_t i = tail + gid; i < n; i += stride) p[i] = first ? z[i] : (T)((double)z[i] + bb * (double)p[i]);
 }
 
 // x += a p, r -= a w, z = D^{-1} r; partials of z.r and z.z.  dpiPart holds
@@ -594,15 +632,39 @@ __global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __rest
     if (indefinite) return;
     const double a = betaSlots[it & 1] / dpi;
     double zr = 0.0, zz = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        x[i] = (T)((double)x[i] + a * (double)p[i]);
-        const T ri = (T)((double)r[i] - a * (double)w[i]);
-        r[i] = ri;
-        const T zi = dinv[i] * ri;
-        z[i] = zi;
+    // one entry: x += a p, r -= a w, z = D^{-1} r, accumulate z.r and z.z
+    auto step = [&](T& xi, T& ri, T& zi, T pi, T wi, T di) {
+        xi = (T)((double)xi + a * (double)pi);
+        ri = (T)((double)ri - a * (double)wi);
+        zi = di * ri;
         zr += (double)zi * (double)ri;
         zz += (double)zi * (double)zi;
+    };
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t tail = 0;
+    if (aligned16(x, r, z, p, w, dinv)) {
+        typedef typename Vec16<T>::type VT;
+        constexpr int V = 16 / sizeof(T);
+        const int64_t nv = n / V;
+        for (int64_t i = gid; i < nv; i += stride) {
+            VT xv = reinterpret_cast<const VT*>(x)[i], rv = reinterpret_cast<const VT*>(r)[i], zv;
+            const VT pv = reinterpret_cast<const VT*>(p)[i], wv = reinterpret_cast<const VT*>(w)[i],
+                     dv = reinterpret_cast<const VT*>(dinv)[i];
+#pragma unroll
+            for (int l = 0; l < V; ++l) {
+                T xi = xv[l], ri = rv[l], zi;
+                step(xi, ri, zi, pv[l], wv[l], dv[l]);
+                xv[l] = xi;
+                rv[l] = ri;
+                zv[l] = zi;
+            }
+            reinterpret_cast<VT*>(x)[i] = xv;
+            reinterpret_cast<VT*>(r)[i] = rv;
+            reinterpret_cast<VT*>(z)[i] = zv;
+        }
+        tail = nv * V;
     }
+    for (int64_t i = tail + gid; i < n; i += stride) step(x[i], r[i], z[i], p[i], w[i], dinv[i]);
     zr = blockReduceSum(zr);
     zz = blockReduceSum(zz);
     if (threadIdx.x == 0) { zrPart[blockIdx.x] = zr; zzPart[blockIdx.x] = zz; }
