@@ -284,6 +284,19 @@ def bench_beam(fa, args):
             assemble()
         ctx.synchronize()
         t_asm = (time.perf_counter() - t1) / reps
+        # the bilinear assembly alone on each element-matrix path (the default
+        # above takes the faster of the two, timed on its first assembly)
+        bilinear = {}
+        for name, m in (("fused", fa.MATRICES_FUSED), ("mfma", fa.MATRICES_MFMA)):
+            for rep in range(reps + 1):
+                if rep == 1:
+                    ctx.synchronize()
+                    tb = time.perf_counter()
+                A.assemble(fa.ELASTICITY, qo, young=young, poisson=poisson, precision=fa.FP64_FP32, matrices=m)
+            ctx.synchronize()
+            bilinear[name] = round((time.perf_counter() - tb) / reps * 1e3, 3)
+        log(f"beam P{k}: bilinear assembly ms by element-matrix path {bilinear}")
+        assemble()
         x.zero()
         st = A.solve(b, x, rtol=1e-13, maxit=200000)
         xref = x.get()
@@ -302,7 +315,7 @@ def bench_beam(fa, args):
                            "rel_err_vs_fp64": err}
             log(f"beam P{k} {form}: {forms[form]}")
         res[f"P{k}"] = {"dofs": ndof, "nnz": len(ja), "elements": ne,
-                        "elements_per_s_assembly": round(ne / t_asm, 1), "forms": forms}
+                        "elements_per_s_assembly": round(ne / t_asm, 1), "bilinear_ms": bilinear, "forms": forms}
     best = {kk: max(v["forms"].values(), key=lambda f: f["solves_per_s"] if f["rel_err_vs_fp64"] <= 1e-8 else 0)
             for kk, v in res.items()}
     line = {
@@ -346,6 +359,42 @@ def bench_kernels(fa, args):
     ctx.synchronize()
     print(json.dumps({"kernels": "copy 1 GiB x5; C2 assembly x3; FP64 and FP32 solves of 20 iterations",
                       "nnz": prob.nnz, "dofs": prob.ndof, "elements": prob.ne}), flush=True)
+
+
+def bench_elastic(fa, args):
+    """Short fixed run of the elasticity element-matrix kernels for
+    hardware-counter passes: a plain 1 GiB device copy (counter calibration),
+    then on the P4 beam (40x10x10 Kuhn cells, 24 000 tets, 811 923 dofs) three
+    FP64+FP32 bilinear assemblies on each element-matrix path: contracted inside
+    the scatter, and materialised on the FP64 matrix cores then scattered."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    ctx = fa.Context(0)
+    k = args.order if args.order in (3, 4) else 4
+    coords, elems = fa.mesh_kuhn_box(40, 10, 10, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=args.seed)
+    ne = len(elems)
+    space = fa.Space.from_mesh(ctx, coords, elems, k, 3)
+    A = fa.Matrix(fa.Graph.from_space(space), fa.FP64_FP32)
+    young = ctx.vector(ne, fa.uniform(ne, 1.0, 100.0, args.seed + 1, log=True))
+    poisson = ctx.vector(ne, fa.uniform(ne, 0.2, 0.35, args.seed + 2))
+    out = {}
+    for name, m in (("fused", fa.MATRICES_FUSED), ("mfma", fa.MATRICES_MFMA)):
+        for rep in range(4):
+            if rep == 1:
+                ctx.synchronize()
+                t = time.perf_counter()
+            A.assemble(fa.ELASTICITY, 2 * k, young=young, poisson=poisson, precision=fa.FP64_FP32, matrices=m)
+        ctx.synchronize()
+        out[name] = round((time.perf_counter() - t) / 3 * 1e3, 3)
+    print(json.dumps({"kernels": f"copy 1 GiB x5; P{k} beam elasticity assembly x4 per element-matrix path",
+                      "elements": ne, "dofs": space.ndof, "bilinear_ms": out}), flush=True)
 
 
 def bench_serving(fa, args):
@@ -414,7 +463,7 @@ def main():
                     help="residual reduction between FP64 reliable updates of the FP32 solves (default 1e-2)")
     ap.add_argument("--forms", default="fp64,mixed")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam", "kernels"])
+    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam", "kernels", "elastic"])
     ap.add_argument("--orders", default="1,2,3,4")
     ap.add_argument("--sizes", default="8,12,16,24,32")
     ap.add_argument("--problems", type=int, default=512)
@@ -437,6 +486,8 @@ def main():
         return bench_batched(fa, args)
     if args.config == "kernels":
         return bench_kernels(fa, args)
+    if args.config == "elastic":
+        return bench_elastic(fa, args)
     if args.config == "beam":
         if args.orders == "1,2,3,4":
             args.orders = "3,4"

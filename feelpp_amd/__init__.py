@@ -22,6 +22,7 @@ _vp = C.c_void_p
 
 FP64, FP32, FP64_FP32 = 1, 2, 3
 MASS, DIFFUSION, ELASTICITY = 1, 2, 4
+MATRICES_AUTO, MATRICES_FUSED, MATRICES_MFMA = 0, 1, 2
 SRC_CONST, SRC_SINSIN, SRC_VECTOR = 0, 1, 2
 ELIMINATION, ELIMINATION_SYMMETRIC = 0, 1
 
@@ -33,7 +34,7 @@ class FeelHipError(RuntimeError):
 class BilinearDesc(C.Structure):
     _fields_ = [("kind", C.c_int), ("quad_order", C.c_int), ("precision", C.c_int),
                 ("diff_scale", C.c_double), ("diff_coef", _vp), ("mass_scale", C.c_double),
-                ("mass_coef", _vp), ("young", _vp), ("poisson", _vp)]
+                ("mass_coef", _vp), ("young", _vp), ("poisson", _vp), ("matrices", C.c_int)]
 
 
 class LinearDesc(C.Structure):
@@ -507,9 +508,10 @@ class Matrix(_Handle):
         _check(lib().feel_hip_mat_mult(self.h, x.h, y.h))
 
     def assemble(self, kind, quad_order, diff=1.0, kdiff=None, mass=1.0, kmass=None, young=None, poisson=None,
-                 precision=0, add=False):
+                 precision=0, add=False, matrices=MATRICES_AUTO):
         d = BilinearDesc(kind, quad_order, precision, diff, kdiff.h if kdiff else None, mass,
-                         kmass.h if kmass else None, young.h if young else None, poisson.h if poisson else None)
+                         kmass.h if kmass else None, young.h if young else None, poisson.h if poisson else None,
+                         matrices)
         _check(lib().feel_hip_assemble_bilinear(self.h, C.byref(d), int(add)))
 
     def zero_rows(self, rows, values=None, rhs=None, mode=ELIMINATION_SYMMETRIC, diag=1.0):

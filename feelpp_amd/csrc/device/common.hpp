@@ -140,6 +140,9 @@ struct feel_hip_graph {
     // SpMV launch shape picked by the autotuner, per precision (0 FP64, 1 FP32):
     // index into the candidate list, -1 until the first solve measures them
     int spmvShape[2] = {-1, -1};
+    // elasticity element-matrix path picked by timing both (FEEL_HIP_MATRICES_*),
+    // 0 until the first assembly with add = 0 measures them
+    int elemPath = 0;
     int64_t ncontrib = 0;
     std::vector<int32_t> hia, hja;
     // SELL-C-sigma image used by the solver (permuted numbering)

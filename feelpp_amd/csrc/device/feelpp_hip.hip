@@ -915,13 +915,88 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
         FEEL_HIP_CHECK(hipGetLastError());
         const int grid = segGrid(g);
         TRY_CALL(clearEmptySlots(c, g, v64, v32, add));
+        if (s->order < 1 || s->order > 4) return setError(FEEL_HIP_EUNSUPPORTED, "no elasticity kernel for this order");
+        int path = d->matrices;
+        if (path < FEEL_HIP_MATRICES_AUTO || path > FEEL_HIP_MATRICES_MFMA)
+            return setError(FEEL_HIP_EINVAL, "unknown element-matrix path");
+        if (path == FEEL_HIP_MATRICES_MFMA && s->order < 3)
+            return setError(FEEL_HIP_EUNSUPPORTED, "matrix-core element matrices are built for P3 and P4");
+        if (path == FEEL_HIP_MATRICES_AUTO) path = s->order < 3 ? FEEL_HIP_MATRICES_FUSED : g->elemPath;
+        auto fused = [&]() -> int {
+            switch (s->order) {
+                case 1: hipLaunchKernelGGL(kGatherElastic<4>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
+                case 2: hipLaunchKernelGGL(kGatherElastic<10>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
+                case 3: hipLaunchKernelGGL(kGatherElastic<20>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
+                default:
+                    hipLaunchKernelGGL((kGatherElastic<35, false>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS);
+            }
+            FEEL_HIP_CHECK(hipGetLastError());
+            return FEEL_HIP_OK;
+        };
+        // element matrices on the matrix cores into A->elemmat, then the plain
+        // segmented sum; the workspace (nelements (3 nloc)^2 doubles) stays
+        // with the matrix for later assemblies
+        auto mfma = [&]() -> int {
+            const size_t bytes = sizeof(double) * (size_t)s->ne * (size_t)(s->nldof * s->nldof);
+            if (A->elemmatBytes < bytes) {
+                c->release(A->elemmat, A->elemmatBytes);
+                A->elemmat = nullptr;
+                A->elemmatBytes = 0;
+                TRY_CALL(c->alloc((void**)&A->elemmat, bytes));
+                A->elemmatBytes = bytes;
+            }
+            const int64_t rtiles = (9 * s->ne + 15) / 16;
+            const int mgrid = (int)std::min<int64_t>((rtiles + kBlock / 64 - 1) / (kBlock / 64), 16384);
+            if (s->order == 3)
+                hipLaunchKernelGGL(kElasticMatMfma<20>, dim3(mgrid), dim3(kBlock), 0, c->stream, s->ne, s->geo, rt,
+                                   A->elemmat);
+            else
+                hipLaunchKernelGGL(kElasticMatMfma<35>, dim3(mgrid), dim3(kBlock), 0, c->stream, s->ne, s->geo, rt,
+                                   A->elemmat);
+            hipLaunchKernelGGL(kGatherMaterialised, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks, g->taskSeg,
+                               g->taskK, g->centry, g->segSlot, A->elemmat, v64, v32, add);
+            FEEL_HIP_CHECK(hipGetLastError());
+            return FEEL_HIP_OK;
+        };
+        if (path == 0 && add) path = FEEL_HIP_MATRICES_FUSED;   // timing both would accumulate twice
         c->tic("assemble_gather");
-        switch (s->order) {
-            case 1: hipLaunchKernelGGL(kGatherElastic<4>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
-            case 2: hipLaunchKernelGGL(kGatherElastic<10>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
-            case 3: hipLaunchKernelGGL(kGatherElastic<20>, dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
-            case 4: hipLaunchKernelGGL((kGatherElastic<35, false>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS); break;
-            default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no elasticity kernel for this order");
+        if (path == 0) {
+            // first assembly of a P3/P4 graph: time one run of each after a
+            // warm-up run, keep the faster; the last run leaves valid values
+            hipEvent_t e0, e1;
+            FEEL_HIP_CHECK(hipEventCreate(&e0));
+            FEEL_HIP_CHECK(hipEventCreate(&e1));
+            float ms[2] = {0.0f, 0.0f};
+            bool haveMfma = true;
+            for (int k = 0; k < 2; ++k) {
+                for (int rep = 0; rep < 2; ++rep) {
+                    if (rep == 1) (void)hipEventRecord(e0, c->stream);
+                    if (k == 0) {
+                        TRY_CALL(fused());
+                    } else if (mfma() != FEEL_HIP_OK) {
+                        (void)hipGetLastError();
+                        haveMfma = false;   // workspace does not fit: the fused path stays
+                        break;
+                    }
+                    if (rep == 1) (void)hipEventRecord(e1, c->stream);
+                }
+                if (k == 1 && !haveMfma) break;
+                FEEL_HIP_CHECK(hipEventSynchronize(e1));
+                FEEL_HIP_CHECK(hipEventElapsedTime(&ms[k], e0, e1));
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            g->elemPath = haveMfma && ms[1] < ms[0] ? FEEL_HIP_MATRICES_MFMA : FEEL_HIP_MATRICES_FUSED;
+            if (!haveMfma) TRY_CALL(fused());
+            if (g->elemPath == FEEL_HIP_MATRICES_FUSED) {
+                c->release(A->elemmat, A->elemmatBytes);
+                A->elemmat = nullptr;
+                A->elemmatBytes = 0;
+            }
+        } else if (path == FEEL_HIP_MATRICES_MFMA) {
+            TRY_CALL(mfma());
+        } else {
+            TRY_CALL(fused());
         }
         c->toc();
         FEEL_HIP_CHECK(hipGetLastError());

@@ -62,7 +62,16 @@ typedef struct {
     const feel_hip_vec* mass_coef;  /* length nelements or null */
     const feel_hip_vec* young;      /* elasticity: E per element */
     const feel_hip_vec* poisson;    /* elasticity: nu per element */
+    int matrices;                   /* elasticity element matrices, FEEL_HIP_MATRICES_* (0 = autotuned) */
 } feel_hip_bilinear_desc;
+
+/* How elasticity element matrices reach the CSR values.  FUSED: each entry is
+ * contracted inside the scatter from the element factors and reference tensors.
+ * MFMA (P3 and P4): all element matrices are first materialised with FP64
+ * matrix-core GEMMs into a workspace of nelements * (3 nloc)^2 doubles held by
+ * the matrix, then summed by the same scatter.  AUTO: P1/P2 use FUSED; P3/P4
+ * time both on the graph's first assembly with add = 0 and keep the faster. */
+enum { FEEL_HIP_MATRICES_AUTO = 0, FEEL_HIP_MATRICES_FUSED = 1, FEEL_HIP_MATRICES_MFMA = 2 };
 
 /* Source of integrate(_range=elements(mesh), _expr=f*id(v)) (linearformcontext.hpp:294-305) */
 enum { FEEL_HIP_SRC_CONST = 0, FEEL_HIP_SRC_SINSIN = 1, FEEL_HIP_SRC_VECTOR = 2 };

@@ -148,8 +148,7 @@ def test_full_size_p2_properties(ctx):
     assert abs(c @ Aa - a @ y.get()) < 1e-10 * abs(c @ Aa) + 1e-12    # symmetric
 
 
-@pytest.mark.parametrize("order,qo", [(1, 2), (2, 4), (3, 6), (4, 8)])
-def test_elasticity_fp64_vs_oracle(ctx, orc, order, qo):
+def beam(ctx, orc, order, qo):
     # 1 x 0.25 x 0.25 beam, E log-uniform in [1,100], nu uniform in [0.2,0.35]
     coords, elems = orc.mesh_kuhn(4, 2, 2, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=13)
     e2d, ndof = orc.doftable(3, order, 3, elems)
@@ -159,10 +158,48 @@ def test_elasticity_fp64_vs_oracle(ctx, orc, order, qo):
     ref = orc.assemble_elasticity(order, qo, coords, elems, e2d, E, nu, ia, ja)
     sp = fa.Space(ctx, coords, elems, order, 3, e2d, ndof)
     g = fa.Graph(sp, ia, ja)
+    return g, ctx.vector(len(elems), E), ctx.vector(len(elems), nu), ref
+
+
+@pytest.mark.parametrize("order,qo,matrices", [(1, 2, fa.MATRICES_AUTO), (2, 4, fa.MATRICES_AUTO),
+                                               (3, 6, fa.MATRICES_AUTO), (4, 8, fa.MATRICES_AUTO),
+                                               (3, 6, fa.MATRICES_FUSED), (4, 8, fa.MATRICES_FUSED),
+                                               (3, 6, fa.MATRICES_MFMA), (4, 8, fa.MATRICES_MFMA)])
+def test_elasticity_fp64_vs_oracle(ctx, orc, order, qo, matrices):
+    # AUTO times both paths on the first P3/P4 assembly; FUSED contracts inside
+    # the scatter; MFMA materialises the element matrices on the matrix cores
+    g, E, nu, ref = beam(ctx, orc, order, qo)
     A = fa.Matrix(g, fa.FP64_FP32)
-    A.assemble(fa.ELASTICITY, qo, young=ctx.vector(len(elems), E), poisson=ctx.vector(len(elems), nu))
+    A.assemble(fa.ELASTICITY, qo, young=E, poisson=nu, matrices=matrices)
     assert rel_fro(A.values(), ref) < TOL64
     assert rel_fro(A.values(fa.FP32).astype(np.float64), ref) < 1e-6
+    A.assemble(fa.ELASTICITY, qo, young=E, poisson=nu, matrices=matrices, add=True)
+    assert rel_fro(A.values(), 2.0 * ref) < TOL64
+
+
+def test_elasticity_paths_agree_and_reject_misuse(ctx, orc):
+    g, E, nu, ref = beam(ctx, orc, 4, 8)
+    out = []
+    for m in (fa.MATRICES_FUSED, fa.MATRICES_MFMA):
+        A = fa.Matrix(g, fa.FP64)
+        A.assemble(fa.ELASTICITY, 8, young=E, poisson=nu, matrices=m)
+        out.append(A.values())
+        A.assemble(fa.ELASTICITY, 8, young=E, poisson=nu, matrices=m)   # workspace reused
+        assert np.array_equal(A.values(), out[-1])
+    # same entries, different summation order inside an entry: round-off apart
+    assert rel_fro(out[1], out[0]) < 1e-14
+    # the first AUTO assembly times both (the matrix-core run last), later ones
+    # take the faster: every result is one of the two paths' bits
+    A = fa.Matrix(g, fa.FP64)
+    for _ in range(2):
+        A.assemble(fa.ELASTICITY, 8, young=E, poisson=nu)
+        v = A.values()
+        assert np.array_equal(v, out[0]) or np.array_equal(v, out[1])
+    g2, E2, nu2, _ = beam(ctx, orc, 2, 4)
+    with pytest.raises(fa.FeelHipError):
+        fa.Matrix(g2).assemble(fa.ELASTICITY, 4, young=E2, poisson=nu2, matrices=fa.MATRICES_MFMA)
+    with pytest.raises(fa.FeelHipError):
+        fa.Matrix(g).assemble(fa.ELASTICITY, 8, young=E, poisson=nu, matrices=7)
 
 
 def test_unsupported_requests_fail_loudly(ctx, orc):
