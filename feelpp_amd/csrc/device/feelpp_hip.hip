@@ -934,10 +934,11 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
             return FEEL_HIP_OK;
         };
         // element matrices on the matrix cores into A->elemmat, then the plain
-        // segmented sum; the workspace (nelements (3 nloc)^2 doubles) stays
-        // with the matrix for later assemblies
+        // segmented sum; the workspace (nelements (3 nloc)^2 doubles, rounded
+        // up to whole 16 x 16 tiles) stays with the matrix for later assemblies
         auto mfma = [&]() -> int {
-            const size_t bytes = sizeof(double) * (size_t)s->ne * (size_t)(s->nldof * s->nldof);
+            const size_t bytes = sizeof(double) * (size_t)(s->order == 3 ? ElasticTiles<20>::doubles(s->ne)
+                                                                          : ElasticTiles<35>::doubles(s->ne));
             if (A->elemmatBytes < bytes) {
                 c->release(A->elemmat, A->elemmatBytes);
                 A->elemmat = nullptr;
@@ -953,8 +954,12 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
             else
                 hipLaunchKernelGGL(kElasticMatMfma<35>, dim3(mgrid), dim3(kBlock), 0, c->stream, s->ne, s->geo, rt,
                                    A->elemmat);
-            hipLaunchKernelGGL(kGatherMaterialised, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks, g->taskSeg,
-                               g->taskK, g->centry, g->segSlot, A->elemmat, v64, v32, add);
+            if (s->order == 3)
+                hipLaunchKernelGGL(kGatherMaterialised<20>, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks,
+                                   g->taskSeg, g->taskK, g->centry, g->segSlot, A->elemmat, v64, v32, add);
+            else
+                hipLaunchKernelGGL(kGatherMaterialised<35>, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks,
+                                   g->taskSeg, g->taskK, g->centry, g->segSlot, A->elemmat, v64, v32, add);
             FEEL_HIP_CHECK(hipGetLastError());
             return FEEL_HIP_OK;
         };

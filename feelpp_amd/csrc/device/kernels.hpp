@@ -342,20 +342,39 @@ __global__ __launch_bounds__(kBlock) void kGatherElastic(int64_t ntasks, const u
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
 // Elasticity element matrices materialised on the FP64 matrix cores.  For all
-// elements at once the contraction above is one GEMM: rows (e, c*3+d), 9 per
-// element, times the reference tensors, columns (i, j), over t = 0..8 (K = 9,
-// padded to 12 with zeros):
+// elements at once the contraction above is one GEMM: rows m = 9e + 3c + d,
+// 9 per element, times the reference tensors, columns i*n + j, over t = 0..8
+// (K = 9, padded to 12 with zeros):
 //   Ke[(c,i),(d,j)] = sum_t F[e][(c,d)][t] R[(i,j)][t].
 // A wave owns a 16-row tile and sweeps every 16-column tile with three
 // v_mfma_f64_16x16x4f64.  Operands: lane l holds A[l%16][l/16] and
 // B[l/16][l%16]; the 16x16 result holds row l/16 + 4r, column l%16 in
-// register r.  Results go to Em in the gather map's entry order
-// e*(3n)^2 + (c*n+i)*3n + d*n + j, so entry c of the map reads Em[c].
+// register r.  Em is tile-major (tile (rt, ct) at (rt*NCT + ct)*256, row-major
+// inside), so each result register is stored as 512 contiguous bytes;
+// emTileIndex maps a gather-map entry id to its place.
+template <int N>
+struct ElasticTiles {
+    static constexpr int NN = N * N, N3 = 3 * N, NCT = (NN + 15) / 16;
+    static constexpr int64_t NN3 = (int64_t)N3 * N3;
+    static int64_t doubles(int64_t ne) { return (ne * 9 + 15) / 16 * NCT * 256; }
+};
+
+template <int N>
+__device__ inline int64_t emTileIndex(uint32_t c) {
+    using T = ElasticTiles<N>;
+    const uint32_t el = c / (uint32_t)T::NN3;
+    const uint32_t loc = c - el * (uint32_t)T::NN3;
+    const uint32_t rl = loc / T::N3, cl = loc - rl * T::N3;
+    const uint32_t cc = rl / N, i = rl - cc * N, dd = cl / N, j = cl - dd * N;
+    const int64_t m = (int64_t)el * 9 + cc * 3 + dd;
+    const uint32_t ij = i * N + j;
+    return ((m >> 4) * T::NCT + (ij >> 4)) * 256 + (m & 15) * 16 + (ij & 15);
+}
+
 template <int N>
 __global__ __launch_bounds__(kBlock) void kElasticMatMfma(int64_t ne, const double* __restrict__ F,
                                                           const double* __restrict__ R, double* __restrict__ Em) {
-    constexpr int NN = N * N, N3 = 3 * N, NCT = (NN + 15) / 16;
-    constexpr int64_t NN3 = (int64_t)N3 * N3;
+    using T = ElasticTiles<N>;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
     const int64_t rows = ne * 9, rtiles = (rows + 15) / 16;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -368,43 +387,34 @@ __global__ __launch_bounds__(kBlock) void kElasticMatMfma(int64_t ne, const doub
             const int k = 4 * s + lk;
             a[s] = (m < rows && k < 9) ? F[(m / 9) * kElasticF + (m % 9) * 10 + k] : 0.0;
         }
-        int64_t obase[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t mo = rt * 16 + lk + 4 * r;
-            const int64_t e = mo / 9;
-            const int cd = (int)(mo - e * 9), cc = cd / 3, dd = cd - cc * 3;
-            obase[r] = mo < rows ? e * NN3 + (int64_t)cc * N * N3 + dd * N : -1;
-        }
-        for (int ct = 0; ct < NCT; ++ct) {
+        double* tile = Em + rt * T::NCT * 256 + lk * 16 + li;
+        for (int ct = 0; ct < T::NCT; ++ct, tile += 256) {
             const int ij = ct * 16 + li;
             f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int s = 0; s < 3; ++s) {
                 const int k = 4 * s + lk;
-                const double b = (ij < NN && k < 9) ? R[ij * 10 + k] : 0.0;
+                const double b = (ij < T::NN && k < 9) ? R[ij * 10 + k] : 0.0;
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b, acc, 0, 0, 0);
             }
-            if (ij < NN) {
-                const int i = ij / N, j = ij - i * N;
-                const int64_t off = (int64_t)i * N3 + j;
+            // rows past 9 ne and columns past n^2 are zero padding, never read
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    if (obase[r] >= 0) Em[obase[r] + off] = acc[r];
-            }
+            for (int r = 0; r < 4; ++r) __builtin_nontemporal_store(acc[r], tile + 64 * r);
         }
     }
 }
 
-// Scatter of materialised element matrices: entry c of the map is Em[c].
-static __global__ __launch_bounds__(kBlock) void kGatherMaterialised(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
-                                                                     const uint32_t* __restrict__ taskK,
-                                                                     const uint32_t* __restrict__ centry,
-                                                                     const int32_t* __restrict__ segSlot,
-                                                                     const double* __restrict__ Em,
-                                                                     double* __restrict__ v64, float* __restrict__ v32,
-                                                                     int add) {
-    auto value = [&](uint32_t c) { return Em[c]; };
+// Scatter of materialised element matrices: entry c of the map is read from
+// its tile-major place in Em.
+template <int N>
+__global__ __launch_bounds__(kBlock) void kGatherMaterialised(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
+                                                              const uint32_t* __restrict__ taskK,
+                                                              const uint32_t* __restrict__ centry,
+                                                              const int32_t* __restrict__ segSlot,
+                                                              const double* __restrict__ Em,
+                                                              double* __restrict__ v64, float* __restrict__ v32,
+                                                              int add) {
+    auto value = [&](uint32_t c) { return Em[emTileIndex<N>(c)]; };
     segScatter(ntasks, taskSeg, taskK, centry, segSlot, v64, v32, add, value);
 }
 

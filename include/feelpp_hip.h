@@ -68,8 +68,8 @@ typedef struct {
 /* How elasticity element matrices reach the CSR values.  FUSED: each entry is
  * contracted inside the scatter from the element factors and reference tensors.
  * MFMA (P3 and P4): all element matrices are first materialised with FP64
- * matrix-core GEMMs into a workspace of nelements * (3 nloc)^2 doubles held by
- * the matrix, then summed by the same scatter.  AUTO: P1/P2 use FUSED; P3/P4
+ * matrix-core GEMMs into a workspace of about nelements * (3 nloc)^2 doubles
+ * held by the matrix, then summed by the same scatter.  AUTO: P1/P2 use FUSED; P3/P4
  * time both on the graph's first assembly with add = 0 and keep the faster. */
 enum { FEEL_HIP_MATRICES_AUTO = 0, FEEL_HIP_MATRICES_FUSED = 1, FEEL_HIP_MATRICES_MFMA = 2 };
 
