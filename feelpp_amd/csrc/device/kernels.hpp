@@ -48,10 +48,20 @@ __device__ inline bool blockDone(const CgState* st) {
     return flag != 0;
 }
 
+// Sum of count partials, identical in every block: four independent
+// accumulators per thread so that its loads are in flight together, then a
+// fixed combination order.
 __device__ inline double reducePartials(const double* __restrict__ part, int count = kReduceBlocks) {
-    double v = 0;
-    for (int i = threadIdx.x; i < count; i += kBlock) v += part[i];
-    return blockReduceSum(v);
+    double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+    int i = threadIdx.x;
+    for (; i + 3 * kBlock < count; i += 4 * kBlock) {
+        v0 += part[i];
+        v1 += part[i + kBlock];
+        v2 += part[i + 2 * kBlock];
+        v3 += part[i + 3 * kBlock];
+    }
+    for (; i < count; i += kBlock) v0 += part[i];
+    return blockReduceSum((v0 + v1) + (v2 + v3));
 }
 
 // 16-byte vector of T (four floats or two doubles) for the streaming CG
