@@ -452,8 +452,9 @@ def bench_serving(fa, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50,
+                    help="timed assemble+solve steps per form (p50/p99 are over these)")
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64, help="cubes per side of the Kuhn mesh")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--rtol", type=float, default=1e-10)
