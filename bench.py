@@ -362,11 +362,12 @@ def bench_kernels(fa, args):
 
 
 def bench_elastic(fa, args):
-    """Short fixed run of the elasticity element-matrix kernels for
-    hardware-counter passes: a plain 1 GiB device copy (counter calibration),
-    then on the P4 beam (40x10x10 Kuhn cells, 24 000 tets, 811 923 dofs) three
-    FP64+FP32 bilinear assemblies on each element-matrix path: contracted inside
-    the scatter, and materialised on the FP64 matrix cores then scattered."""
+    """Short fixed run of the element-matrix kernels for hardware-counter
+    passes: a plain 1 GiB device copy (counter calibration), then four FP64+FP32
+    bilinear assemblies on each element-matrix path (contracted inside the
+    scatter; materialised on the FP64 matrix cores then scattered) for the P4
+    elasticity beam (40x10x10 Kuhn cells, 24 000 tets, 811 923 dofs) and for P4
+    diffusion on the Kuhn 32^3 cube (196 608 tets, quadrature order 6)."""
     import torch
     nbytes = 1 << 30
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -393,8 +394,24 @@ def bench_elastic(fa, args):
             A.assemble(fa.ELASTICITY, 2 * k, young=young, poisson=poisson, precision=fa.FP64_FP32, matrices=m)
         ctx.synchronize()
         out[name] = round((time.perf_counter() - t) / 3 * 1e3, 3)
-    print(json.dumps({"kernels": f"copy 1 GiB x5; P{k} beam elasticity assembly x4 per element-matrix path",
-                      "elements": ne, "dofs": space.ndof, "bilinear_ms": out}), flush=True)
+    # scalar P4 diffusion on the Kuhn 32^3 cube (196 608 tets), the largest problem of the batched mix
+    sc, se = fa.mesh_kuhn_box(32, jitter=0.15, seed=args.seed)
+    sspace = fa.Space.from_mesh(ctx, sc, se, 4, 1)
+    S = fa.Matrix(fa.Graph.from_space(sspace), fa.FP64_FP32)
+    kd = ctx.vector(len(se), fa.lognormal(len(se), 0.0, 1.0, args.seed))
+    sout = {}
+    for name, m in (("fused", fa.MATRICES_FUSED), ("mfma", fa.MATRICES_MFMA)):
+        for rep in range(4):
+            if rep == 1:
+                ctx.synchronize()
+                t = time.perf_counter()
+            S.assemble(fa.DIFFUSION, 6, kdiff=kd, precision=fa.FP64_FP32, matrices=m)
+        ctx.synchronize()
+        sout[name] = round((time.perf_counter() - t) / 3 * 1e3, 3)
+    print(json.dumps({"kernels": f"copy 1 GiB x5; P{k} beam elasticity and P4 Kuhn 32^3 diffusion assemblies, "
+                                 "x4 per element-matrix path",
+                      "elements": ne, "dofs": space.ndof, "bilinear_ms": out,
+                      "scalar_p4": {"elements": len(se), "dofs": sspace.ndof, "bilinear_ms": sout}}), flush=True)
 
 
 def bench_serving(fa, args):

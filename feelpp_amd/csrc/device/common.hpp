@@ -87,6 +87,11 @@ struct feel_hip_ctx {
     feelpp_amd::CgState* state = nullptr;
     feelpp_amd::CgState* hstate = nullptr;   // pinned host mirror
     double* hist = nullptr;   // 2 x kHistMax CG coefficients (beta, p.Ap) of the running solve
+    // element matrices materialised by the matrix-core assembly path; shared by
+    // every assembly on this context (they are ordered on its stream), grown
+    // to the largest request and kept until the context goes
+    double* elemmat = nullptr;
+    size_t elemmatBytes = 0;
 
     int alloc(void** p, size_t bytes);
     void release(void* p, size_t bytes);
@@ -169,8 +174,6 @@ struct feel_hip_mat {
     double* dinv64 = nullptr;
     float* dinv32 = nullptr;
     bool dinvValid = false;
-    double* elemmat = nullptr;    // materialised element matrices (elasticity path)
-    size_t elemmatBytes = 0;
     // SELL values for the solver, refreshed from the CSR values when stale
     double* s64 = nullptr;
     float* s32 = nullptr;

@@ -134,6 +134,7 @@ void unref(feel_hip_ctx* c) {
     c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvPartCap));
     c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
     c->release(c->hist, sizeof(double) * 2 * kHistMax);
+    c->release(c->elemmat, c->elemmatBytes);
     if (c->hstate) (void)hipHostFree(c->hstate);
     for (auto& ev : c->events) {
         (void)hipEventDestroy(ev.a);
@@ -656,7 +657,6 @@ int feel_hip_mat_destroy(feel_hip_mat* A) {
     c->release(A->v32, sizeof(float) * A->g->nnz);
     c->release(A->dinv64, sizeof(double) * A->g->n);
     c->release(A->dinv32, sizeof(float) * A->g->n);
-    c->release(A->elemmat, A->elemmatBytes);
     const int64_t n = A->g->n;
     for (double* v : {A->r64, A->z64, A->p64, A->w64}) c->release(v, sizeof(double) * n);
     for (float* v : {A->r32, A->z32, A->p32, A->w32, A->x32, A->b32}) c->release(v, sizeof(float) * n);
@@ -888,7 +888,89 @@ int feel_hip_vec_dot(const feel_hip_vec* a, const feel_hip_vec* b, double* out) 
 int64_t feel_hip_vec_size(const feel_hip_vec* v) { return v ? v->n : -1; }
 void* feel_hip_vec_device_ptr(feel_hip_vec* v) { return v ? (void*)v->d : nullptr; }
 
+}  // extern "C"
+
 // ============================================================== assembly
+namespace {
+
+// The context's workspace of materialised element matrices, at least bytes.
+int ensureElemmat(feel_hip_ctx* c, size_t bytes) {
+    if (c->elemmatBytes >= bytes) return FEEL_HIP_OK;
+    c->release(c->elemmat, c->elemmatBytes);
+    c->elemmat = nullptr;
+    c->elemmatBytes = 0;
+    TRY_CALL(c->alloc((void**)&c->elemmat, bytes));
+    c->elemmatBytes = bytes;
+    return FEEL_HIP_OK;
+}
+
+// Element matrices on the matrix cores (kElemMatMfma, tile-major into the
+// context's workspace), then their segmented sum into the CSR values.
+template <int NN, int RPE, int FS, int FR, int RS, int KV, class Entry>
+int materialisedPath(feel_hip_mat* A, const double* F, const double* rt, double* v64, float* v32, int add, int grid) {
+    feel_hip_graph* g = A->g;
+    feel_hip_space* s = g->space;
+    feel_hip_ctx* c = s->ctx;
+    TRY_CALL(ensureElemmat(c, sizeof(double) * (size_t)MatTiles<NN, RPE>::doubles(s->ne)));
+    const int64_t rtiles = (RPE * s->ne + 15) / 16;
+    const int mgrid = (int)std::min<int64_t>((rtiles + kBlock / 64 - 1) / (kBlock / 64), 16384);
+    hipLaunchKernelGGL((kElemMatMfma<NN, RPE, FS, FR, RS, KV>), dim3(mgrid), dim3(kBlock), 0, c->stream, s->ne, F, rt,
+                       c->elemmat);
+    hipLaunchKernelGGL(kGatherMaterialised<Entry>, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks, g->taskSeg,
+                       g->taskK, g->centry, g->segSlot, c->elemmat, v64, v32, add);
+    FEEL_HIP_CHECK(hipGetLastError());
+    return FEEL_HIP_OK;
+}
+
+// Runs the element-matrix path for this assembly (FEEL_HIP_MATRICES_*; 0 =
+// the graph's tuned choice).  An untuned graph assembled with add = 0 times
+// one run of each path after a warm-up run and keeps the faster; the last run
+// leaves valid values.  With add = 1 an untuned graph takes the fused path
+// (timing both would accumulate twice).
+template <class Fused, class Mfma>
+int runElementPath(feel_hip_mat* A, int path, int add, Fused fused, Mfma mfma) {
+    feel_hip_graph* g = A->g;
+    feel_hip_ctx* c = g->space->ctx;
+    if (path == 0 && add) path = FEEL_HIP_MATRICES_FUSED;
+    if (path == FEEL_HIP_MATRICES_MFMA) return mfma();
+    if (path == FEEL_HIP_MATRICES_FUSED) return fused();
+    hipEvent_t e0, e1;
+    FEEL_HIP_CHECK(hipEventCreate(&e0));
+    FEEL_HIP_CHECK(hipEventCreate(&e1));
+    float ms[2] = {0.0f, 0.0f};
+    bool haveMfma = true;
+    for (int k = 0; k < 2; ++k) {
+        for (int rep = 0; rep < 2; ++rep) {
+            if (rep == 1) (void)hipEventRecord(e0, c->stream);
+            if (k == 0) {
+                TRY_CALL(fused());
+            } else if (mfma() != FEEL_HIP_OK) {
+                (void)hipGetLastError();
+                haveMfma = false;   // workspace does not fit: the fused path stays
+                break;
+            }
+            if (rep == 1) (void)hipEventRecord(e1, c->stream);
+        }
+        if (k == 1 && !haveMfma) break;
+        FEEL_HIP_CHECK(hipEventSynchronize(e1));
+        FEEL_HIP_CHECK(hipEventElapsedTime(&ms[k], e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->elemPath = haveMfma && ms[1] < ms[0] ? FEEL_HIP_MATRICES_MFMA : FEEL_HIP_MATRICES_FUSED;
+    if (!haveMfma) TRY_CALL(fused());
+    if (g->elemPath == FEEL_HIP_MATRICES_FUSED) {
+        c->release(c->elemmat, c->elemmatBytes);
+        c->elemmat = nullptr;
+        c->elemmatBytes = 0;
+    }
+    return FEEL_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d, int add) {
     GUARD_BEGIN
     if (!A || !d) return setError(FEEL_HIP_EINVAL, "null argument");
@@ -933,76 +1015,16 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
             FEEL_HIP_CHECK(hipGetLastError());
             return FEEL_HIP_OK;
         };
-        // element matrices on the matrix cores into A->elemmat, then the plain
-        // segmented sum; the workspace (nelements (3 nloc)^2 doubles, rounded
-        // up to whole 16 x 16 tiles) stays with the matrix for later assemblies
+        // elasticity GEMM: 9 rows per element, factors (c*3+d)*10 + t, tables (i*n+j)*10 + t, t < 9
         auto mfma = [&]() -> int {
-            const size_t bytes = sizeof(double) * (size_t)(s->order == 3 ? ElasticTiles<20>::doubles(s->ne)
-                                                                          : ElasticTiles<35>::doubles(s->ne));
-            if (A->elemmatBytes < bytes) {
-                c->release(A->elemmat, A->elemmatBytes);
-                A->elemmat = nullptr;
-                A->elemmatBytes = 0;
-                TRY_CALL(c->alloc((void**)&A->elemmat, bytes));
-                A->elemmatBytes = bytes;
-            }
-            const int64_t rtiles = (9 * s->ne + 15) / 16;
-            const int mgrid = (int)std::min<int64_t>((rtiles + kBlock / 64 - 1) / (kBlock / 64), 16384);
             if (s->order == 3)
-                hipLaunchKernelGGL(kElasticMatMfma<20>, dim3(mgrid), dim3(kBlock), 0, c->stream, s->ne, s->geo, rt,
-                                   A->elemmat);
-            else
-                hipLaunchKernelGGL(kElasticMatMfma<35>, dim3(mgrid), dim3(kBlock), 0, c->stream, s->ne, s->geo, rt,
-                                   A->elemmat);
-            if (s->order == 3)
-                hipLaunchKernelGGL(kGatherMaterialised<20>, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks,
-                                   g->taskSeg, g->taskK, g->centry, g->segSlot, A->elemmat, v64, v32, add);
-            else
-                hipLaunchKernelGGL(kGatherMaterialised<35>, dim3(grid), dim3(kBlock), 0, c->stream, g->ntasks,
-                                   g->taskSeg, g->taskK, g->centry, g->segSlot, A->elemmat, v64, v32, add);
-            FEEL_HIP_CHECK(hipGetLastError());
-            return FEEL_HIP_OK;
+                return materialisedPath<400, 9, kElasticF, 10, 10, 9, ElasticEntry<20>>(A, s->geo, rt, v64, v32, add,
+                                                                                       grid);
+            return materialisedPath<1225, 9, kElasticF, 10, 10, 9, ElasticEntry<35>>(A, s->geo, rt, v64, v32, add,
+                                                                                    grid);
         };
-        if (path == 0 && add) path = FEEL_HIP_MATRICES_FUSED;   // timing both would accumulate twice
         c->tic("assemble_gather");
-        if (path == 0) {
-            // first assembly of a P3/P4 graph: time one run of each after a
-            // warm-up run, keep the faster; the last run leaves valid values
-            hipEvent_t e0, e1;
-            FEEL_HIP_CHECK(hipEventCreate(&e0));
-            FEEL_HIP_CHECK(hipEventCreate(&e1));
-            float ms[2] = {0.0f, 0.0f};
-            bool haveMfma = true;
-            for (int k = 0; k < 2; ++k) {
-                for (int rep = 0; rep < 2; ++rep) {
-                    if (rep == 1) (void)hipEventRecord(e0, c->stream);
-                    if (k == 0) {
-                        TRY_CALL(fused());
-                    } else if (mfma() != FEEL_HIP_OK) {
-                        (void)hipGetLastError();
-                        haveMfma = false;   // workspace does not fit: the fused path stays
-                        break;
-                    }
-                    if (rep == 1) (void)hipEventRecord(e1, c->stream);
-                }
-                if (k == 1 && !haveMfma) break;
-                FEEL_HIP_CHECK(hipEventSynchronize(e1));
-                FEEL_HIP_CHECK(hipEventElapsedTime(&ms[k], e0, e1));
-            }
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            g->elemPath = haveMfma && ms[1] < ms[0] ? FEEL_HIP_MATRICES_MFMA : FEEL_HIP_MATRICES_FUSED;
-            if (!haveMfma) TRY_CALL(fused());
-            if (g->elemPath == FEEL_HIP_MATRICES_FUSED) {
-                c->release(A->elemmat, A->elemmatBytes);
-                A->elemmat = nullptr;
-                A->elemmatBytes = 0;
-            }
-        } else if (path == FEEL_HIP_MATRICES_MFMA) {
-            TRY_CALL(mfma());
-        } else {
-            TRY_CALL(fused());
-        }
+        TRY_CALL(runElementPath(A, path, add, fused, mfma));
         c->toc();
         FEEL_HIP_CHECK(hipGetLastError());
         if (v64 && A->v32 && !v32) hipLaunchKernelGGL(kCastF32, dim3(gridFor(g->nnz)), dim3(kBlock), 0, c->stream, g->nnz, v64, A->v32);
@@ -1041,21 +1063,39 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     FEEL_HIP_CHECK(hipGetLastError());
     const int grid = segGrid(g);
     TRY_CALL(clearEmptySlots(c, g, v64, v32, add));
-    c->tic("assemble_gather");
-#define GATHER(NN, ST) hipLaunchKernelGGL((kGatherScalar<NN, ST>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS)
     const int key = s->dim * 10 + s->order;
-    switch (key) {
-        case 21: GATHER(9, 4); break;
-        case 22: GATHER(36, 4); break;
-        case 31: GATHER(16, 8); break;
-        case 32: GATHER(100, 8); break;
-        case 33: GATHER(400, 8); break;
-        case 34:
-            hipLaunchKernelGGL((kGatherScalar<1225, 8, false>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS);
-            break;
-        default: c->toc(); return setError(FEEL_HIP_EUNSUPPORTED, "no scatter kernel for this space");
-    }
+    if (key != 21 && key != 22 && (key < 31 || key > 34))
+        return setError(FEEL_HIP_EUNSUPPORTED, "no scatter kernel for this space");
+    int path = d->matrices;
+    if (path < FEEL_HIP_MATRICES_AUTO || path > FEEL_HIP_MATRICES_MFMA)
+        return setError(FEEL_HIP_EINVAL, "unknown element-matrix path");
+    const bool dense = key == 33 || key == 34;   // P3/P4 tetrahedra
+    if (path == FEEL_HIP_MATRICES_MFMA && !dense)
+        return setError(FEEL_HIP_EUNSUPPORTED, "matrix-core element matrices are built for P3 and P4 tetrahedra");
+    if (path == FEEL_HIP_MATRICES_AUTO) path = dense ? g->elemPath : FEEL_HIP_MATRICES_FUSED;
+    auto fused = [&]() -> int {
+#define GATHER(NN, ST) hipLaunchKernelGGL((kGatherScalar<NN, ST>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS)
+        switch (key) {
+            case 21: GATHER(9, 4); break;
+            case 22: GATHER(36, 4); break;
+            case 31: GATHER(16, 8); break;
+            case 32: GATHER(100, 8); break;
+            case 33: GATHER(400, 8); break;
+            default:
+                hipLaunchKernelGGL((kGatherScalar<1225, 8, false>), dim3(grid), dim3(kBlock), 0, c->stream, SEG_ARGS);
+        }
 #undef GATHER
+        FEEL_HIP_CHECK(hipGetLastError());
+        return FEEL_HIP_OK;
+    };
+    // scalar GEMM: one row per element, factors e*8 + t, tables (i*n+j)*8 + t, t < 7
+    auto mfma = [&]() -> int {
+        if (key == 33)
+            return materialisedPath<400, 1, 8, 0, 8, 7, ScalarEntry<20>>(A, s->geo, rt, v64, v32, add, grid);
+        return materialisedPath<1225, 1, 8, 0, 8, 7, ScalarEntry<35>>(A, s->geo, rt, v64, v32, add, grid);
+    };
+    c->tic("assemble_gather");
+    TRY_CALL(runElementPath(A, path, add, fused, mfma));
     c->toc();
     FEEL_HIP_CHECK(hipGetLastError());
     if (v64 && A->v32 && !v32) hipLaunchKernelGGL(kCastF32, dim3(gridFor(g->nnz)), dim3(kBlock), 0, c->stream, g->nnz, v64, A->v32);

@@ -351,63 +351,81 @@ __global__ __launch_bounds__(kBlock) void kGatherElastic(int64_t ntasks, const u
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-// Elasticity element matrices materialised on the FP64 matrix cores.  For all
-// elements at once the contraction above is one GEMM: rows m = 9e + 3c + d,
-// 9 per element, times the reference tensors, columns i*n + j, over t = 0..8
-// (K = 9, padded to 12 with zeros):
-//   Ke[(c,i),(d,j)] = sum_t F[e][(c,d)][t] R[(i,j)][t].
-// A wave owns a 16-row tile and sweeps every 16-column tile with three
-// v_mfma_f64_16x16x4f64.  Operands: lane l holds A[l%16][l/16] and
-// B[l/16][l%16]; the 16x16 result holds row l/16 + 4r, column l%16 in
-// register r.  Em is tile-major (tile (rt, ct) at (rt*NCT + ct)*256, row-major
-// inside), so each result register is stored as 512 contiguous bytes;
-// emTileIndex maps a gather-map entry id to its place.
-template <int N>
-struct ElasticTiles {
-    static constexpr int NN = N * N, N3 = 3 * N, NCT = (NN + 15) / 16;
-    static constexpr int64_t NN3 = (int64_t)N3 * N3;
-    static int64_t doubles(int64_t ne) { return (ne * 9 + 15) / 16 * NCT * 256; }
+// Element matrices materialised on the FP64 matrix cores.  For all elements
+// at once the reference-tensor contraction is one GEMM: RPE rows per element
+// (elasticity: m = 9e + 3c + d; scalar forms: m = e) times the reference
+// tensors, columns i*nloc + j, over the factor index t < KV (padded to a
+// multiple of 4 with zeros):
+//   elasticity  Ke[(c,i),(d,j)] = sum_{t<9} F[e][(c,d)][t] R[(i,j)][t]
+//   scalar      A_e(i,j)        = sum_{t<7} F[e][t] R[(i,j)][t]
+// Factor of row m, index t: F[(m / RPE) * FS + (m % RPE) * FR + t]; table
+// entry: R[(i*nloc + j) * RS + t].  A wave owns a 16-row tile and sweeps every
+// 16-column tile with v_mfma_f64_16x16x4f64.  Operands: lane l holds
+// A[l%16][l/16] and B[l/16][l%16]; the 16x16 result holds row l/16 + 4r,
+// column l%16 in register r.  Em is tile-major (tile (rt, ct) at
+// (rt*NCT + ct)*256, row-major inside), so each result register is stored as
+// 512 contiguous bytes; the entry maps below give a gather-map entry's place.
+template <int NN, int RPE>
+struct MatTiles {
+    static constexpr int NCT = (NN + 15) / 16;
+    static int64_t doubles(int64_t ne) { return (ne * RPE + 15) / 16 * NCT * 256; }
+    __device__ static int64_t at(int64_t m, uint32_t col) {
+        return ((m >> 4) * NCT + (col >> 4)) * 256 + (m & 15) * 16 + (col & 15);
+    }
 };
 
+// entry id e*(3n)^2 + (c*n+i)*3n + d*n + j of an elasticity element matrix
 template <int N>
-__device__ inline int64_t emTileIndex(uint32_t c) {
-    using T = ElasticTiles<N>;
-    const uint32_t el = c / (uint32_t)T::NN3;
-    const uint32_t loc = c - el * (uint32_t)T::NN3;
-    const uint32_t rl = loc / T::N3, cl = loc - rl * T::N3;
-    const uint32_t cc = rl / N, i = rl - cc * N, dd = cl / N, j = cl - dd * N;
-    const int64_t m = (int64_t)el * 9 + cc * 3 + dd;
-    const uint32_t ij = i * N + j;
-    return ((m >> 4) * T::NCT + (ij >> 4)) * 256 + (m & 15) * 16 + (ij & 15);
-}
+struct ElasticEntry {
+    using Tiles = MatTiles<N * N, 9>;
+    __device__ static int64_t at(uint32_t c) {
+        constexpr uint32_t N3 = 3 * N, NN3 = N3 * N3;
+        const uint32_t el = c / NN3;
+        const uint32_t loc = c - el * NN3;
+        const uint32_t rl = loc / N3, cl = loc - rl * N3;
+        const uint32_t cc = rl / N, i = rl - cc * N, dd = cl / N, j = cl - dd * N;
+        return Tiles::at((int64_t)el * 9 + cc * 3 + dd, i * N + j);
+    }
+};
 
+// entry id e*n^2 + i*n + j of a scalar element matrix
 template <int N>
-__global__ __launch_bounds__(kBlock) void kElasticMatMfma(int64_t ne, const double* __restrict__ F,
-                                                          const double* __restrict__ R, double* __restrict__ Em) {
-    using T = ElasticTiles<N>;
+struct ScalarEntry {
+    using Tiles = MatTiles<N * N, 1>;
+    __device__ static int64_t at(uint32_t c) {
+        const uint32_t el = c / (uint32_t)(N * N);
+        return Tiles::at(el, c - el * (uint32_t)(N * N));
+    }
+};
+
+template <int NN, int RPE, int FS, int FR, int RS, int KV>
+__global__ __launch_bounds__(kBlock) void kElemMatMfma(int64_t ne, const double* __restrict__ F,
+                                                       const double* __restrict__ R, double* __restrict__ Em) {
+    using T = MatTiles<NN, RPE>;
+    constexpr int KS = (KV + 3) / 4;
     const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
-    const int64_t rows = ne * 9, rtiles = (rows + 15) / 16;
+    const int64_t rows = ne * RPE, rtiles = (rows + 15) / 16;
     const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
     for (int64_t rt = wave; rt < rtiles; rt += nwaves) {
         const int64_t m = rt * 16 + li;
-        double a[3];
+        double a[KS];
 #pragma unroll
-        for (int s = 0; s < 3; ++s) {
+        for (int s = 0; s < KS; ++s) {
             const int k = 4 * s + lk;
-            a[s] = (m < rows && k < 9) ? F[(m / 9) * kElasticF + (m % 9) * 10 + k] : 0.0;
+            a[s] = (m < rows && k < KV) ? F[(m / RPE) * FS + (m % RPE) * FR + k] : 0.0;
         }
         double* tile = Em + rt * T::NCT * 256 + lk * 16 + li;
         for (int ct = 0; ct < T::NCT; ++ct, tile += 256) {
             const int ij = ct * 16 + li;
             f64x4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int s = 0; s < 3; ++s) {
+            for (int s = 0; s < KS; ++s) {
                 const int k = 4 * s + lk;
-                const double b = (ij < T::NN && k < 9) ? R[ij * 10 + k] : 0.0;
+                const double b = (ij < NN && k < KV) ? R[ij * RS + k] : 0.0;
                 acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b, acc, 0, 0, 0);
             }
-            // rows past 9 ne and columns past n^2 are zero padding, never read
+            // rows past RPE ne and columns past nloc^2 are zero padding, never read
 #pragma unroll
             for (int r = 0; r < 4; ++r) __builtin_nontemporal_store(acc[r], tile + 64 * r);
         }
@@ -416,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void kElasticMatMfma(int64_t ne, const doub
 
 // Scatter of materialised element matrices: entry c of the map is read from
 // its tile-major place in Em.
-template <int N>
+template <class Entry>
 __global__ __launch_bounds__(kBlock) void kGatherMaterialised(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
                                                               const uint32_t* __restrict__ taskK,
                                                               const uint32_t* __restrict__ centry,
@@ -424,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void kGatherMaterialised(int64_t ntasks, co
                                                               const double* __restrict__ Em,
                                                               double* __restrict__ v64, float* __restrict__ v32,
                                                               int add) {
-    auto value = [&](uint32_t c) { return Em[emTileIndex<N>(c)]; };
+    auto value = [&](uint32_t c) { return Em[Entry::at(c)]; };
     segScatter(ntasks, taskSeg, taskK, centry, segSlot, v64, v32, add, value);
 }
 

@@ -62,15 +62,17 @@ typedef struct {
     const feel_hip_vec* mass_coef;  /* length nelements or null */
     const feel_hip_vec* young;      /* elasticity: E per element */
     const feel_hip_vec* poisson;    /* elasticity: nu per element */
-    int matrices;                   /* elasticity element matrices, FEEL_HIP_MATRICES_* (0 = autotuned) */
+    int matrices;                   /* element-matrix path, FEEL_HIP_MATRICES_* (0 = autotuned) */
 } feel_hip_bilinear_desc;
 
-/* How elasticity element matrices reach the CSR values.  FUSED: each entry is
- * contracted inside the scatter from the element factors and reference tensors.
- * MFMA (P3 and P4): all element matrices are first materialised with FP64
- * matrix-core GEMMs into a workspace of about nelements * (3 nloc)^2 doubles
- * held by the matrix, then summed by the same scatter.  AUTO: P1/P2 use FUSED; P3/P4
- * time both on the graph's first assembly with add = 0 and keep the faster. */
+/* How element matrices reach the CSR values.  FUSED: each entry is contracted
+ * inside the scatter from the element factors and reference tensors.  MFMA
+ * (P3 and P4 on tetrahedra, scalar forms and elasticity): all element matrices
+ * are first materialised with FP64 matrix-core GEMMs into a workspace of about
+ * nelements * (ncomp nloc)^2 doubles held by the context (one per context, grown
+ * to the largest assembly, freed with the context), then summed by the same
+ * scatter.  AUTO: other spaces use FUSED; P3/P4 tetrahedra time both on the
+ * graph's first assembly with add = 0 and keep the faster. */
 enum { FEEL_HIP_MATRICES_AUTO = 0, FEEL_HIP_MATRICES_FUSED = 1, FEEL_HIP_MATRICES_MFMA = 2 };
 
 /* Source of integrate(_range=elements(mesh), _expr=f*id(v)) (linearformcontext.hpp:294-305) */

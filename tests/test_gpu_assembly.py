@@ -48,6 +48,30 @@ def test_diffusion_mass_fp64_vs_oracle(ctx, orc, dim, order, n, qo):
     assert rel_fro(v32, ref) < 1e-6
 
 
+@pytest.mark.parametrize("order,n,qo", [(3, 3, 6), (4, 2, 8)])
+def test_scalar_element_matrix_paths(ctx, orc, order, n, qo):
+    # P3/P4 tetrahedra: fused contraction, matrix-core GEMM, and the timed
+    # choice all match the oracle; replace and accumulate both honoured
+    coords, elems, e2d, ndof, ia, ja, sp, g = setup(ctx, orc, 3, order, n)
+    k = orc.lognormal(len(elems), 0.0, 1.0, 11)
+    m = orc.uniform(len(elems), 0.5, 2.0, 12)
+    ref = orc.assemble_scalar(3, order, qo, coords, elems, e2d, ia, ja, diff=1.3, kdiff=k, mass=0.7, kmass=m)
+    kd, km = ctx.vector(len(elems), k), ctx.vector(len(elems), m)
+    out = {}
+    for mat in (fa.MATRICES_FUSED, fa.MATRICES_MFMA, fa.MATRICES_AUTO):
+        A = fa.Matrix(g, fa.FP64_FP32)
+        A.assemble(fa.MASS | fa.DIFFUSION, qo, diff=1.3, kdiff=kd, mass=0.7, kmass=km, matrices=mat)
+        assert rel_fro(A.values(), ref) < TOL64
+        assert rel_fro(A.values(fa.FP32).astype(np.float64), ref) < 1e-6
+        out[mat] = A.values()
+        A.assemble(fa.MASS | fa.DIFFUSION, qo, diff=1.3, kdiff=kd, mass=0.7, kmass=km, matrices=mat, add=True)
+        assert rel_fro(A.values(), 2.0 * ref) < TOL64
+    assert rel_fro(out[fa.MATRICES_MFMA], out[fa.MATRICES_FUSED]) < 1e-14
+    coords2, elems2, *_, g2 = setup(ctx, orc, 3, 2, 2)
+    with pytest.raises(fa.FeelHipError):
+        fa.Matrix(g2).assemble(fa.DIFFUSION, 4, matrices=fa.MATRICES_MFMA)
+
+
 def test_c1_full_size_stiffness_plus_mass(ctx, orc):
     # 128x128 P1 triangles, jitter 0.2h, k = 1, quadrature order 2
     coords, elems, e2d, ndof, ia, ja, sp, g = setup(ctx, orc, 2, 1, 128, jitter=0.2, seed=1)
