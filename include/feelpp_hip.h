@@ -72,7 +72,8 @@ typedef struct {
  * nelements * (ncomp nloc)^2 doubles held by the context (one per context, grown
  * to the largest assembly, freed with the context), then summed by the same
  * scatter.  AUTO: other spaces use FUSED; P3/P4 tetrahedra time both on the
- * graph's first assembly with add = 0 and keep the faster. */
+ * graph's first assembly with add = 0 and keep the faster (that one assembly
+ * waits for the context stream; later ones are asynchronous). */
 enum { FEEL_HIP_MATRICES_AUTO = 0, FEEL_HIP_MATRICES_FUSED = 1, FEEL_HIP_MATRICES_MFMA = 2 };
 
 /* Source of integrate(_range=elements(mesh), _expr=f*id(v)) (linearformcontext.hpp:294-305) */
