@@ -177,6 +177,32 @@ __global__ __launch_bounds__(kBlock) void kGeoScalar(int64_t ne, const double* _
 // lane holding a slot's last contribution writes it.  Slot index = task's
 // first segment + ends before the lane; segSlot maps segments to CSR slots
 // when the pattern has slots without contributions (else identity).
+// DPP move of v from another lane of the wavefront (dpp_ctrl CTRL, rows
+// enabled by ROWS); lanes without a source and lanes of disabled rows get 0.
+// row_shr:n (0x110 + n) reads lane - n of the same 16-lane row;
+// row_bcast:15 (0x142) gives row r+1 lane 15 of row r; row_bcast:31 (0x143)
+// gives rows 2 and 3 lane 31.
+template <int CTRL, int ROWS>
+__device__ inline int dppMove(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ inline double dppMove(double v) {
+    const int lo = dppMove<CTRL, ROWS>(__double2loint(v));
+    const int hi = dppMove<CTRL, ROWS>(__double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// One step of a segmented inclusive scan: x takes the value moved in unless a
+// segment starts at or after its source; head records a start.
+template <int CTRL, int ROWS>
+__device__ inline void segScanStep(double& x, int& head) {
+    const double y = dppMove<CTRL, ROWS>(x);
+    const int hy = dppMove<CTRL, ROWS>(head);
+    if (!head) x = y + x;
+    head |= hy;
+}
+
 template <class Value>
 __device__ inline void segScatter(int64_t ntasks, const uint32_t* __restrict__ taskSeg,
                                   const uint32_t* __restrict__ taskK, const uint32_t* __restrict__ centry,
@@ -198,17 +224,17 @@ __device__ inline void segScatter(int64_t ntasks, const uint32_t* __restrict__ t
             const bool isEnd = valid && (c >> 31);
             double x = valid ? value(c & 0x7fffffffu) : 0.0;
             const uint64_t ends = __ballot(isEnd);
-            bool head = lane == 0 ? !open : (((ends >> (lane - 1)) & 1ull) != 0);
+            int head = lane == 0 ? !open : (int)((ends >> (lane - 1)) & 1ull);
             if (lane == 0 && open) x = carry + x;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const double y = __shfl_up(x, off, 64);
-                const int hy = __shfl_up((int)head, off, 64);
-                if (lane >= off) {
-                    if (!head) x = y + x;
-                    head = head || hy;
-                }
-            }
+            // segmented inclusive scan in registers: Hillis-Steele inside each
+            // 16-lane row, then the row carries (lane 15 into row 1 and row 3,
+            // lane 31 into rows 2 and 3); a fixed order, so run-to-run identical
+            segScanStep<0x111, 0xf>(x, head);
+            segScanStep<0x112, 0xf>(x, head);
+            segScanStep<0x114, 0xf>(x, head);
+            segScanStep<0x118, 0xf>(x, head);
+            segScanStep<0x142, 0xa>(x, head);
+            segScanStep<0x143, 0xc>(x, head);
             if (isEnd) {
                 const uint32_t sidx = seg + (uint32_t)__popcll(ends & below);
                 const int64_t p = segSlot ? (int64_t)segSlot[sidx] : (int64_t)sidx;
@@ -222,7 +248,8 @@ __device__ inline void segScatter(int64_t ntasks, const uint32_t* __restrict__ t
             }
             seg += (uint32_t)__popcll(ends);
             const int last = (int)((kend - 1 - k) < 63u ? (kend - 1 - k) : 63u);
-            carry = __shfl(x, last, 64);
+            carry = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), last),
+                                     __builtin_amdgcn_readlane(__double2loint(x), last));
             open = ((ends >> last) & 1ull) == 0;
         }
     }
