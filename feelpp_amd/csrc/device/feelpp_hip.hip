@@ -1172,16 +1172,14 @@ int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const fee
     if (mode != FEEL_HIP_ON_ELIMINATION && mode != FEEL_HIP_ON_ELIMINATION_SYMMETRIC)
         return setError(FEEL_HIP_EINVAL, "unknown elimination mode");
     feel_hip_ctx* c = g->space->ctx;
-    const int grid = gridFor(g->n, 8192);
+    constexpr int L = 16;   // lanes per row
+    const int grid = (int)std::min<int64_t>((g->n + kBlock / L - 1) / (kBlock / L), 16384);
     const double* gv = values ? values->d : nullptr;
     double* b = rhs ? rhs->d : nullptr;
     const int sym = mode == FEEL_HIP_ON_ELIMINATION_SYMMETRIC;
-    if (A->v64)
-        hipLaunchKernelGGL(kDirichlet<double>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, A->v64, b,
-                           rows->flag, gv, sym, diag, 1);
-    if (A->v32)
-        hipLaunchKernelGGL(kDirichlet<float>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, A->v32, b,
-                           rows->flag, gv, sym, diag, A->v64 ? 0 : 1);
+    if (A->v64 || A->v32)
+        hipLaunchKernelGGL(kDirichletRows<L>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, A->v64,
+                           A->v32, b, rows->flag, gv, sym, diag);
     FEEL_HIP_CHECK(hipGetLastError());
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;

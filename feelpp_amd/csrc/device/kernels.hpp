@@ -634,27 +634,43 @@ static __global__ __launch_bounds__(kBlock) void kGatherVec(int64_t n, const uin
 
 // ---------------------------------------------------------------- Dirichlet
 // MatZeroRows / MatZeroRowsColumns semantics (matrixpetsc.cpp:1764-1850).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void kDirichlet(int64_t n, const int32_t* __restrict__ ia,
-                                                     const int32_t* __restrict__ ja, T* __restrict__ val,
-                                                     double* __restrict__ rhs, const uint8_t* __restrict__ flag,
-                                                     const double* __restrict__ g, int symmetric, double diag,
-                                                     int touchRhs) {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+// Both precisions of on() in one pass, L lanes per row so that a row's
+// column indices and values are read coalesced: flagged rows become
+// diag * e_r (rhs_r = diag g_r); with symmetric elimination the flagged
+// columns of the other rows are zeroed and rhs_r -= sum A_rc g_c, summed from
+// the FP64 values when present (else the FP32 ones) over the row's lanes.
+template <int L>
+__global__ __launch_bounds__(kBlock) void kDirichletRows(int64_t n, const int32_t* __restrict__ ia,
+                                                         const int32_t* __restrict__ ja, double* __restrict__ v64,
+                                                         float* __restrict__ v32, double* __restrict__ rhs,
+                                                         const uint8_t* __restrict__ flag,
+                                                         const double* __restrict__ g, int symmetric, double diag) {
+    const int lane = threadIdx.x & (L - 1);
+    constexpr int RPB = kBlock / L;
+    for (int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / L; r < n; r += (int64_t)gridDim.x * RPB) {
         const int b = ia[r], e = ia[r + 1];
         if (flag[r]) {
-            for (int p = b; p < e; ++p) val[p] = (ja[p] == r) ? (T)diag : (T)0;
-            if (touchRhs && rhs) rhs[r] = diag * (g ? g[r] : 0.0);
+            for (int p = b + lane; p < e; p += L) {
+                const bool d = ja[p] == r;
+                if (v64) v64[p] = d ? diag : 0.0;
+                if (v32) v32[p] = d ? (float)diag : 0.0f;
+            }
+            if (lane == 0 && rhs) rhs[r] = diag * (g ? g[r] : 0.0);
         } else if (symmetric) {
             double corr = 0.0;
-            for (int p = b; p < e; ++p) {
+            for (int p = b + lane; p < e; p += L) {
                 const int c = ja[p];
                 if (flag[c]) {
-                    if (g) corr += (double)val[p] * g[c];
-                    val[p] = (T)0;
+                    if (g) corr += (v64 ? v64[p] : (double)v32[p]) * g[c];
+                    if (v64) v64[p] = 0.0;
+                    if (v32) v32[p] = 0.0f;
                 }
             }
-            if (touchRhs && rhs && g) rhs[r] -= corr;
+            if (rhs && g) {
+#pragma unroll
+                for (int off = L / 2; off > 0; off >>= 1) corr += __shfl_xor(corr, off, L);
+                if (lane == 0) rhs[r] -= corr;
+            }
         }
     }
 }
