@@ -1124,8 +1124,9 @@ int feel_hip_assemble_linear(feel_hip_space* s, const feel_hip_linear_desc* d, f
         hipLaunchKernelGGL(kElemRhs<2>, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->nloc, s->ncomp, Q,
                            s->coords, s->elems, qt, d->kind, d->params[0], d->params[1], d->params[2], s->elemvec);
     FEEL_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(kGatherVec, dim3(gridFor(s->ndof, 8192)), dim3(kBlock), 0, c->stream, s->ndof, s->rptr, s->rentry,
-                       s->elemvec, b->d, add);
+    constexpr int L = 8;   // lanes per row
+    hipLaunchKernelGGL(kGatherVec<L>, dim3((int)std::min<int64_t>((s->ndof + kBlock / L - 1) / (kBlock / L), 16384)),
+                       dim3(kBlock), 0, c->stream, s->ndof, s->rptr, s->rentry, s->elemvec, b->d, add);
     FEEL_HIP_CHECK(hipGetLastError());
     return FEEL_HIP_OK;
     GUARD_END

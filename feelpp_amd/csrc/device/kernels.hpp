@@ -621,14 +621,21 @@ __global__ __launch_bounds__(kBlock) void kElemNorm(int64_t ne, int nloc, int nc
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-static __global__ __launch_bounds__(kBlock) void kGatherVec(int64_t n, const uint32_t* __restrict__ rptr,
+// Linear-form scatter: b_r = sum of the element-vector entries landing in row
+// r (rentry, ascending element order), L lanes per row so that the row's entry
+// list is read coalesced; lane partials are combined in a fixed order.
+template <int L>
+__global__ __launch_bounds__(kBlock) void kGatherVec(int64_t n, const uint32_t* __restrict__ rptr,
                                                      const uint32_t* __restrict__ rentry,
-                                                     const double* __restrict__ be, double* __restrict__ b,
-                                                     int add) {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+                                                     const double* __restrict__ be, double* __restrict__ b, int add) {
+    const int lane = threadIdx.x & (L - 1);
+    constexpr int RPB = kBlock / L;
+    for (int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / L; r < n; r += (int64_t)gridDim.x * RPB) {
         double s = 0.0;
-        for (uint32_t k = rptr[r]; k < rptr[r + 1]; ++k) s += be[rentry[k]];
-        b[r] = add ? b[r] + s : s;
+        for (uint32_t k = rptr[r] + lane; k < rptr[r + 1]; k += L) s += be[rentry[k]];
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+        if (lane == 0) b[r] = add ? b[r] + s : s;
     }
 }
 
@@ -1186,6 +1193,19 @@ __global__ __launch_bounds__(kBlock) void kCsrToSell(int64_t nslots, const int32
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
         const int32_t p = slot2csr[s];
         out[s] = p < 0 ? (T)0 : (c64 ? (T)c64[p] : (T)c32[p]);
+    }
+}
+
+// Both SELL value images in one pass over the slot map: FP64 from the FP64
+// CSR values, FP32 from the FP32 ones.
+static __global__ __launch_bounds__(kBlock) void kCsrToSellBoth(int64_t nslots, const int32_t* __restrict__ slot2csr,
+                                                                const double* __restrict__ c64,
+                                                                const float* __restrict__ c32,
+                                                                double* __restrict__ s64, float* __restrict__ s32) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t p = slot2csr[s];
+        s64[s] = p < 0 ? 0.0 : c64[p];
+        s32[s] = p < 0 ? 0.0f : c32[p];
     }
 }
 

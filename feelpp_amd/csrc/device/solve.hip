@@ -127,6 +127,13 @@ int ensureSell(feel_hip_mat* A, bool need64, bool need32) {
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
     bool changed = false;
+    if (need64 && need32 && !A->sell64Valid && !A->sell32Valid && A->v64 && A->v32) {
+        TRY_CALL(allocWork(c, &A->s64, g->nslots));
+        TRY_CALL(allocWork(c, &A->s32, g->nslots));
+        hipLaunchKernelGGL(kCsrToSellBoth, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,
+                           A->v64, A->v32, A->s64, A->s32);
+        A->sell64Valid = A->sell32Valid = changed = true;
+    }
     if (need64 && !A->sell64Valid) {
         if (!A->v64) return setError(FEEL_HIP_EINVAL, "matrix holds no FP64 values");
         TRY_CALL(allocWork(c, &A->s64, g->nslots));
