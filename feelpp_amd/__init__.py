@@ -25,6 +25,9 @@ MASS, DIFFUSION, ELASTICITY = 1, 2, 4
 MATRICES_AUTO, MATRICES_FUSED, MATRICES_MFMA = 0, 1, 2
 SRC_CONST, SRC_SINSIN, SRC_VECTOR = 0, 1, 2
 ELIMINATION, ELIMINATION_SYMMETRIC = 0, 1
+SAME_NONZERO_PATTERN, DIFFERENT_NONZERO_PATTERN, SUBSET_NONZERO_PATTERN = 0, 1, 3
+NORM_1, NORM_FROBENIUS, NORM_INF = 0, 2, 3
+VEC_SUM, VEC_L1, VEC_L2, VEC_MIN, VEC_MAX, VEC_LINF = 0, 1, 2, 3, 4, 5
 
 
 class FeelHipError(RuntimeError):
@@ -115,6 +118,15 @@ _SIGS = {
     "feel_hip_mat_get_values": (C.c_int, [_vp, C.c_int, _vp]),
     "feel_hip_mat_set_values": (C.c_int, [_vp, _f64p]),
     "feel_hip_mat_mult": (C.c_int, [_vp, _vp, _vp]),
+    "feel_hip_mat_scale": (C.c_int, [_vp, C.c_double]),
+    "feel_hip_mat_axpy": (C.c_int, [_vp, C.c_double, _vp, C.c_int]),
+    "feel_hip_mat_copy": (C.c_int, [_vp, _vp]),
+    "feel_hip_mat_diagonal": (C.c_int, [_vp, _vp]),
+    "feel_hip_mat_transpose": (C.c_int, [_vp, _vp]),
+    "feel_hip_mat_norm": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
+    "feel_hip_mat_energy": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
+    "feel_hip_mat_set_matrix": (C.c_int, [_vp, C.c_int, _i32p, C.c_int, _i32p, _f64p]),
+    "feel_hip_mat_get_entries": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),
     "feel_hip_vec_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp)]),
     "feel_hip_vec_destroy": (C.c_int, [_vp]),
     "feel_hip_vec_set": (C.c_int, [_vp, _f64p]),
@@ -126,6 +138,12 @@ _SIGS = {
     "feel_hip_vec_copy": (C.c_int, [_vp, _vp]),
     "feel_hip_vec_set_rows": (C.c_int, [_vp, _vp, _vp]),
     "feel_hip_vec_size": (C.c_int64, [_vp]),
+    "feel_hip_vec_reduce": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_double)]),
+    "feel_hip_vec_set_constant": (C.c_int, [_vp, C.c_double]),
+    "feel_hip_vec_shift": (C.c_int, [_vp, C.c_double]),
+    "feel_hip_vec_set_indexed": (C.c_int, [_vp, C.c_int64, _i32p, _f64p]),
+    "feel_hip_vec_add_indexed": (C.c_int, [_vp, C.c_int64, _i32p, _f64p]),
+    "feel_hip_vec_mult_add": (C.c_int, [_vp, _vp, _vp]),
     "feel_hip_vec_device_ptr": (_vp, [_vp]),
     "feel_hip_assemble_bilinear": (C.c_int, [_vp, C.POINTER(BilinearDesc), C.c_int]),
     "feel_hip_assemble_linear": (C.c_int, [_vp, C.POINTER(LinearDesc), _vp, C.c_int]),
@@ -404,6 +422,54 @@ class Vector(_Handle):
         """self[r] = values[r] (0 without values) for the rows of a Rows set"""
         _check(lib().feel_hip_vec_set_rows(self.h, rows.h, values.h if values else None))
 
+    def reduce(self, kind):
+        out = C.c_double()
+        _check(lib().feel_hip_vec_reduce(self.h, kind, C.byref(out)))
+        return out.value
+
+    def sum(self):
+        return self.reduce(VEC_SUM)
+
+    def min(self):
+        return self.reduce(VEC_MIN)
+
+    def max(self):
+        return self.reduce(VEC_MAX)
+
+    def l1_norm(self):
+        return self.reduce(VEC_L1)
+
+    def l2_norm(self):
+        return self.reduce(VEC_L2)
+
+    def linfty_norm(self):
+        return self.reduce(VEC_LINF)
+
+    def set_constant(self, s):
+        _check(lib().feel_hip_vec_set_constant(self.h, s))
+
+    def shift(self, s):
+        """self += s (Vector::add(s))"""
+        _check(lib().feel_hip_vec_shift(self.h, s))
+
+    def set_indexed(self, idx, vals):
+        """Vector::setVector(i, n, v): self[idx[k]] = vals[k]"""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        assert idx.shape == vals.shape
+        _check(lib().feel_hip_vec_set_indexed(self.h, idx.shape[0], idx, vals))
+
+    def add_indexed(self, idx, vals):
+        """Vector::addVector(i, n, v): self[idx[k]] += vals[k]"""
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        assert idx.shape == vals.shape
+        _check(lib().feel_hip_vec_add_indexed(self.h, idx.shape[0], idx, vals))
+
+    def mult_add(self, A, x):
+        """Vector::addVector(V, A): self += A x"""
+        _check(lib().feel_hip_vec_mult_add(self.h, A.h, x.h))
+
 
 class Space(_Handle):
     """A mesh plus the DofTable of Pch<order> (ncomp=1) or Pchv<order> (ncomp=dim), on the device."""
@@ -506,6 +572,49 @@ class Matrix(_Handle):
 
     def mult(self, x, y):
         _check(lib().feel_hip_mat_mult(self.h, x.h, y.h))
+
+    def scale(self, a):
+        _check(lib().feel_hip_mat_scale(self.h, a))
+
+    def axpy(self, a, X, structure=SAME_NONZERO_PATTERN):
+        """self += a X (MatrixSparse::addMatrix(a, X, structure))"""
+        _check(lib().feel_hip_mat_axpy(self.h, a, X.h, structure))
+
+    def copy_from(self, src):
+        _check(lib().feel_hip_mat_copy(self.h, src.h))
+
+    def diagonal(self):
+        ctx = self.graph.space.ctx
+        d = Vector(ctx, self.graph.space.ndof)
+        _check(lib().feel_hip_mat_diagonal(self.h, d.h))
+        return d.get()
+
+    def transpose_into(self, At):
+        _check(lib().feel_hip_mat_transpose(self.h, At.h))
+
+    def norm(self, kind):
+        out = C.c_double()
+        _check(lib().feel_hip_mat_norm(self.h, kind, C.byref(out)))
+        return out.value
+
+    def energy(self, v, u, transpose=False):
+        """v^T A u (v^T A^T u with transpose)"""
+        out = C.c_double()
+        _check(lib().feel_hip_mat_energy(self.h, v.h, u.h, int(transpose), C.byref(out)))
+        return out.value
+
+    def set_matrix(self, rows, cols, data):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        data = np.ascontiguousarray(data, dtype=np.float64).reshape(len(rows), len(cols))
+        _check(lib().feel_hip_mat_set_matrix(self.h, len(rows), rows, len(cols), cols, data))
+
+    def get_entries(self, rows, cols):
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        out = np.zeros(len(rows))
+        _check(lib().feel_hip_mat_get_entries(self.h, len(rows), rows, cols, out))
+        return out
 
     def assemble(self, kind, quad_order, diff=1.0, kdiff=None, mass=1.0, kmass=None, young=None, poisson=None,
                  precision=0, add=False, matrices=MATRICES_AUTO):

@@ -239,10 +239,11 @@ __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ 
     else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
     else if (it >= st->maxit) reason = -3;
     else if (beta == 0.0) reason = 3;
+    else if (it > 0 && beta * I.slots[(it - 1) & 1] < 0.0) reason = -8;   // indefinite preconditioner
     if (item.item == 0 && threadIdx.x == 0) {
         st->dp = dp;
         if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
-        if (reason) { st->done = 1; st->reason = reason; st->its = it; }
+        if (reason) { st->done = 1; st->reason = reason; st->its = reason == -8 ? it + 1 : it; }
         I.slots[it & 1] = beta;
     }
     if (reason) return;
@@ -498,10 +499,11 @@ __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restr
         else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
         else if (it >= st->maxit) reason = -3;
         else if (beta == 0.0) reason = 3;
+        else if (it > 0 && beta * I.slots[(it - 1) & 1] < 0.0) reason = -8;   // indefinite preconditioner
         if (item.item == 0 && threadIdx.x == 0) {
             st->dp = dp;
             if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
-            if (reason) { st->done = 1; st->reason = reason; st->its = it; }
+            if (reason) { st->done = 1; st->reason = reason; st->its = reason == -8 ? it + 1 : it; }
             I.slots[it & 1] = beta;
         }
         if (!reason && threadIdx.x == 0) {
@@ -563,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict
         const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
         if (item.item == 0 && threadIdx.x == 0) {
             I.slots[2 + (it & 1)] = dpi;
-            if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
+            if (indefinite) { st->done = 1; st->reason = -10; st->its = it + 1; }
         }
         if (!indefinite && threadIdx.x == 0) {
             sact[b] = 1;
@@ -644,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ ite
     const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
     if (item.item == 0 && threadIdx.x == 0) {
         I.slots[2 + (it & 1)] = dpi;
-        if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
+        if (indefinite) { st->done = 1; st->reason = -10; st->its = it + 1; }
     }
     if (indefinite) return;
     const double a = I.slots[it & 1] / dpi;
@@ -1110,6 +1112,15 @@ void finish(feel_hip_batch* B, Request& r, int reason) {
     }
 }
 
+// copies the request's FP64 solution to Feel++ dof numbering (instance buffer `out`)
+void permuteOut(feel_hip_batch* B, Request& r) {
+    const BatchFamily& f = B->fams[r.family];
+    const InstBuffers& ib = B->bufs[r.buf];
+    hipLaunchKernelGGL(kBPermuteOut, dim3((int)std::min<int64_t>(4096, (f.n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       B->ctx->stream, f.n, f.g->perm, ib.x64, ib.out);
+    r.unpermuted = true;
+}
+
 template <typename T>
 int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
     feel_hip_ctx* c = B->ctx;
@@ -1167,16 +1178,6 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         else if (r.outer >= B->opts.max_outer || r.its >= B->opts.maxit) finish(B, r, -3);
         else toInner.push_back(&r);
     }
-    for (auto& kv : B->reqs) {
-        Request& r = kv.second;
-        if (r.phase == Phase::Done && r.buf >= 0 && !r.unpermuted) {
-            r.unpermuted = true;
-            const BatchFamily& f = B->fams[r.family];
-            const InstBuffers& ib = B->bufs[r.buf];
-            hipLaunchKernelGGL(kBPermuteOut, dim3((int)std::min<int64_t>(4096, (f.n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
-                               c->stream, f.n, f.g->perm, ib.x64, ib.out);
-        }
-    }
     // FP32 CG with reliable updates: the device stops once its residual
     // estimate has dropped by inner_rtol since the last FP64 residual (or
     // reaches the tolerance); the FP32 residual then restarts from the true one
@@ -1221,6 +1222,7 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         FEEL_HIP_CHECK(hipMemcpyAsync(hs.data(), B->states, sizeof(CgState) * B->nStates, hipMemcpyDeviceToHost, c->stream));
         FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
         std::vector<Request*> ended;
+        std::vector<int> stopReason;   // per ended request: 0, or the reason the inner solve broke down
         for (Request* r : inner) {
             const CgState& st = hs[r->state];
             if (!st.done) continue;
@@ -1228,11 +1230,14 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
             r->outer += 1;
             if (r->outer == 1) r->devRnorm0 = st.rnorm0;
             r->estimate = st.dp * r->sigma;
-            if (st.reason == -8 || st.reason == -9 || st.reason == -4) { finish(B, *r, st.reason); continue; }
+            if (st.reason == -9) { finish(B, *r, -9); continue; }   // x64 keeps the last finite update
             r->xscale = r->sigma;   // x64 += sigma * x_fp32 (the FP32 residual is r64 / sigma)
             ended.push_back(r);
+            stopReason.push_back(st.reason == -8 || st.reason == -10 || st.reason == -4 ? st.reason : 0);
         }
-        // 5. FP64 residual of the instances whose inner solve ended
+        // 5. FP64 residual of the instances whose inner solve ended; a breakdown
+        // (indefinite preconditioner or matrix, divergence) returns the iterate it reached with its true
+        // FP64 residual, as KSPSolve leaves x at the last iterate
         if (!ended.empty()) {
             TRY_CALL(buildWork(B, ended, &nItems));
             hipLaunchKernelGGL(kBAxpy<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
@@ -1240,9 +1245,17 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
             hipLaunchKernelGGL(kBResidual, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
             FEEL_HIP_CHECK(hipGetLastError());
             TRY_CALL(residualNorms(B, ended));
-            for (Request* r : ended) { r->xscale = 0; r->phase = Phase::Resid; }
+            for (size_t i = 0; i < ended.size(); ++i) {
+                Request* r = ended[i];
+                r->xscale = 0;
+                r->phase = Phase::Resid;
+                if (stopReason[i]) finish(B, *r, stopReason[i]);
+            }
         }
     }
+    // 6. solutions of the requests finished in this step, back to Feel++ numbering
+    for (auto& kv : B->reqs)
+        if (kv.second.phase == Phase::Done && kv.second.buf >= 0 && !kv.second.unpermuted) permuteOut(B, kv.second);
     int act = 0, done = 0;
     for (auto& kv : B->reqs) {
         if (kv.second.phase == Phase::Done) ++done;
@@ -1403,10 +1416,16 @@ int feel_hip_batch_step(feel_hip_batch* B, int iterations, int* n_active, int* n
 
 int feel_hip_batch_poll(feel_hip_batch* B, int64_t* request, feel_hip_solve_data* sd) {
     if (!B || !request) return setError(FEEL_HIP_EINVAL, "null argument");
-    if (B->finished.empty()) return 0;
-    const int64_t id = B->finished.front();
-    B->finished.pop_front();
-    const Request& r = B->reqs[id];
+    // ids whose solution was already taken (and the request released) are skipped
+    auto it = B->reqs.end();
+    int64_t id = -1;
+    while (!B->finished.empty() && it == B->reqs.end()) {
+        id = B->finished.front();
+        B->finished.pop_front();
+        it = B->reqs.find(id);
+    }
+    if (it == B->reqs.end()) return 0;
+    const Request& r = it->second;
     *request = id;
     if (sd) {
         sd->converged = r.reason > 0;
@@ -1415,6 +1434,7 @@ int feel_hip_batch_poll(feel_hip_batch* B, int64_t* request, feel_hip_solve_data
         sd->reason = r.reason;
         sd->outer_its = r.outer;
         sd->rnorm0 = r.rnorm0;
+        sd->fp64_its = B->opts.precision == FEEL_HIP_FP64 ? r.its : 0;
     }
     return 1;
 }
@@ -1425,6 +1445,7 @@ int feel_hip_batch_solution(feel_hip_batch* B, int64_t request, double* host_x) 
     if (it == B->reqs.end() || it->second.phase != Phase::Done) return setError(FEEL_HIP_EINVAL, "request not finished");
     Request& r = it->second;
     const BatchFamily& f = B->fams[r.family];
+    if (!r.unpermuted) permuteOut(B, r);
     if (host_x) {
         FEEL_HIP_CHECK(hipMemcpyAsync(host_x, B->bufs[r.buf].out, sizeof(double) * f.n, hipMemcpyDeviceToHost, B->ctx->stream));
     }

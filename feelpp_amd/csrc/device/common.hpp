@@ -164,6 +164,10 @@ struct feel_hip_graph {
     int64_t* ccptr = nullptr;     // nslices, slice start in packed (uint16 units)
     uint16_t* packed = nullptr;   // npacked
     int64_t npacked = 0;
+    // transpose map of a structurally symmetric pattern (built on first use):
+    // tpos[p] = CSR position of (j, i) for the entry p = (i, j)
+    int32_t* tpos = nullptr;
+    int symmetric = 0;            // 0 unknown, 1 structurally symmetric, -1 not
 };
 
 struct feel_hip_mat {

@@ -170,6 +170,7 @@ void unref(feel_hip_graph* g) {
     c->release(g->ia, sizeof(int32_t) * (g->n + 1));
     c->release(g->ja, sizeof(int32_t) * g->nnz);
     c->release(g->diagpos, sizeof(int32_t) * g->n);
+    if (g->tpos) c->release(g->tpos, sizeof(int32_t) * g->nnz);
     c->release(g->centry, sizeof(uint32_t) * g->ncontrib);
     c->release(g->taskSeg, sizeof(uint32_t) * (g->ntasks + 1));
     c->release(g->taskK, sizeof(uint32_t) * (g->ntasks + 1));

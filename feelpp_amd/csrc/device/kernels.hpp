@@ -732,10 +732,11 @@ __global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const 
     else if (it > 0 && dp >= st->dtol * rnorm0) reason = -4;
     else if (it >= st->maxit) reason = -3;
     else if (beta == 0.0) reason = 3;
+    else if (it > 0 && beta * betaSlots[(it - 1) & 1] < 0.0) reason = -8;   // indefinite preconditioner
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->dp = dp;
         if (it == 0) { st->ttol = ttol; st->rnorm0 = rnorm0; }
-        if (reason) { st->done = 1; st->reason = reason; st->its = it; }
+        if (reason) { st->done = 1; st->reason = reason; st->its = reason == -8 ? it + 1 : it; }
         betaSlots[it & 1] = beta;
         if (hist && it < kHistMax) hist[2 * it] = beta;
     }
@@ -784,7 +785,7 @@ __global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __rest
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         dpiSlots[it & 1] = dpi;
         if (hist && it < kHistMax) hist[2 * it + 1] = dpi;
-        if (indefinite) { st->done = 1; st->reason = -8; st->its = it + 1; }
+        if (indefinite) { st->done = 1; st->reason = -10; st->its = it + 1; }
     }
     if (indefinite) return;
     const double a = betaSlots[it & 1] / dpi;

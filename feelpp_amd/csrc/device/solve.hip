@@ -364,7 +364,7 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, MixedRun& out) {
         FEEL_HIP_CHECK(hipGetLastError());
         TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
         ++out.updates;
-        if (reason < 0) break;   // -3, -4, -8, -9 from the device tests
+        if (reason < 0) break;   // -3, -4, -8, -9, -10 from the device tests
         if (dp != dp) { reason = -9; break; }
         if (dp <= ttol) { reason = dp < o.atol ? 3 : 2; break; }
         if (dp >= o.dtol * rnorm0) { reason = -4; break; }

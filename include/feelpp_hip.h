@@ -104,7 +104,8 @@ typedef struct {
     int converged;      /* SolveData::isConverged */
     int its;            /* total Jacobi-CG iterations (A*p products) */
     double residual;    /* final ||B r|| (preconditioned norm), FP64 */
-    int reason;         /* KSPConvergedReason-like: 2 rtol, 3 atol, -3 maxit, -4 dtol, -8 indefinite, -9 nan */
+    int reason;         /* KSPConvergedReason: 2 rtol, 3 atol, -3 maxit, -4 dtol, -8 indefinite preconditioner,
+                           -9 nan, -10 indefinite matrix */
     int outer_its;      /* FP64 residual evaluations after the first (reliable updates; 1 for FP64) */
     double rnorm0;      /* initial ||B r_0|| */
     int fp64_its;       /* iterations run in FP64: all of them for FP64; for FP32, those after the solve
@@ -191,6 +192,33 @@ int feel_hip_mat_get_values(feel_hip_mat* A, int precision, void* host_values); 
 int feel_hip_mat_set_values(feel_hip_mat* A, const double* host_values);        /* FP64 values (and FP32 copy) */
 int feel_hip_mat_mult(feel_hip_mat* A, const feel_hip_vec* x, feel_hip_vec* y);  /* multVector, matrixsparse.hpp:530 */
 
+/* The rest of MatrixSparse's pure virtual interface on the device.
+ * Structure of addMatrix(a, X, structure), Feel::MatrixStructure (feelalg/enums.hpp:194-200):
+ * SAME requires X on the same pattern as Y; SUBSET and DIFFERENT accept any X whose
+ * entries all lie in Y's pattern (the graph of Y cannot grow: other X fail with
+ * FEEL_HIP_EPATTERN and leave Y unchanged). */
+enum { FEEL_HIP_SAME_NONZERO_PATTERN = 0, FEEL_HIP_DIFFERENT_NONZERO_PATTERN = 1,
+       FEEL_HIP_SUBSET_NONZERO_PATTERN = 3 };
+/* MatNorm types: l1Norm (max column sum), linftyNorm (max row sum), Frobenius */
+enum { FEEL_HIP_NORM_1 = 0, FEEL_HIP_NORM_FROBENIUS = 2, FEEL_HIP_NORM_INF = 3 };
+int feel_hip_mat_scale(feel_hip_mat* A, double a);                                 /* scale(a), matrixsparse.hpp:524 */
+int feel_hip_mat_axpy(feel_hip_mat* Y, double a, const feel_hip_mat* X, int structure); /* addMatrix(a, X), :512 */
+int feel_hip_mat_copy(feel_hip_mat* dst, const feel_hip_mat* src);                 /* operator=(M), :593 */
+int feel_hip_mat_diagonal(const feel_hip_mat* A, feel_hip_vec* d);                  /* diagonal(d), :602 */
+/* transpose(Mt), :626: At(i,j) = A(j,i) into a matrix on the same, structurally
+ * symmetric pattern (every graph built from one space is); EUNSUPPORTED otherwise */
+int feel_hip_mat_transpose(const feel_hip_mat* A, feel_hip_mat* At);
+/* l1Norm :697 (needs a structurally symmetric pattern), linftyNorm :709, Frobenius */
+int feel_hip_mat_norm(const feel_hip_mat* A, int type, double* out);
+/* energy(v, u, transpose), :673: v^T A u, or v^T A^T u */
+int feel_hip_mat_energy(const feel_hip_mat* A, const feel_hip_vec* v, const feel_hip_vec* u, int transpose,
+                        double* out);
+/* set(i, j, value) over a block (INSERT_VALUES), :466; repeated entries: the last wins */
+int feel_hip_mat_set_matrix(feel_hip_mat* A, int nrows, const int32_t* rows, int ncols, const int32_t* cols,
+                            const double* data);
+/* operator()(i, j), :591: m entries (rows[k], cols[k]); entries outside the pattern read 0 */
+int feel_hip_mat_get_entries(const feel_hip_mat* A, int64_t m, const int32_t* rows, const int32_t* cols, double* out);
+
 /* ---- vectors (Vector<double>, feelalg/vector.hpp:143-576) ------------------ */
 int feel_hip_vec_create(feel_hip_ctx* ctx, int64_t n, feel_hip_vec** out);      /* Backend::newVector, backend.hpp:571-592 */
 int feel_hip_vec_destroy(feel_hip_vec* v);
@@ -204,6 +232,20 @@ int feel_hip_vec_copy(feel_hip_vec* dst, const feel_hip_vec* src);              
 /* v[r] = values[r] (0 when values is null) for the rows of `rows`: the rhs part of
  * on() re-applied to a new right-hand side (feelvf/on.hpp:457-701) */
 int feel_hip_vec_set_rows(feel_hip_vec* v, const feel_hip_rows* rows, const feel_hip_vec* values);
+/* Reductions of Vector (vector.hpp:291-339): sum, l1Norm, l2Norm, min, max, linftyNorm */
+enum { FEEL_HIP_VEC_SUM = 0, FEEL_HIP_VEC_L1 = 1, FEEL_HIP_VEC_L2 = 2, FEEL_HIP_VEC_MIN = 3, FEEL_HIP_VEC_MAX = 4,
+       FEEL_HIP_VEC_LINF = 5 };
+int feel_hip_vec_reduce(const feel_hip_vec* v, int kind, double* out);
+int feel_hip_vec_set_constant(feel_hip_vec* v, double s);                         /* setConstant(s), vector.hpp:186 */
+int feel_hip_vec_shift(feel_hip_vec* v, double s);                                /* add(s), vector.hpp:460 */
+/* setVector(i, n, v), vector.hpp:443 (set(i, v) :438 for n = 1): v[idx[k]] = vals[k];
+ * repeated indices: the last value wins */
+int feel_hip_vec_set_indexed(feel_hip_vec* v, int64_t n, const int32_t* idx, const double* vals);
+/* addVector(i, n, v, K, K2), vector.hpp:453 (add(i, v) :448 for n = 1): v[idx[k]] += vals[k];
+ * repeated indices add up in order */
+int feel_hip_vec_add_indexed(feel_hip_vec* v, int64_t n, const int32_t* idx, const double* vals);
+/* addVector(V, A), vector.hpp:507: y += A x (x and y distinct) */
+int feel_hip_vec_mult_add(feel_hip_vec* y, const feel_hip_mat* A, const feel_hip_vec* x);
 int64_t feel_hip_vec_size(const feel_hip_vec* v);
 /* raw device pointer (double*) of the vector's storage */
 void* feel_hip_vec_device_ptr(feel_hip_vec* v);

@@ -39,6 +39,8 @@ def lib():
         L.orc_uniform.argtypes = [C.c_long, C.c_double, C.c_double, C.c_int, C.c_uint64, _f64p]
         L.orc_quadrature.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.orc_lagrange_nodes.argtypes = [C.c_int, C.c_int, C.c_void_p]
+        L.orc_set_p4_face_weight.restype = None
+        L.orc_set_p4_face_weight.argtypes = [C.c_double]
         L.orc_lagrange_eval.argtypes = [C.c_int, C.c_int, C.c_int, _f64p, _f64p, _f64p]
         L.orc_doftable.restype = C.c_long
         L.orc_doftable.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _i32p, _i32p]
@@ -125,6 +127,11 @@ def lagrange_nodes(dim, order):
     nodes = np.zeros((n, dim))
     lib().orc_lagrange_nodes(dim, order, _ptr(nodes))
     return nodes
+
+
+def set_p4_face_weight(w):
+    """Replace the P4 face-interior node weight (0: back to warp & blend); a test hook."""
+    lib().orc_set_p4_face_weight(float(w))
 
 
 def lagrange_eval(dim, order, pts):

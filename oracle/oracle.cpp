@@ -134,6 +134,11 @@ double p4_face_weight() {
     return (std::sqrt(3.0) * y + 1) / 3;
 }
 
+// Test hook: a positive value replaces the P4 face-interior weight above (0.5
+// is the equispaced lattice point), so that a test can solve one problem in
+// two nodal bases of the same P4 space; 0 restores the warp & blend weight.
+double g_p4_face_weight = 0.0;
+
 bool lagrange_nodes(int dim, int k, std::vector<double>& nodes) {
     nodes.clear();
     if (k < 1 || k > 4) return false;
@@ -160,7 +165,7 @@ bool lagrange_nodes(int dim, int k, std::vector<double>& nodes) {
         // three points per face; the l-th sits near the face's l-th vertex (equispaced
         // interior triangle lattice order, equispaced.hpp:520-545, mapped by
         // pt_to_face_tetrahedron :671-690)
-        const double tb = p4_face_weight(), ts = (1 - tb) / 2;
+        const double tb = g_p4_face_weight > 0 ? g_p4_face_weight : p4_face_weight(), ts = (1 - tb) / 2;
         for (int f = 0; f < 4; ++f)
             for (int l = 0; l < 3; ++l)
                 for (int c = 0; c < 3; ++c) {
@@ -476,6 +481,8 @@ int orc_quadrature(int dim, int order, double* pts, double* w) {
     if (w) std::copy(ww.begin(), ww.end(), w);
     return (int)ww.size();
 }
+
+void orc_set_p4_face_weight(double w) { g_p4_face_weight = w; }
 
 int orc_lagrange_nodes(int dim, int order, double* nodes) {
     std::vector<double> nd;
@@ -948,7 +955,9 @@ void orc_spmv(long n, const int* ia, const int* ja, const double* val, const dou
 // norm = preconditioned, ||B r_k|| <= max(rtol*||B r_0||, atol) converges,
 // ||B r_k|| >= dtol*||B r_0|| diverges; x0 = x on entry (nonzero guess).
 // its = number of A*p products; reason > 0 converged (2 rtol, 3 atol),
-// -3 max iterations, -4 dtol, -8 indefinite.
+// -3 max iterations, -4 dtol, -8 indefinite preconditioner ((r,z) changes sign,
+// KSP_DIVERGED_INDEFINITE_PC), -10 indefinite matrix ((p,Ap) zero or changes sign,
+// KSP_DIVERGED_INDEFINITE_MAT), -9 NaN (KSP_DIVERGED_NANORINF).
 // ---------------------------------------------------------------------------
 int orc_pcg(long n, const int* ia, const int* ja, const double* val, const double* b, double* x,
             double rtol, double atol, double dtol, int maxit, int* its_out, double* resid_out,
@@ -973,6 +982,7 @@ int orc_pcg(long n, const int* ia, const int* ja, const double* val, const doubl
     for (int i = 0; !reason && i < maxit; ++i) {
         its = i + 1;
         if (beta == 0.0) { reason = 3; break; }
+        if (i > 0 && beta * betaold < 0.0) { reason = -8; break; }
         if (i == 0) {
             p = z;
         } else {
@@ -983,7 +993,7 @@ int orc_pcg(long n, const int* ia, const int* ja, const double* val, const doubl
         orc_spmv(n, ia, ja, val, p.data(), w.data());
         dpi = 0;
         for (long k = 0; k < n; ++k) dpi += p[k] * w[k];
-        if (dpi == 0.0 || (i > 0 && dpi * dpiold <= 0.0)) { reason = -8; break; }
+        if (dpi == 0.0 || (i > 0 && dpi * dpiold <= 0.0)) { reason = -10; break; }
         double a = beta / dpi;
         for (long k = 0; k < n; ++k) { x[k] += a * p[k]; r[k] -= a * w[k]; }
         dp = 0;
@@ -1034,6 +1044,7 @@ int orc_pcg_mt(long n, const int* ia, const int* ja, const double* val, const do
     for (int it = 0; !reason && it < maxit; ++it) {
         its = it + 1;
         if (beta == 0.0) { reason = 3; break; }
+        if (it > 0 && beta * betaold < 0.0) { reason = -8; break; }
         const double bb = it == 0 ? 0.0 : beta / betaold;
         double pw = 0;
 #pragma omp parallel num_threads(nthreads)
@@ -1050,7 +1061,7 @@ int orc_pcg_mt(long n, const int* ia, const int* ja, const double* val, const do
         }
         dpiold = dpi;
         dpi = pw;
-        if (dpi == 0.0 || (it > 0 && dpi * dpiold <= 0.0)) { reason = -8; break; }
+        if (dpi == 0.0 || (it > 0 && dpi * dpiold <= 0.0)) { reason = -10; break; }
         const double a = beta / dpi;
         double zz = 0, zr = 0;
 #pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : zz, zr)

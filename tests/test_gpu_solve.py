@@ -100,6 +100,37 @@ def test_status_semantics(ctx, orc):
     assert st2["rnorm0"] < 1e-10 * st1["rnorm0"]
 
 
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32])
+def test_indefinite_reasons(ctx, orc, precision):
+    # KSPCG's two breakdowns on a 4-dof P1 mesh whose graph is completed to a
+    # dense 4 x 4 pattern: A = blockdiag(M, I), b = (1, 0, 0, 0).
+    # M = [[1,2],[2,1]]: (p, Ap) changes sign at the second step (-10,
+    # KSP_DIVERGED_INDEFINITE_MAT); M = [[1,1/2],[1/2,-1]]: the Jacobi
+    # preconditioner is indefinite and (r, z) changes sign at the top of the
+    # second step (-8, KSP_DIVERGED_INDEFINITE_PC); both after 2 iterations,
+    # as the oracle reports
+    coords, elems = orc.mesh_square(1, 0.0, 0)
+    e2d, ndof = orc.doftable(2, 1, 1, elems)
+    assert ndof == 4
+    ia = np.array([0, 4, 8, 12, 16], np.int32)
+    ja = np.tile(np.arange(4, dtype=np.int32), 4)
+    space = fa.Space(ctx, coords, elems, 1, 1, e2d, ndof)
+    g = fa.Graph(space, ia, ja)
+    b = np.array([1.0, 0.0, 0.0, 0.0])
+    for M, reason in (([[1.0, 2.0], [2.0, 1.0]], -10), ([[1.0, 0.5], [0.5, -1.0]], -8)):
+        dense = np.eye(4)
+        dense[:2, :2] = M
+        val = dense.ravel().copy()
+        xo, its_o, _, reason_o = orc.pcg(ia, ja, val, b, rtol=1e-12, maxit=50)
+        assert (reason_o, its_o) == (reason, 2)
+        A = fa.Matrix(g, fa.FP64_FP32)
+        A.set_values(val)
+        x = ctx.vector(4)
+        st = A.solve(ctx.vector(4, b), x, rtol=1e-12, maxit=50, precision=precision)
+        assert st["reason"] == reason and st["its"] == 2 and not st["converged"], st
+        assert np.allclose(x.get(), xo, rtol=1e-6, atol=1e-12)
+
+
 def test_solve_deterministic(ctx, orc):
     P = problem(ctx, orc, 3, 2, 5, 4, orc.FK_CONST, 2, 0.15, 4)
     x1, x2 = ctx.vector(P["ndof"]), ctx.vector(P["ndof"])
@@ -108,10 +139,12 @@ def test_solve_deterministic(ctx, orc):
     assert s1 == s2 and np.array_equal(x1.get(), x2.get())
 
 
-@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("order", [2, 3, 4])
 def test_clamped_beam_elasticity(ctx, orc, order):
     # left face clamped, body force (0,0,-1); FP64 and mixed solves vs the oracle
-    coords, elems = orc.mesh_kuhn(8, 2, 2, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=17)
+    # (P4 on a 4 x 1 x 1 beam: 17 x 5 x 5 nodes, 6375 dofs)
+    cells = (8, 2, 2) if order < 4 else (4, 1, 1)
+    coords, elems = orc.mesh_kuhn(*cells, lx=1.0, ly=0.25, lz=0.25, jitter=0.1, seed=17)
     e2d, ndof = orc.doftable(3, order, 3, elems)
     ia, ja = orc.graph(e2d, ndof)
     E = orc.uniform(len(elems), 1.0, 100.0, 1, log=True)
@@ -140,5 +173,68 @@ def test_clamped_beam_elasticity(ctx, orc, order):
     st = A.solve(bb, x, rtol=1e-10, maxit=50000, precision=fa.FP32)
     print(st)
     assert st["converged"] and rel_fro(x.get(), xo) < 1e-8
-    # kappa(D^-1 A) of the clamped beam is ~1e5-1e6: the FP32 solve hands over to FP64
-    assert 0 < st["fp64_its"] < st["its"]
+    # kappa(D^-1 A) of the clamped 8 x 2 x 2 beam is ~1e5-1e6: the FP32 solve hands over to FP64
+    if order < 4:
+        assert 0 < st["fp64_its"] < st["its"]
+    else:
+        assert st["fp64_its"] < st["its"]
+
+
+def test_full_size_beam_p4_properties(ctx):
+    # C3 at full size, P4 (40 x 10 x 10 Kuhn beam, 24 000 tets, 811 923 dofs):
+    # rigid-body modes in the kernel of the unconstrained operator, symmetry,
+    # the exact energy of a constant-strain field, then the clamped FP64 and
+    # mixed solves, whose true FP64 residual (CSR product, independent of the
+    # solver's block image) meets the tolerance
+    from helpers import dof_positions
+    coords, elems = fa.mesh_kuhn_box(40, 10, 10, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=7)
+    ne = len(elems)
+    E = fa.uniform(ne, 1.0, 100.0, 8, log=True)
+    nu = fa.uniform(ne, 0.2, 0.35, 9)
+    space = fa.Space.from_mesh(ctx, coords, elems, 4, 3)
+    g = fa.Graph.from_space(space)
+    ndof = space.ndof
+    assert ndof == 811923
+    A = fa.Matrix(g, fa.FP64_FP32)
+    young, poisson = ctx.vector(ne, E), ctx.vector(ne, nu)
+    A.assemble(fa.ELASTICITY, 8, young=young, poisson=poisson)
+    pos, _ = dof_positions(coords, elems, space.elem2dof, fa.lagrange_nodes(3, 4), ndof, ncomp=3)
+    X = pos[0::3]                                        # node positions (dof = 3 node + c)
+    scale = A.norm(fa.NORM_INF)
+    y = ctx.vector(ndof)
+    modes = [np.tile(np.eye(3)[c], len(X)) for c in range(3)]
+    for w in np.eye(3):
+        modes.append(np.cross(w, X).ravel())
+    for r in modes:
+        A.mult(ctx.vector(ndof, r), y)
+        assert np.abs(y.get()).max() <= 1e-12 * scale * np.abs(r).max()
+    rng = np.random.default_rng(3)
+    a, c = ctx.vector(ndof, rng.standard_normal(ndof)), ctx.vector(ndof, rng.standard_normal(ndof))
+    eac, eca = A.energy(a, c), A.energy(c, a)
+    assert abs(eac - eca) <= 1e-12 * abs(eac)
+    G = np.array([[0.3, -0.2, 0.1], [0.05, -0.4, 0.25], [0.15, 0.2, 0.35]])
+    u = ctx.vector(ndof, (X @ G.T).ravel())
+    eps = 0.5 * (G + G.T)
+    P = coords[elems]
+    vol = np.abs(np.linalg.det(P[:, 1:] - P[:, :1])) / 6.0
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    exact = np.sum(vol * (lam * np.trace(eps) ** 2 + 2 * mu * np.sum(eps * eps)))
+    assert abs(A.energy(u, u) - exact) <= 1e-10 * exact
+    # clamped solve
+    isb = fa.boundary_dofs(3, 4, 3, coords, elems, space.elem2dof, ndof, sel=1)
+    rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
+    b = ctx.vector(ndof)
+    fa.assemble_linear(space, b, fa.SRC_VECTOR, 8, (0.0, 0.0, -1.0))
+    A.zero_rows(rows, None, b, mode=fa.ELIMINATION_SYMMETRIC)
+    dinv = 1.0 / A.diagonal()
+    bh = b.get()
+    sols = {}
+    for prec in (fa.FP64, fa.FP32):
+        x = ctx.vector(ndof)
+        st = A.solve(b, x, rtol=1e-10, maxit=200000, precision=prec)
+        assert st["converged"], st
+        A.mult(x, y)
+        res = np.linalg.norm(dinv * (bh - y.get()))
+        assert res <= 1e-10 * np.linalg.norm(dinv * bh) * 1.01, (prec, st)
+        sols[prec] = x.get()
+    assert rel_fro(sols[fa.FP32], sols[fa.FP64]) < 1e-8

@@ -102,6 +102,59 @@ def test_msh22_roundtrip(tmp_path):
     assert np.array_equal(elems, e2) and np.abs(coords - c2).max() < 1e-15
 
 
+MSH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "msh")
+
+
+def _msh_sections(path):
+    """Nodes (id -> xyz) and elements (type, node ids) of an MSH 2.2 text file, read plainly."""
+    lines = open(path).read().split("\n")
+    nodes, elems = {}, []
+    i = lines.index("$Nodes")
+    for ln in lines[i + 2:i + 2 + int(lines[i + 1])]:
+        t = ln.split()
+        nodes[int(t[0])] = [float(v) for v in t[1:4]]
+    i = lines.index("$Elements")
+    for ln in lines[i + 2:i + 2 + int(lines[i + 1])]:
+        t = [int(v) for v in ln.split()]
+        elems.append((t[1], t[3 + t[2]:]))
+    return nodes, elems
+
+
+def test_msh22_reads_feelpp_eight_elements_mesh():
+    # Feel++'s own 2D test mesh (points, lines and triangles, no tags): point id =
+    # file id (importergmsh.hpp:2040-2066), only the triangles become elements,
+    # element id = their count in file order (addVolume/addFace: mesh->elements().size())
+    path = os.path.join(MSH, "eight_elements_mesh.msh")
+    coords, elems = fa.read_msh22(path, 2)
+    nodes, raw = _msh_sections(path)
+    assert coords.shape == (9, 2) and elems.shape == (8, 3)
+    assert np.array_equal(coords, np.array([nodes[i][:2] for i in sorted(nodes)]))
+    assert elems.tolist() == [[v - 1 for v in vs] for t, vs in raw if t == 2]
+    # the numbering built on it is the hand-derived first-touch table
+    e2d, ndof = fa.doftable(2, 1, 1, elems, 9)
+    assert ndof == 9
+    assert e2d.tolist() == [[0, 1, 2], [1, 3, 4], [2, 4, 5], [4, 6, 7], [1, 4, 2], [3, 6, 4], [4, 7, 5], [6, 8, 7]]
+    e2d2, ndof2 = fa.doftable(2, 2, 1, elems, 9)
+    assert ndof2 == 25 and e2d2[4].tolist() == [1, 7, 2, 14, 3, 9]
+
+
+@pytest.mark.parametrize("name,verts", [("one-elt-real-rotx.msh", [2, 3, 0, 1]), ("tetra.msh", [0, 2, 3, 1])])
+def test_msh22_reads_feelpp_tetrahedra(name, verts, orc):
+    # one-tetrahedron meshes of Feel++ with $PhysicalNames, tagged faces and
+    # edges, and vertex orders that are not ascending: the element keeps the
+    # file's vertex order (GmshOrdering is the identity on P1 simplices), so the
+    # edge orientations and the P2/P3 numbering follow it
+    path = os.path.join(MSH, name)
+    coords, elems = fa.read_msh22(path, 3)
+    nodes, raw = _msh_sections(path)
+    assert np.array_equal(coords, np.array([nodes[i] for i in sorted(nodes)]))
+    assert elems.tolist() == [verts]
+    for order in (2, 3):
+        e2d, ndof = fa.doftable(3, order, 1, elems, 4)
+        e2o, ndo = orc.doftable(3, order, 1, elems)
+        assert ndof == ndo and np.array_equal(e2d, e2o)
+
+
 def test_host_rejects_bad_input():
     with pytest.raises(fa.FeelHipError):
         fa.doftable(3, 2, 1, np.array([[0, 1, 2, 99]], np.int32), 4)

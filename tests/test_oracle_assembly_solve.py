@@ -178,10 +178,18 @@ def test_pcg_status_semantics(orc):
     assert its == 0 and reason > 0          # zero rhs converges immediately
     # an indefinite operator is reported through the status, not raised:
     # A = [[1,2],[2,1]] has eigenvalues 3 and -1; the second step has p'Ap < 0
+    # (KSP_DIVERGED_INDEFINITE_MAT)
     ia2 = np.array([0, 2, 4], np.int32); ja2 = np.array([0, 1, 0, 1], np.int32)
     x, its, res, reason = orc.pcg(ia2, ja2, np.array([1.0, 2.0, 2.0, 1.0]), np.array([1.0, 0.0]),
                                   rtol=1e-12, maxit=50)
+    assert reason == -10 and its == 2
+    # A = [[1,1/2],[1/2,-1]]: the Jacobi preconditioner diag(1,-1) is indefinite;
+    # r1 = (0,-1/2), z1 = (0,1/2), so (r,z) turns from 1 to -1/4 at the top of
+    # the second step (KSP_DIVERGED_INDEFINITE_PC)
+    x, its, res, reason = orc.pcg(ia2, ja2, np.array([1.0, 0.5, 0.5, -1.0]), np.array([1.0, 0.0]),
+                                  rtol=1e-12, maxit=50)
     assert reason == -8 and its == 2
+    assert np.allclose(x, [1.0, 0.0])             # x1 = x0 + p0 (a = 1), left where CG stopped
 
 
 @pytest.mark.parametrize("threads", [1, 3, 8])

@@ -138,3 +138,79 @@ def test_p4_node_set_structure(orc):
             w[l] = t
             assert np.allclose(nodes[22 + 3 * f + l], w @ V[list(vs)], atol=1e-14)
     assert np.allclose(nodes[34], [-0.5, -0.5, -0.5])
+
+
+def _bary_tri(pts):
+    """Barycentric coordinates on the [-1,1] reference triangle (-1,-1), (1,-1), (-1,1)."""
+    l1, l2 = (pts[:, 0] + 1) / 2, (pts[:, 1] + 1) / 2
+    return np.stack([1 - l1 - l2, l1, l2], axis=1)
+
+
+@pytest.mark.parametrize("name", ["nodes_tri_p4_equispaced.json", "nodes_tri_p4_fekete.json"])
+def test_p4_triangle_fixtures_interior_order(name):
+    # Feel++'s own P4 triangle node sets: vertices, 3 points per edge, then the
+    # three interior points, the l-th nearest vertex l (the lattice order that
+    # the 3D face points inherit through pointsBySubEntity, warpblend.hpp:100-118)
+    g = golden(name)
+    pts = np.array([n[1:3] for n in g["nodes"]])
+    assert len(pts) == 15
+    lam = _bary_tri(pts)
+    assert np.allclose(lam[:3], np.eye(3), atol=1e-15)
+    inner = lam[12:15]
+    assert list(inner.argmax(axis=1)) == [0, 1, 2]
+    for l in range(3):
+        other = np.delete(inner[l], l)
+        assert abs(other[0] - other[1]) < 1e-14          # on the median through vertex l
+    if "equispaced" in name:
+        assert np.allclose(inner.max(axis=1), 0.5, atol=1e-15)
+
+
+def test_p4_face_points_follow_the_lattice_order(orc):
+    # every face's three interior points in the order of the fixtures above:
+    # point l on the median through the face's l-th vertex, nearest to it, and
+    # off the opposite vertex of the tetrahedron
+    nodes = orc.lagrange_nodes(3, 4)
+    lam = np.hstack([1 - ((nodes + 1) / 2).sum(axis=1, keepdims=True), (nodes + 1) / 2])
+    F = [(1, 2, 3), (0, 2, 3), (0, 1, 3), (0, 1, 2)]
+    for f, vs in enumerate(F):
+        opp = ({0, 1, 2, 3} - set(vs)).pop()
+        for l in range(3):
+            w = lam[22 + 3 * f + l]
+            assert abs(w[opp]) < 1e-15
+            assert w.argmax() == vs[l]
+            others = [w[v] for m, v in enumerate(vs) if m != l]
+            assert abs(others[0] - others[1]) < 1e-15
+
+
+@pytest.mark.parametrize("t", [0.5, 0.6])
+def test_p4_solution_independent_of_face_node_position(orc, t):
+    # The P4 space does not depend on where the face nodes sit, so the Galerkin
+    # solution is the same function whatever the face weight; its values at the
+    # vertices and edge nodes (shared by both node sets) must agree.  This pins
+    # the P4 face-dof numbering (a non-conforming face numbering makes the two
+    # solutions differ) and the basis built on the face nodes, leaving only the
+    # node position itself as a convention.
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    coords, elems = orc.mesh_kuhn(2, jitter=0.15, seed=41)
+    e2d, ndof = orc.doftable(3, 4, 1, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    k = orc.lognormal(len(elems), 0.0, 1.0, 42)
+    isb = orc.boundary_dofs(3, 4, 1, coords, elems, e2d, ndof)
+
+    def solve():
+        val = orc.assemble_scalar(3, 4, 6, coords, elems, e2d, ia, ja, diff=1.0, kdiff=k, mass=0.5)
+        b = orc.assemble_rhs(3, 4, 1, 8, coords, elems, e2d, ndof, orc.FK_SINSIN, [1.0])
+        orc.dirichlet(1, ia, ja, val, b, isb)
+        return spla.spsolve(sp.csr_matrix((val, ja, ia), shape=(ndof, ndof)).tocsc(), b)
+
+    x_ref = solve()
+    try:
+        orc.set_p4_face_weight(t)
+        x_t = solve()
+    finally:
+        orc.set_p4_face_weight(0.0)
+    ve = np.unique(e2d[:, :22])                     # vertex and edge dofs (local 0..21)
+    fa = np.setdiff1d(np.unique(e2d[:, 22:34]), np.nonzero(isb)[0])   # interior face dofs
+    assert np.abs(x_t[ve] - x_ref[ve]).max() < 1e-10 * np.abs(x_ref).max()
+    assert np.abs(x_t[fa] - x_ref[fa]).max() > 1e-6 * np.abs(x_ref).max()   # the bases do differ
