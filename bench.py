@@ -69,19 +69,26 @@ class Problem:
         ctx.synchronize()
         log(f"device buffers {time.perf_counter() - t0:.1f}s, resident {ctx.device_bytes() / 2**30:.2f} GiB")
 
-    def step(self, precision, rtol, inner_rtol=1e-2):
+    def step(self, precision, rtol, inner_rtol=1e-2, inner_operator=0):
         fa = self.fa
         self.A.assemble(fa.DIFFUSION, self.qorder, kdiff=self.k, precision=fa.FP64_FP32)
         fa.assemble_linear(self.space, self.b, fa.SRC_CONST, self.order, (1.0,))
         self.A.zero_rows(self.rows, None, self.b, mode=fa.ELIMINATION_SYMMETRIC)
         self.x.zero()
-        return self.A.solve(self.b, self.x, rtol=rtol, maxit=100000, precision=precision, inner_rtol=inner_rtol)
+        return self.A.solve(self.b, self.x, rtol=rtol, maxit=100000, precision=precision, inner_rtol=inner_rtol,
+                            inner_operator=inner_operator)
 
 
-def time_form(prob, precision, rtol, steps, warmup, inner_rtol):
-    ctx = prob.ctx
+# solver forms: (precision, operator of the FP32 inner iterations)
+FORMS = {"fp64": ("FP64", "INNER_AUTO"), "mixed": ("FP32", "INNER_AUTO"),
+         "mixed_assembled": ("FP32", "INNER_ASSEMBLED")}
+
+
+def time_form(prob, form, rtol, steps, warmup, inner_rtol):
+    ctx, fa = prob.ctx, prob.fa
+    precision, op = (getattr(fa, name) for name in FORMS[form])
     for _ in range(warmup):
-        prob.step(precision, rtol, inner_rtol)
+        prob.step(precision, rtol, inner_rtol, op)
     ctx.synchronize()
     ctx.reset_timing()
     # events around every 16th launch of each timed kernel: the kernel averages
@@ -92,13 +99,14 @@ def time_form(prob, precision, rtol, steps, warmup, inner_rtol):
     t0 = time.perf_counter()
     for _ in range(steps):
         s = time.perf_counter()
-        st = prob.step(precision, rtol, inner_rtol)
+        st = prob.step(precision, rtol, inner_rtol, op)
         ctx.synchronize()
         lat.append(time.perf_counter() - s)
         its.append(st)
     total = time.perf_counter() - t0
     ctx.set_timing(False)
-    timers = {name: ctx.timing(name) for name in ("spmv_fp64", "spmv_fp32", "assemble_gather", "assemble_geo")}
+    timers = {name: ctx.timing(name)
+              for name in ("spmv_fp64", "spmv_fp32", "elemop_fp32", "assemble_gather", "assemble_geo")}
     return total, lat, its, timers
 
 
@@ -176,13 +184,14 @@ def affine_families(fa, ctx, orders, sizes):
     return fams
 
 
-def run_batch(fa, B, fams, ids, draws, chunk):
-    """Submit all draws at once, step until done; returns per-request latency (s) and wall time."""
+def run_batch(fa, B, fams, ids, draws, chunk, keep=0):
+    """Submit all draws at once, step until done; returns per-request latency (s), solve data,
+    wall time and the solutions of the first `keep` draws (by draw index)."""
     t0 = time.perf_counter()
     sub = {}
-    for key, mu in draws:
-        sub[B.submit(ids[key], mu)] = (key, t0)
-    lat, stats = {}, {}
+    for i, (key, mu) in enumerate(draws):
+        sub[B.submit(ids[key], mu)] = (key, t0, i)
+    lat, stats, sols = {}, {}, {}
     last_log = t0
     while len(lat) < len(sub):
         if time.perf_counter() - last_log > 10:
@@ -194,10 +203,12 @@ def run_batch(fa, B, fams, ids, draws, chunk):
             if not got:
                 break
             rid, sd = got
-            B.solution(rid, fams[sub[rid][0]].ndof)
+            xs = B.solution(rid, fams[sub[rid][0]].ndof)
             lat[rid] = time.perf_counter() - sub[rid][1]
             stats[rid] = sd
-    return lat, stats, time.perf_counter() - t0
+            if sub[rid][2] < keep:
+                sols[sub[rid][2]] = xs
+    return lat, stats, time.perf_counter() - t0, sols
 
 
 def bench_batched(fa, args):
@@ -214,9 +225,22 @@ def bench_batched(fa, args):
                  group_spmv=not args.no_group)
     ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
     run_batch(fa, B, fams, ids, draws[:min(8, len(draws))], args.chunk)      # warm-up
-    lat, stats, wall = run_batch(fa, B, fams, ids, draws, args.chunk)
+    keep = min(len(draws), 64)
+    lat, stats, wall, sols = run_batch(fa, B, fams, ids, draws, args.chunk, keep=keep)
     its = [s["its"] for s in stats.values()]
     ok = all(s["converged"] for s in stats.values())
+    log(f"batched mixed: {len(draws) / wall:.2f} solves/s, mean {np.mean(its):.0f} iterations")
+    del B
+    # the same engine and draws, plain FP64 Jacobi-CG (like for like: batching without the precision change)
+    B64 = fa.Batch(ctx, rtol=args.rtol, precision=fa.FP64, use_graph=True, group_spmv=False)
+    ids64 = {k: B64.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
+    run_batch(fa, B64, fams, ids64, draws[:min(8, len(draws))], args.chunk)
+    lat64, stats64, wall64, sols64 = run_batch(fa, B64, fams, ids64, draws, args.chunk, keep=keep)
+    ok64 = all(s["converged"] for s in stats64.values())
+    err_mixed = max(float(np.linalg.norm(sols[i] - sols64[i]) / np.linalg.norm(sols64[i])) for i in range(keep))
+    log(f"batched fp64: {len(draws) / wall64:.2f} solves/s; mixed vs fp64 max rel err {err_mixed:.2e} "
+        f"over the first {keep} requests")
+    del B64
     # plain FP64 GPU form on a sample: assemble A(mu) + Dirichlet + FP64 Jacobi-CG, one at a time
     sample = draws[:args.plain_sample]
     t1 = time.perf_counter()
@@ -239,6 +263,11 @@ def bench_batched(fa, args):
         "wall_s": round(wall, 3), "converged": ok, "iterations_mean": float(np.mean(its)),
         "latency_ms": {"p50": round(float(np.percentile(L, 50)) * 1e3, 2),
                        "p99": round(float(np.percentile(L, 99)) * 1e3, 2)},
+        "batched_fp64_solves_per_s": round(len(draws) / wall64, 3),
+        "batched_fp64_converged": ok64,
+        "speedup_vs_batched_fp64": round(wall64 / wall, 3),
+        "rel_err_mixed_vs_batched_fp64": err_mixed,
+        "rel_err_sample": keep,
         "plain_fp64_sequential_solves_per_s": round(len(draws) / plain_s, 3),
         "plain_fp64_sample": len(sample),
         "speedup_vs_plain_fp64": round(plain_s / wall, 3),
@@ -361,6 +390,55 @@ def bench_kernels(fa, args):
                       "nnz": prob.nnz, "dofs": prob.ndof, "elements": prob.ne}), flush=True)
 
 
+def bench_beam_kernels(fa, args):
+    """Short fixed run of the block SpMV for hardware-counter passes: a plain
+    1 GiB device copy (counter calibration), then on the P3 or P4 elasticity
+    beam (40x10x10 Kuhn cells, 24 000 tets) one assembly and 40-iteration FP64
+    and FP32 solves (3x3 block SELL SpMV kernels)."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    ctx = fa.Context(0)
+    k = args.order if args.order in (3, 4) else 4
+    coords, elems = fa.mesh_kuhn_box(40, 10, 10, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=args.seed)
+    ne = len(elems)
+    space = fa.Space.from_mesh(ctx, coords, elems, k, 3)
+    g = fa.Graph.from_space(space)
+    ndof = space.ndof
+    isb = fa.boundary_dofs(3, k, 3, coords, elems, space.elem2dof, ndof, sel=1)
+    A = fa.Matrix(g, fa.FP64_FP32)
+    young = ctx.vector(ne, fa.uniform(ne, 1.0, 100.0, args.seed + 1, log=True))
+    poisson = ctx.vector(ne, fa.uniform(ne, 0.2, 0.35, args.seed + 2))
+    bvec, x = ctx.vector(ndof), ctx.vector(ndof)
+    rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
+    A.assemble(fa.ELASTICITY, 2 * k, young=young, poisson=poisson, precision=fa.FP64_FP32)
+    fa.assemble_linear(space, bvec, fa.SRC_VECTOR, 2 * k, (0.0, 0.0, -1.0))
+    A.zero_rows(rows, None, bvec, mode=fa.ELIMINATION_SYMMETRIC)
+    ctx.synchronize()
+    its = 40
+    out = {}
+    for name, prec in (("fp64", fa.FP64), ("fp32", fa.FP32)):
+        x.zero()
+        ctx.synchronize()
+        t = time.perf_counter()
+        st = A.solve(bvec, x, rtol=1e-30, maxit=its, precision=prec, inner_rtol=1e-30, check_every=its)
+        ctx.synchronize()
+        out[name] = {"ms": round((time.perf_counter() - t) * 1e3, 2), "its": st["its"]}
+        log(f"beam P{k} {name}: {st}")
+    nnz = g.nnz
+    print(json.dumps({"kernels": f"copy 1 GiB x5; P{k} beam elasticity: one assembly, {its}-iteration FP64 and "
+                                 "FP32 solves", "dofs": ndof, "nnz": nnz, "block_slots": nnz // 9,
+                      "algorithmic_bytes": {"fp64": 8 * nnz + 4 * (nnz // 9) + 20 * ndof,
+                                            "fp32": 4 * nnz + 4 * (nnz // 9) + 12 * ndof},
+                      "solves": out}), flush=True)
+
+
 def bench_elastic(fa, args):
     """Short fixed run of the element-matrix kernels for hardware-counter
     passes: a plain 1 GiB device copy (counter calibration), then four FP64+FP32
@@ -421,6 +499,18 @@ def bench_serving(fa, args):
     sizes = [int(n) for n in args.sizes.split(",")]
     fams = affine_families(fa, ctx, orders, sizes)
     keys = list(fams)
+    # saturation throughput of the same mix: requests submitted at once (closed loop)
+    rng = np.random.default_rng(args.seed)
+    sat_draws = [(keys[rng.integers(len(keys))], np.exp(rng.uniform(np.log(0.1), np.log(10.0), 4)))
+                 for _ in range(min(args.requests, 128))]
+    B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol or 1e-2, precision=fa.FP32, use_graph=True,
+                 group_spmv=not args.no_group, hbm_budget=int(args.hbm_gib * 2**30), backfill=args.backfill)
+    ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
+    run_batch(fa, B, fams, ids, sat_draws[:4], args.chunk)
+    _, _, sat_wall, _ = run_batch(fa, B, fams, ids, sat_draws, args.chunk)
+    saturation = len(sat_draws) / sat_wall
+    log(f"saturation: {saturation:.2f} solves/s ({len(sat_draws)} requests submitted at once)")
+    del B
     out = []
     for rate in [float(r) for r in args.rates.split(",")]:
         rng = np.random.default_rng(int(rate) + args.seed)
@@ -429,8 +519,7 @@ def bench_serving(fa, args):
         draws = [(keys[rng.integers(len(keys))], np.exp(rng.uniform(np.log(0.1), np.log(10.0), 4)))
                  for _ in range(nreq)]
         B = fa.Batch(ctx, rtol=args.rtol, inner_rtol=args.inner_rtol or 1e-2, precision=fa.FP32, use_graph=True,
-                     group_spmv=not args.no_group,
-                     hbm_budget=int(args.hbm_gib * 2**30))
+                     group_spmv=not args.no_group, hbm_budget=int(args.hbm_gib * 2**30), backfill=args.backfill)
         ids = {k: B.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
         run_batch(fa, B, fams, ids, draws[:4], args.chunk)   # warm-up (graph capture, allocations)
         t0 = time.perf_counter()
@@ -457,13 +546,15 @@ def bench_serving(fa, args):
         L = np.array(lat)
         rec = {"rate_per_s": rate, "requests": nreq, "p50_ms": round(float(np.percentile(L, 50)) * 1e3, 2),
                "p99_ms": round(float(np.percentile(L, 99)) * 1e3, 2),
-               "sustained_solves_per_s": round(nreq / (last - arrivals[0]), 2)}
+               "sustained_solves_per_s": round(nreq / (last - arrivals[0]), 2),
+               "load": round(rate / saturation, 3)}
         log(f"serving {rec}")
         out.append(rec)
     print(json.dumps({"metric": "serving p50/p99 latency and sustained solves/s (one MI355X)", "n_gpus": 1,
                       "config": {"workload": f"Poisson arrivals, affine Poisson P{orders} Kuhn n in {sizes}, "
-                                             f"rtol {args.rtol:g}, chunk {args.chunk} CG iterations"},
-                      "rates": out}), flush=True)
+                                             f"rtol {args.rtol:g}, chunk {args.chunk} CG iterations, "
+                                             f"HBM budget {args.hbm_gib:g} GiB, backfill {args.backfill}"},
+                      "saturation_solves_per_s": round(saturation, 2), "rates": out}), flush=True)
 
 
 def main():
@@ -479,18 +570,20 @@ def main():
                     help="batched/serving: one SpMV per instance instead of one per family group")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="residual reduction between FP64 reliable updates of the FP32 solves (default 1e-2)")
-    ap.add_argument("--forms", default="fp64,mixed")
+    ap.add_argument("--forms", default="fp64,mixed,mixed_assembled")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam", "kernels", "elastic"])
+    ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam", "kernels", "elastic", "beamkernels"])
     ap.add_argument("--orders", default="1,2,3,4")
     ap.add_argument("--sizes", default="8,12,16,24,32")
     ap.add_argument("--problems", type=int, default=512)
     ap.add_argument("--plain-sample", type=int, default=32)
     ap.add_argument("--chunk", type=int, default=16)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--rates", default="50,200,1000,2000")
+    ap.add_argument("--rates", default="5,10,20,30,50,200,1000,2000")
+    ap.add_argument("--backfill", type=int, default=8,
+                    help="serving: steps a request that does not fit the HBM budget lets smaller ones pass it")
     ap.add_argument("--requests", type=int, default=200)
-    ap.add_argument("--seconds", type=float, default=4.0)
+    ap.add_argument("--seconds", type=float, default=8.0)
     ap.add_argument("--hbm-gib", type=float, default=64.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -506,6 +599,8 @@ def main():
         return bench_kernels(fa, args)
     if args.config == "elastic":
         return bench_elastic(fa, args)
+    if args.config == "beamkernels":
+        return bench_beam_kernels(fa, args)
     if args.config == "beam":
         if args.orders == "1,2,3,4":
             args.orders = "3,4"
@@ -526,8 +621,7 @@ def main():
 
     results = {}
     for form in args.forms.split(","):
-        prec = fa.FP64 if form == "fp64" else fa.FP32
-        total, lat, sts, timers = time_form(prob, prec, args.rtol, args.steps, args.warmup, args.inner_rtol or 1e-2)
+        total, lat, sts, timers = time_form(prob, form, args.rtol, args.steps, args.warmup, args.inner_rtol or 1e-2)
         x = prob.x.get()
         err = float(np.linalg.norm(x - xref) / np.linalg.norm(xref))
         results[form] = dict(total=total, lat=lat, st=sts[-1], timers=timers, err=err)
@@ -539,20 +633,41 @@ def main():
     R = results[best]
     sps = args.steps / R["total"]
     lat = np.array(R["lat"])
-    spmv_name = "spmv_fp64" if best == "fp64" else "spmv_fp32"
-    nl, ms = R["timers"][spmv_name]
     n, nnz = prob.ndof, prob.nnz
-    if spmv_name == "spmv_fp64":
-        bytes_spmv = 12 * nnz + 20 * n + 4
-    else:
-        bytes_spmv = 8 * nnz + 12 * n + 4
-    avg_ms = ms / max(nl, 1)
-    achieved = bytes_spmv / (avg_ms * 1e-3) / 1e9 if nl else 0.0
-    spmv_kernel = "kSellSpmv<double,1024,2,true>" if spmv_name == "spmv_fp64" else "kSellSpmv<float,1024,2,true>"
     # HBM bytes per launch from the committed counter passes (calibrated FETCH_SIZE + WRITE_SIZE),
     # valid for the workload they were taken on
     traffic = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_p2_kuhn64.txt"))
     measured_workload = args.n == 64 and order == 2
+
+    def spmv_roofline(form, timer):
+        nl, ms = results[form]["timers"][timer]
+        if timer == "spmv_fp64":
+            kernel, nbytes = "kSellSpmv<double,1024,2,true>", 12 * nnz + 20 * n + 4
+        elif timer == "spmv_fp32":
+            kernel, nbytes = "kSellSpmv<float,1024,2,true>", 8 * nnz + 12 * n + 4
+        else:
+            # element operator: pass 1 streams 6 factors + 10 dof ids per element, writes 10 element
+            # results, gathers p once; pass 2 reads one index and one element result per incidence,
+            # p, the Dirichlet flag and writes A p per row
+            ne, nloc = prob.ne, 10
+            kernel = "kElemApplyP2+kElemGather<256>"
+            nbytes = ne * (6 * 4 + nloc * 4 + nloc * 4) + 4 * n + ne * nloc * 8 + n * (4 + 1 + 4) + 4 * (n // 64 + 1)
+        avg = ms / max(nl, 1)
+        ach = nbytes / (avg * 1e-3) / 1e9 if nl else 0.0
+        return {"bound": "hbm", "kernel": kernel, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic.get(kernel) if measured_workload else None,
+                "avg_launch_us": round(avg * 1e3, 2), "launches": nl, "algorithmic_bytes": nbytes}
+
+    if best == "fp64":
+        roof = spmv_roofline(best, "spmv_fp64")
+    elif R["timers"]["elemop_fp32"][0] > 0:
+        roof = spmv_roofline(best, "elemop_fp32")
+    else:
+        roof = spmv_roofline(best, "spmv_fp32")
+    others = {}
+    for f, timer in (("mixed_assembled", "spmv_fp32"), ("fp64", "spmv_fp64")):
+        if f in results and f != best and results[f]["timers"][timer][0] > 0:
+            others[f] = spmv_roofline(f, timer)
     # scatter (owner-computes gather of element contributions into CSR values, FP64 + FP32 copies):
     # algorithmic bytes = nnz*(8+4) values + E*nloc*4 elem2dof + E*8 coefficient + E*4*4 + V*3*8 geometry
     ng, gms = R["timers"]["assemble_gather"]
@@ -601,19 +716,7 @@ def main():
         "forms": {f: {"solves_per_s": round(args.steps / v["total"], 4), "its": v["st"]["its"],
                       "rel_err_vs_fp64": v["err"], "p50_ms": round(float(np.median(v["lat"])) * 1e3, 3)}
                   for f, v in results.items()},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": spmv_kernel,
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic.get(spmv_kernel) if measured_workload else None,
-            "avg_launch_us": round(avg_ms * 1e3, 2),
-            "launches": nl,
-            "algorithmic_bytes": bytes_spmv,
-            "scatter": scatter,
-        },
+        "roofline": dict(roof, scatter=scatter, other_forms=others),
         "cpu_baseline": cpu and {k: v for k, v in cpu.items()
                                  if k in ("value", "unit", "cores", "kind", "sample", "one_core")},
     }

@@ -25,6 +25,7 @@ MASS, DIFFUSION, ELASTICITY = 1, 2, 4
 MATRICES_AUTO, MATRICES_FUSED, MATRICES_MFMA = 0, 1, 2
 SRC_CONST, SRC_SINSIN, SRC_VECTOR = 0, 1, 2
 ELIMINATION, ELIMINATION_SYMMETRIC = 0, 1
+INNER_AUTO, INNER_ASSEMBLED, INNER_ELEMENT = 0, 1, 2
 SAME_NONZERO_PATTERN, DIFFERENT_NONZERO_PATTERN, SUBSET_NONZERO_PATTERN = 0, 1, 3
 NORM_1, NORM_FROBENIUS, NORM_INF = 0, 2, 3
 VEC_SUM, VEC_L1, VEC_L2, VEC_MIN, VEC_MAX, VEC_LINF = 0, 1, 2, 3, 4, 5
@@ -47,7 +48,7 @@ class LinearDesc(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double), ("maxit", C.c_int),
                 ("precision", C.c_int), ("inner_rtol", C.c_double), ("inner_maxit", C.c_int),
-                ("max_outer", C.c_int), ("check_every", C.c_int)]
+                ("max_outer", C.c_int), ("check_every", C.c_int), ("inner_operator", C.c_int)]
 
 
 class SolveData(C.Structure):
@@ -61,7 +62,7 @@ class SolveData(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("rtol", C.c_double), ("atol", C.c_double), ("inner_rtol", C.c_double), ("maxit", C.c_int),
                 ("inner_maxit", C.c_int), ("max_outer", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64),
-                ("use_graph", C.c_int), ("group_spmv", C.c_int)]
+                ("use_graph", C.c_int), ("group_spmv", C.c_int), ("backfill", C.c_int)]
 
 
 # exported symbols of the two C-ABIs: name -> (restype, argtypes)
@@ -152,6 +153,10 @@ _SIGS = {
     "feel_hip_mat_zero_rows": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int, C.c_double]),
     "feel_hip_solve_opts_default": (None, [C.POINTER(SolveOpts)]),
     "feel_hip_solve_pcg": (C.c_int, [_vp, _vp, _vp, C.POINTER(SolveOpts), C.POINTER(SolveData)]),
+    "feel_hip_graph_spmv_shapes": (C.c_int, []),
+    "feel_hip_graph_set_spmv_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "feel_hip_graph_spmv_shape": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int)]),
     "feel_hip_norm_l2": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
     "feel_hip_norm_semi_h1": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
     "feel_hip_batch_opts_default": (None, [C.POINTER(BatchOpts)]),
@@ -540,6 +545,16 @@ class Graph(_Handle):
         self.space, self.ia, self.ja, self.nnz = space, ia, ja, nnz
         return self
 
+    def spmv_shape(self, precision):
+        """(shape index, block, grid) of the SpMV launch for FP64 or FP32 solves; -1 until tuned."""
+        sh, bl, gr = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().feel_hip_graph_spmv_shape(self.h, precision, C.byref(sh), C.byref(bl), C.byref(gr)))
+        return sh.value, bl.value, gr.value
+
+    def set_spmv_shape(self, precision, shape):
+        """Pins the SpMV launch shape (bitwise reproducible solves across processes); -1 re-tunes."""
+        _check(lib().feel_hip_graph_set_spmv_shape(self.h, precision, shape))
+
 
 class Matrix(_Handle):
     _destroy = "feel_hip_mat_destroy"
@@ -628,8 +643,9 @@ class Matrix(_Handle):
                                             mode, diag))
 
     def solve(self, b, x, rtol=1e-8, atol=1e-50, dtol=1e5, maxit=1000, precision=FP64, inner_rtol=1e-2,
-              inner_maxit=100000, max_outer=50, check_every=0):
-        o = SolveOpts(rtol, atol, dtol, maxit, precision, inner_rtol, inner_maxit, max_outer, check_every)
+              inner_maxit=100000, max_outer=50, check_every=0, inner_operator=0):
+        o = SolveOpts(rtol, atol, dtol, maxit, precision, inner_rtol, inner_maxit, max_outer, check_every,
+                      inner_operator)
         out = SolveData()
         _check(lib().feel_hip_solve_pcg(self.h, b.h, x.h, C.byref(o), C.byref(out)))
         return out.as_dict()
@@ -689,14 +705,14 @@ class Batch(_Handle):
     _destroy = "feel_hip_batch_destroy"
 
     def __init__(self, ctx, rtol=1e-8, inner_rtol=1e-2, precision=FP32, hbm_budget=0, use_graph=True,
-                 maxit=100000, max_outer=50, group_spmv=None):
+                 maxit=100000, max_outer=50, group_spmv=None, backfill=0):
         o = BatchOpts()
         lib().feel_hip_batch_opts_default(C.byref(o))
         o.rtol, o.inner_rtol, o.precision, o.hbm_budget, o.use_graph = rtol, inner_rtol, precision, hbm_budget, int(
             use_graph)
         if group_spmv is not None:
             o.group_spmv = int(group_spmv)
-        o.maxit, o.max_outer = maxit, max_outer
+        o.maxit, o.max_outer, o.backfill = maxit, max_outer, backfill
         h = _vp()
         _check(lib().feel_hip_batch_create(ctx.h, C.byref(o), C.byref(h)))
         super().__init__(h, keep=(ctx,))

@@ -728,6 +728,7 @@ struct Request {
     double devRnorm0 = 0;  // the device's initial residual norm (FP32 units), for the dtol test
     bool unpermuted = false;   // solution copied to Feel++ numbering
     int pgroup = -1, pslot = -1;   // family group slot of the grouped FP32 SpMV
+    int waited = 0;        // steps spent at the head of the queue without fitting (backfill)
 };
 
 struct feel_hip_batch {
@@ -1072,8 +1073,9 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
 // when every slot of the family's groups is taken)
 bool hasFreeSlot(const BatchFamily& f) {
     for (const auto& pg : f.pgroups)
-        for (int b = 0; b < kGroupMax; ++b)
-            if (pg.req[b] < 0) return true;
+        if (pg.P)
+            for (int b = 0; b < kGroupMax; ++b)
+                if (pg.req[b] < 0) return true;
     return false;
 }
 
@@ -1082,7 +1084,7 @@ size_t groupBytes(const BatchFamily& f) { return sizeof(float) * kGroupMax * (si
 int assignSlot(feel_hip_batch* B, Request& r) {
     BatchFamily& f = B->fams[r.family];
     for (size_t g = 0; g < f.pgroups.size(); ++g)
-        for (int b = 0; b < kGroupMax; ++b)
+        for (int b = 0; f.pgroups[g].P && b < kGroupMax; ++b)
             if (f.pgroups[g].req[b] < 0) {
                 f.pgroups[g].req[b] = r.id;
                 r.pgroup = (int)g;
@@ -1121,22 +1123,74 @@ void permuteOut(feel_hip_batch* B, Request& r) {
     r.unpermuted = true;
 }
 
+// Frees cached memory no request holds -- instance buffer sets kept from
+// finished requests, family groups whose eight slots are all free -- of the
+// families other than `keep` (whose cached memory the request being admitted
+// counts on) until `need` more bytes fit the HBM budget.  Released groups stay
+// in their family's list (request slot indices refer to it) with no storage.
+// Returns whether the bytes fit.
+bool releaseIdle(feel_hip_batch* B, int64_t need, int keep) {
+    feel_hip_ctx* c = B->ctx;
+    auto fits = [&] { return B->used + need <= B->opts.hbm_budget; };
+    bool freed = false;
+    for (size_t fi = 0; fi < B->fams.size(); ++fi) {
+        if ((int)fi == keep) continue;
+        BatchFamily& f = B->fams[fi];
+        while (!fits() && !f.freeBufs.empty()) {
+            InstBuffers& ib = B->bufs[f.freeBufs.back()];
+            f.freeBufs.pop_back();
+            for (double* q : {ib.val64, ib.dinv64, ib.x64, ib.r64, ib.w64, ib.out, ib.part}) if (q) (void)hipFree(q);
+            for (void* q : {ib.valT, ib.dinvT, ib.xT, ib.rT, ib.zT, ib.pT, ib.wT, ib.bT}) if (q) (void)hipFree(q);
+            ib = InstBuffers{};
+            const int64_t bytes = (int64_t)instanceBytes(B, f);
+            B->used -= bytes;
+            c->bytes -= bytes;
+            freed = true;
+        }
+        for (auto& pg : f.pgroups) {
+            if (fits()) break;
+            bool idle = pg.P != nullptr;
+            for (int b = 0; idle && b < kGroupMax; ++b) idle = pg.req[b] < 0;
+            if (!idle) continue;
+            (void)hipFree(pg.P);
+            pg.P = nullptr;
+            B->used -= (int64_t)groupBytes(f);
+            c->bytes -= (int64_t)groupBytes(f);
+            freed = true;
+        }
+    }
+    if (freed) ++B->version;
+    return fits();
+}
+
 template <typename T>
 int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
     feel_hip_ctx* c = B->ctx;
     int nItems = 0;
-    // 1. admission by HBM footprint, FIFO
+    // 1. admission by HBM footprint in submission order.  With backfill > 0 a
+    //    request that does not fit lets later ones that fit pass it, until it
+    //    has waited `backfill` steps; then nothing passes it until it fits.
     std::vector<Request*> admitted;
-    while (!B->pending.empty()) {
-        Request& r = B->reqs[B->pending.front()];
+    bool blocked = false;
+    for (auto it = B->pending.begin(); it != B->pending.end();) {
+        Request& r = B->reqs[*it];
         BatchFamily& f = B->fams[r.family];
         const bool reuse = !f.freeBufs.empty();
         const bool grouped = sizeof(T) == 4 && B->opts.group_spmv;
         const int64_t need = (reuse ? 0 : (int64_t)instanceBytes(B, f)) +
                              (grouped && !hasFreeSlot(f) ? (int64_t)groupBytes(f) : 0);
-        if (need > 0 && B->opts.hbm_budget > 0 && B->used + need > B->opts.hbm_budget) {
+        if (need > 0 && B->opts.hbm_budget > 0 && B->used + need > B->opts.hbm_budget &&
+            !releaseIdle(B, need, r.family)) {
             if (B->used == 0) return setError(FEEL_HIP_ENOMEM, "one instance exceeds the batch HBM budget");
-            break;
+            if (B->opts.backfill <= 0 || blocked) {
+                if (B->opts.backfill <= 0) break;
+                ++it;
+                continue;
+            }
+            blocked = true;
+            if (++r.waited > B->opts.backfill) break;
+            ++it;
+            continue;
         }
         TRY_CALL(ensureStates(B, B->nStates - (int)B->freeStates.size() + 1));
         TRY_CALL(allocInstance(B, r.family, &r.buf));
@@ -1144,7 +1198,7 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         r.state = B->freeStates.back();
         B->freeStates.pop_back();
         admitted.push_back(&r);
-        B->pending.pop_front();
+        it = B->pending.erase(it);
     }
     // 2. assembly of admitted instances: A(mu), D^{-1}; x = 0 so r = b
     if (!admitted.empty()) {
@@ -1282,6 +1336,7 @@ void feel_hip_batch_opts_default(feel_hip_batch_opts* o) {
     o->hbm_budget = 0;
     o->use_graph = 1;
     o->group_spmv = 1;
+    o->backfill = 0;
 }
 
 int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out) {

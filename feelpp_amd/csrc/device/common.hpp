@@ -54,6 +54,14 @@ int buildIncidenceDevice(feel_hip_space* s);
 int buildSellDevice(feel_hip_graph* g, const std::vector<int32_t>& perm);
 // the 3 x 3 block image of a vector-space graph for the node order nperm (sell.hip)
 int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm);
+// element-by-element FP32 operator of P2 diffusion matrices (elemop.hip)
+struct ElemOp;
+int elemOpAssembled(feel_hip_mat* A, bool qualifies);
+int elemOpZeroRows(feel_hip_mat* A, const feel_hip_rows* rows, int mode, double diag);
+void elemOpInvalidate(feel_hip_mat* A);
+bool elemOpReady(const feel_hip_mat* A);
+void elemOpFree(feel_hip_mat* A);
+int elemOpApply(feel_hip_mat* A, const float* p, float* w, double* part, const CgState* st, int* npart);
 void retain(feel_hip_ctx* c);
 void retain(feel_hip_space* s);
 void retain(feel_hip_graph* g);
@@ -186,6 +194,10 @@ struct feel_hip_mat {
     double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *w64 = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *w32 = nullptr, *x32 = nullptr, *b32 = nullptr;
     double *bp64 = nullptr, *xp64 = nullptr;   // b and x in the solver numbering
+    // element-by-element FP32 operator: 1 while the values are a P2 diffusion
+    // assembly followed only by symmetric unit-diagonal eliminations
+    int mfState = 0;
+    feelpp_amd::ElemOp* eop = nullptr;
 };
 
 struct feel_hip_vec {

@@ -664,6 +664,7 @@ int feel_hip_mat_destroy(feel_hip_mat* A) {
     for (double* v : {A->bp64, A->xp64}) c->release(v, sizeof(double) * n);
     c->release(A->s64, sizeof(double) * A->g->nslots);
     c->release(A->s32, sizeof(float) * A->g->nslots);
+    elemOpFree(A);
     feel_hip_graph* g = A->g;
     delete A;
     unref(g);
@@ -677,6 +678,7 @@ int feel_hip_mat_zero(feel_hip_mat* A) {
     if (A->v32) FEEL_HIP_CHECK(hipMemsetAsync(A->v32, 0, sizeof(float) * A->g->nnz, c->stream));
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;
+    elemOpInvalidate(A);
     return FEEL_HIP_OK;
 }
 
@@ -711,6 +713,7 @@ int feel_hip_mat_add_matrix(feel_hip_mat* A, int nrows, const int32_t* rows, int
     c->release(dval, sizeof(double) * m);
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;
+    elemOpInvalidate(A);
     return rc;
 }
 
@@ -746,6 +749,7 @@ int feel_hip_mat_set_values(feel_hip_mat* A, const double* host) {
     FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;
+    elemOpInvalidate(A);
     return FEEL_HIP_OK;
 }
 
@@ -982,6 +986,7 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     double* v64 = (wantPrec & FEEL_HIP_FP64) ? A->v64 : nullptr;
     float* v32 = (wantPrec & FEEL_HIP_FP32) ? A->v32 : nullptr;
     if (!v64 && !v32) return setError(FEEL_HIP_EINVAL, "matrix holds none of the requested precisions");
+    elemOpInvalidate(A);
     if (d->kind & FEEL_HIP_EXPR_ELASTICITY) {
         if (d->kind != FEEL_HIP_EXPR_ELASTICITY)
             return setError(FEEL_HIP_EUNSUPPORTED, "elasticity cannot be combined with other terms in one call");
@@ -1032,6 +1037,7 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
         FEEL_HIP_CHECK(hipGetLastError());
         A->dinvValid = false;
         A->sell64Valid = A->sell32Valid = false;
+    elemOpInvalidate(A);
         return FEEL_HIP_OK;
     }
     if ((d->kind & ~(FEEL_HIP_EXPR_MASS | FEEL_HIP_EXPR_DIFFUSION)) || d->kind == 0)
@@ -1103,6 +1109,8 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     FEEL_HIP_CHECK(hipGetLastError());
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;
+    // diffusion alone, operator=, a rule exact for the degree-2 integrand of P2
+    TRY_CALL(elemOpAssembled(A, d->kind == FEEL_HIP_EXPR_DIFFUSION && !add && d->quad_order >= 2 && v64));
     return FEEL_HIP_OK;
     GUARD_END
 }
@@ -1185,6 +1193,7 @@ int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const fee
     FEEL_HIP_CHECK(hipGetLastError());
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;
+    TRY_CALL(elemOpZeroRows(A, rows, mode, diag));
     return FEEL_HIP_OK;
 }
 

@@ -305,6 +305,7 @@ feel_hip_ctx* ctxOf(const feel_hip_mat* A) { return A->g->space->ctx; }
 void invalidate(feel_hip_mat* A) {
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;
+    elemOpInvalidate(A);
 }
 
 // sum (or max) of the first count block partials in c->part, in block order

@@ -263,7 +263,7 @@ constexpr int kKappaPollIts = 1024;
 // this factor restarts the search direction (the FP32 recurrence drifted).
 constexpr double kDriftRestart = 2.0;
 
-int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, MixedRun& out) {
+int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, MixedRun& out) {
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
     const int64_t n = g->n;
@@ -326,9 +326,14 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, MixedRun& out) {
         for (int k = 0; k < chunk && it <= o.maxit; ++k, ++it) {
             hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->z32,
                                A->p32, zrPart, zzPart, st, betaSlots, c->hist);
-            c->tic("spmv_fp32");
             int npart = 0;
-            TRY_CALL(spmv<float>(c, g, A->s32, A->p32, A->w32, dpiPart, st, &npart));
+            if (elemOp) {
+                c->tic("elemop_fp32");
+                TRY_CALL(elemOpApply(A, A->p32, A->w32, dpiPart, st, &npart));
+            } else {
+                c->tic("spmv_fp32");
+                TRY_CALL(spmv<float>(c, g, A->s32, A->p32, A->w32, dpiPart, st, &npart));
+            }
             c->toc();
             hipLaunchKernelGGL(kCgUpdate<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->x32,
                                A->r32, A->z32, A->p32, A->w32, A->dinv32, dpiPart, npart, zrPart, zzPart, st,
@@ -413,6 +418,26 @@ void feel_hip_solve_opts_default(feel_hip_solve_opts* o) {
     o->inner_maxit = 100000;
     o->max_outer = 50;
     o->check_every = 0;
+    o->inner_operator = FEEL_HIP_INNER_AUTO;
+}
+
+int feel_hip_graph_spmv_shapes(void) { return kNumSpmvShapes; }
+
+int feel_hip_graph_set_spmv_shape(feel_hip_graph* g, int precision, int shape) {
+    if (!g) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (precision != FEEL_HIP_FP64 && precision != FEEL_HIP_FP32) return setError(FEEL_HIP_EINVAL, "bad precision");
+    if (shape < -1 || shape >= kNumSpmvShapes) return setError(FEEL_HIP_EINVAL, "bad SpMV launch shape");
+    g->spmvShape[precision == FEEL_HIP_FP64 ? 0 : 1] = shape;
+    return FEEL_HIP_OK;
+}
+
+int feel_hip_graph_spmv_shape(const feel_hip_graph* g, int precision, int* shape, int* block, int* grid) {
+    if (!g || !shape) return setError(FEEL_HIP_EINVAL, "null argument");
+    if (precision != FEEL_HIP_FP64 && precision != FEEL_HIP_FP32) return setError(FEEL_HIP_EINVAL, "bad precision");
+    *shape = g->spmvShape[precision == FEEL_HIP_FP64 ? 0 : 1];
+    if (block) *block = *shape >= 0 ? kSpmvShapes[*shape].block : 0;
+    if (grid) *grid = *shape >= 0 ? kSpmvShapes[*shape].grid : 0;
+    return FEEL_HIP_OK;
 }
 
 int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, const feel_hip_solve_opts* opts,
@@ -465,11 +490,18 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
     TRY_CALL(allocWork(c, &A->x32, n));
     TRY_CALL(allocWork(c, &A->bp64, n));
     TRY_CALL(allocWork(c, &A->xp64, n));
-    TRY_CALL(ensureSell(A, true, true));
+    if (o.inner_operator < FEEL_HIP_INNER_AUTO || o.inner_operator > FEEL_HIP_INNER_ELEMENT)
+        return setError(FEEL_HIP_EINVAL, "unknown inner operator");
+    if (o.inner_operator == FEEL_HIP_INNER_ELEMENT && !elemOpReady(A))
+        return setError(FEEL_HIP_EUNSUPPORTED,
+                        "element-by-element inner operator needs a P2 diffusion assembly (symmetric on() only)");
+    const bool elemOp = o.inner_operator != FEEL_HIP_INNER_ASSEMBLED && elemOpReady(A);
+    // the element operator replaces the FP32 SELL values of the inner iterations
+    TRY_CALL(ensureSell(A, true, !elemOp));
     hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, b->d, A->bp64);
     hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, x->d, A->xp64);
     MixedRun run;
-    TRY_CALL(runMixedCg(A, o, run));
+    TRY_CALL(runMixedCg(A, o, elemOp, run));
     hipLaunchKernelGGL(kPermuteOut, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, A->xp64, x->d);
     FEEL_HIP_CHECK(hipGetLastError());
     c->collectTimers();

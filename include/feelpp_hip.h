@@ -98,7 +98,17 @@ typedef struct {
     int inner_maxit;    /* unused by the single solve */
     int max_outer;      /* FP32: cap on reliable updates */
     int check_every;    /* host polls the device convergence flag every this many iterations (0 = auto) */
+    int inner_operator; /* FP32: operator of the inner iterations, FEEL_HIP_INNER_* */
 } feel_hip_solve_opts;
+
+/* Operator of the FP32 inner iterations.  ASSEMBLED: the matrix's FP32 values
+ * (rounded once from the FP64 assembly).  ELEMENT: element by element from the
+ * per-element factors k J K^{-1} K^{-T} in FP32, available while the matrix
+ * holds a P2 tetrahedral diffusion assembly (operator=, quadrature order >= 2)
+ * followed only by symmetric unit-diagonal eliminations; FP64 residuals still
+ * use the assembled FP64 values.  AUTO: ELEMENT when available, else ASSEMBLED.
+ * No counterpart in the reference (PETSc applies the assembled matrix). */
+enum { FEEL_HIP_INNER_AUTO = 0, FEEL_HIP_INNER_ASSEMBLED = 1, FEEL_HIP_INNER_ELEMENT = 2 };
 
 typedef struct {
     int converged;      /* SolveData::isConverged */
@@ -276,6 +286,17 @@ int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const fee
  * Backend::solve -> SolverLinearPetsc::solve (feelalg/solverlinearpetsc.cpp:575-640)
  * with ksp-type=cg, pc-type=jacobi.  x holds the initial guess on entry. */
 void feel_hip_solve_opts_default(feel_hip_solve_opts* o);
+/* SpMV launch shape (block size, grid) of a graph, per precision.  The shape
+ * decides how rows map to wavefronts and how many p.Ap block partials the CG
+ * dots sum, so solutions are bitwise reproducible for a given shape.  By default
+ * (-1) the first solve times every candidate once and keeps the fastest, which
+ * can differ between processes; a caller that needs bitwise reproducibility
+ * across processes pins a shape (0 .. feel_hip_graph_spmv_shapes()-1, 0 = the
+ * default (1024, 2048)), or reads the tuned one and pins it in the next run.
+ * No counterpart in the reference (PETSc's MatMult has no launch shape). */
+int feel_hip_graph_spmv_shapes(void);
+int feel_hip_graph_set_spmv_shape(feel_hip_graph* g, int precision, int shape);
+int feel_hip_graph_spmv_shape(const feel_hip_graph* g, int precision, int* shape, int* block, int* grid);
 int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, const feel_hip_solve_opts* opts,
                        feel_hip_solve_data* out);
 
@@ -305,6 +326,8 @@ typedef struct {
     int64_t hbm_budget;       /* bytes of instance buffers allowed at once, 0 = unlimited */
     int use_graph;            /* capture each iteration chunk in a hipGraph */
     int group_spmv;           /* FP32: one SpMV pass per family group, A(mu) formed on the fly */
+    int backfill;             /* 0: strict submission order; N > 0: requests that fit the budget may pass
+                                 one that does not, until it has waited N steps */
 } feel_hip_batch_opts;
 void feel_hip_batch_opts_default(feel_hip_batch_opts* o);
 int feel_hip_batch_create(feel_hip_ctx* ctx, const feel_hip_batch_opts* opts, feel_hip_batch** out);

@@ -306,3 +306,59 @@ def test_groups_of_mixed_term_counts(ctx, group):
     r = A.submit(a2, reqs[rid2][::2])
     alone = run_all(A, {r: F["ndof"]}, chunk=9)[r][1]
     assert np.array_equal(alone, done[rid2][1])
+
+
+def test_backfill_admission(ctx):
+    # A budget for one large instance and a few small ones: in strict order the
+    # small requests wait behind the second large one; with backfill they pass
+    # it, until it has waited `backfill` steps.  Results do not depend on the
+    # admission order.
+    big, small = family(ctx, 6, 2), family(ctx, 2, 1)
+
+    def one_instance(F):
+        P = fa.Batch(ctx, rtol=1e-9, group_spmv=False)
+        f = P.add_family(F["terms"], F["b"], F["rows"])
+        base = P.device_bytes()
+        P.submit(f, np.ones(4))
+        P.step(0)
+        return P.device_bytes() - base
+
+    Bb, Bs = one_instance(big), one_instance(small)
+    assert Bb > 4 * Bs
+    budget = Bb + 2 * Bs + Bs // 2
+    mus = [np.full(4, 1.0 + k) for k in range(4)]
+
+    def batch(backfill):
+        B = fa.Batch(ctx, rtol=1e-9, group_spmv=False, hbm_budget=budget, backfill=backfill)
+        fb = B.add_family(big["terms"], big["b"], big["rows"])
+        fs = B.add_family(small["terms"], small["b"], small["rows"])
+        return B, fb, fs
+
+    results = {}
+    for backfill in (0, 2):
+        B, fb, fs = batch(backfill)
+        base = B.device_bytes()
+        ids = [B.submit(fb, mus[0]), B.submit(fb, mus[1]), B.submit(fs, mus[2]), B.submit(fs, mus[3])]
+        B.step(0)
+        held = B.device_bytes() - base
+        assert held == (Bb if backfill == 0 else Bb + 2 * Bs), (backfill, held)
+        ndofs = dict(zip(ids, [big["ndof"]] * 2 + [small["ndof"]] * 2))
+        done = run_all(B, ndofs)
+        assert all(d[0]["converged"] for d in done.values())
+        results[backfill] = [done[i][1] for i in ids]
+    for a, b in zip(results[0], results[2]):
+        assert np.array_equal(a, b)
+    # no starvation: after `backfill` steps nothing passes the waiting request
+    B, fb, fs = batch(1)
+    base = B.device_bytes()
+    ids = [B.submit(fb, mus[0]), B.submit(fb, mus[1]), B.submit(fs, mus[2])]
+    B.step(0)
+    assert B.device_bytes() - base == Bb + Bs
+    ids.append(B.submit(fs, mus[3]))
+    B.step(0)
+    assert B.device_bytes() - base == Bb + Bs
+    ndofs = dict(zip(ids, [big["ndof"]] * 2 + [small["ndof"]] * 2))
+    done = run_all(B, ndofs)
+    assert set(done) == set(ids) and all(d[0]["converged"] for d in done.values())
+    for i, x in zip(ids, results[0]):
+        assert np.array_equal(done[i][1], x)

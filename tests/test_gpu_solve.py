@@ -139,6 +139,32 @@ def test_solve_deterministic(ctx, orc):
     assert s1 == s2 and np.array_equal(x1.get(), x2.get())
 
 
+@pytest.mark.parametrize("precision", [fa.FP64, fa.FP32])
+def test_pinned_spmv_shape_reproducible(ctx, orc, precision):
+    # A pinned launch shape fixes the p.Ap partial layout: two independently built
+    # problems (as two processes would build them) solve bit for bit alike, and
+    # every shape converges to the same solution within the solver's tolerance.
+    P = problem(ctx, orc, 3, 2, 6, 4, orc.FK_CONST, 2, 0.15, 5)
+    Q = problem(ctx, orc, 3, 2, 6, 4, orc.FK_CONST, 2, 0.15, 5)
+    gP, gQ = P["keep"][1], Q["keep"][1]
+    assert gP.spmv_shape(precision)[0] == -1
+    xo, _, _, _ = orc.pcg(P["ia"], P["ja"], P["val"], P["b"], rtol=1e-13, maxit=10000)
+    nshapes = fa.lib().feel_hip_graph_spmv_shapes()
+    assert nshapes >= 2
+    for shape in range(nshapes):
+        for prec in (fa.FP64, fa.FP32):   # the mixed solve's reliable updates run the FP64 SpMV
+            gP.set_spmv_shape(prec, shape)
+            gQ.set_spmv_shape(prec, shape)
+        assert gP.spmv_shape(precision)[0] == shape
+        xp, xq = ctx.vector(P["ndof"]), ctx.vector(Q["ndof"])
+        sp_ = P["A"].solve(P["bb"], xp, rtol=1e-10, maxit=5000, precision=precision)
+        sq_ = Q["A"].solve(Q["bb"], xq, rtol=1e-10, maxit=5000, precision=precision)
+        assert sp_ == sq_ and np.array_equal(xp.get(), xq.get()), shape
+        assert sp_["converged"] and rel_fro(xp.get(), xo) < 1e-8, (shape, sp_)
+    with pytest.raises(fa.FeelHipError):
+        gP.set_spmv_shape(precision, nshapes)
+
+
 @pytest.mark.parametrize("order", [2, 3, 4])
 def test_clamped_beam_elasticity(ctx, orc, order):
     # left face clamped, body force (0,0,-1); FP64 and mixed solves vs the oracle
@@ -232,9 +258,98 @@ def test_full_size_beam_p4_properties(ctx):
     for prec in (fa.FP64, fa.FP32):
         x = ctx.vector(ndof)
         st = A.solve(b, x, rtol=1e-10, maxit=200000, precision=prec)
-        assert st["converged"], st
+        assert st["converged"] and st["residual"] <= 1e-10 * st["rnorm0"], st
         A.mult(x, y)
-        res = np.linalg.norm(dinv * (bh - y.get()))
-        assert res <= 1e-10 * np.linalg.norm(dinv * bh) * 1.01, (prec, st)
+        res = np.linalg.norm(dinv * (bh - y.get())) / np.linalg.norm(dinv * bh)
+        # Plain CG, like KSPCG, stops on its recursively updated residual, which
+        # drifts from the true one by about eps * kappa per step (kappa ~ 1e6
+        # here, ~8000 steps); the mixed form hands this clamped beam over to FP64
+        # CG after its first condition estimate, so both are held to that
+        # attainable level of the true residual.
+        assert res <= 1e-8, (prec, res, st)
         sols[prec] = x.get()
-    assert rel_fro(sols[fa.FP32], sols[fa.FP64]) < 1e-8
+    # two solutions within kappa * (true residuals) of each other
+    d = rel_fro(sols[fa.FP32], sols[fa.FP64])
+    print("mixed vs FP64 solution", d)
+    assert d < 1e-5
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_element_operator_inner_iterations(ctx, orc, seed):
+    # P2 diffusion with lognormal k and symmetric elimination: the mixed solve
+    # with the element-by-element FP32 inner operator and with the assembled
+    # FP32 matrix both meet the FP64 stopping test and match the oracle to 1e-8
+    P = problem(ctx, orc, 3, 2, 7, 4, orc.FK_CONST, 2, 0.15, seed)
+    xt, its_o, _, _ = orc.pcg(P["ia"], P["ja"], P["val"], P["b"], rtol=1e-14, maxit=10000)
+    out = {}
+    for op in (fa.INNER_ELEMENT, fa.INNER_ASSEMBLED, fa.INNER_AUTO):
+        x = ctx.vector(P["ndof"])
+        st = P["A"].solve(P["bb"], x, rtol=1e-10, maxit=20000, precision=fa.FP32, inner_operator=op)
+        assert st["converged"] and st["residual"] <= 1e-10 * st["rnorm0"], (op, st)
+        assert rel_fro(x.get(), xt) < 1e-8, op
+        assert st["its"] <= 1.5 * its_o, (op, st, its_o)
+        out[op] = (st, x.get())
+    # AUTO takes the element operator for this matrix
+    assert out[fa.INNER_AUTO][0] == out[fa.INNER_ELEMENT][0]
+    assert np.array_equal(out[fa.INNER_AUTO][1], out[fa.INNER_ELEMENT][1])
+
+
+def test_element_operator_nonzero_dirichlet(ctx, orc):
+    # u = 1 + x - 2y + 3z on the boundary, k = 1, f = 0: the P2 solution is that
+    # linear field (patch test); the inner iterations see nonzero Dirichlet
+    # columns, which the element operator must exclude
+    coords, elems = orc.mesh_kuhn(6, jitter=0.15, seed=5)
+    e2d, ndof = orc.doftable(3, 2, 1, elems)
+    ia, ja = orc.graph(e2d, ndof)
+    space = fa.Space(ctx, coords, elems, 2, 1, e2d, ndof)
+    g = fa.Graph(space, ia, ja)
+    A = fa.Matrix(g, fa.FP64_FP32)
+    A.assemble(fa.DIFFUSION, 4)
+    isb = orc.boundary_dofs(3, 2, 1, coords, elems, e2d, ndof)
+    # dof coordinates: the reference P2 nodes mapped by each element's affine map
+    # (barycentric coordinates on the [-1,1] reference tetrahedron)
+    ref = np.asarray(fa.lagrange_nodes(3, 2), dtype=np.float64).reshape(-1, 3)
+    assert ref.shape == (10, 3)
+    X = np.zeros((ndof, 3))
+    lam = np.column_stack([-(ref[:, 0] + ref[:, 1] + ref[:, 2] + 1) / 2, (ref[:, 0] + 1) / 2, (ref[:, 1] + 1) / 2,
+                           (ref[:, 2] + 1) / 2])
+    for e in range(len(elems)):
+        X[e2d[e]] = lam @ coords[elems[e]]
+    gfun = 1.0 + X[:, 0] - 2.0 * X[:, 1] + 3.0 * X[:, 2]
+    b = ctx.vector(ndof)
+    rows = fa.Rows(ctx, ndof, np.nonzero(isb)[0])
+    A.zero_rows(rows, ctx.vector(ndof, gfun), b, mode=fa.ELIMINATION_SYMMETRIC)
+    for op in (fa.INNER_ELEMENT, fa.INNER_ASSEMBLED):
+        x = ctx.vector(ndof)
+        st = A.solve(b, x, rtol=1e-12, maxit=20000, precision=fa.FP32, inner_operator=op)
+        assert st["converged"], (op, st)
+        assert np.max(np.abs(x.get() - gfun)) < 1e-9 * np.max(np.abs(gfun)), op
+
+
+def test_element_operator_follows_the_values(ctx, orc):
+    # the element operator is available only while the values are a P2
+    # diffusion assembly followed by symmetric unit-diagonal eliminations
+    P = problem(ctx, orc, 3, 2, 4, 4, orc.FK_CONST, 2, 0.15, 8)
+    A, bb, n = P["A"], P["bb"], P["ndof"]
+    x = ctx.vector(n)
+    assert A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)["converged"]
+    A.scale(2.0)
+    with pytest.raises(fa.FeelHipError):
+        A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
+    x.zero()
+    st = A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32)   # AUTO: the assembled values
+    xt, _, _, _ = orc.pcg(P["ia"], P["ja"], 2.0 * P["val"], P["b"], rtol=1e-14, maxit=10000)
+    # rows of the eliminated identity were scaled too: compare with the scaled oracle system
+    assert st["converged"] and rel_fro(x.get(), xt) < 1e-8
+    # re-assembled: available again; a nonsymmetric elimination takes it away
+    kd = P["keep"][2]
+    A.assemble(fa.DIFFUSION, 4, kdiff=kd)
+    x.zero()
+    assert A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)["converged"]
+    A.zero_rows(P["keep"][3], None, None, mode=fa.ELIMINATION)
+    with pytest.raises(fa.FeelHipError):
+        A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
+    # mass + diffusion is not covered
+    A.assemble(fa.DIFFUSION | fa.MASS, 4, kdiff=kd, mass=1.0)
+    with pytest.raises(fa.FeelHipError):
+        A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)

@@ -226,3 +226,18 @@ def test_msh22_solution_export(orc, tmp_path, dim, order, ncomp):
         got = np.array([float(t) for t in f[2:]]).reshape(dim + 1, ncomp)
         want = np.array([[vals[e2d[e, c * nloc + v]] for c in range(ncomp)] for v in range(dim + 1)])
         assert np.array_equal(got, want)
+
+
+def test_p2_stiffness_tensors_exact_at_degree_two():
+    # The element-by-element FP32 operator of P2 diffusion integrates
+    # grad phi_i . F grad phi_j with the 4-point degree-2 rule: the integrand is
+    # quadratic, so that rule gives the same reference tensors as the rules the
+    # assembly uses (order 4, 11-point Keast with its negative weight; order 6)
+    R2, _, _ = fa.reference_tensors(3, 2, 2)
+    p, w = fa.quadrature(3, 2)
+    assert len(w) == 4
+    for qo in (4, 6):
+        Rq, _, _ = fa.reference_tensors(3, 2, qo)
+        assert np.max(np.abs(R2 - Rq)) < 1e-13 * np.max(np.abs(Rq))
+    # stiffness rows sum to zero (constants in the kernel) for every a, b
+    assert np.max(np.abs(R2.sum(axis=2))) < 1e-13
