@@ -128,6 +128,7 @@ _SIGS = {
     "feel_hip_mat_energy": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
     "feel_hip_mat_set_matrix": (C.c_int, [_vp, C.c_int, _i32p, C.c_int, _i32p, _f64p]),
     "feel_hip_mat_get_entries": (C.c_int, [_vp, C.c_int64, _i32p, _i32p, _f64p]),
+    "feel_hip_mat_add_matrix_noalloc": (C.c_int, [_vp, C.c_int, _i32p, C.c_int, _i32p, _f64p]),
     "feel_hip_vec_create": (C.c_int, [_vp, C.c_int64, C.POINTER(_vp)]),
     "feel_hip_vec_destroy": (C.c_int, [_vp]),
     "feel_hip_vec_set": (C.c_int, [_vp, _f64p]),

@@ -54,6 +54,9 @@ int buildIncidenceDevice(feel_hip_space* s);
 int buildSellDevice(feel_hip_graph* g, const std::vector<int32_t>& perm);
 // the 3 x 3 block image of a vector-space graph for the node order nperm (sell.hip)
 int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm);
+// device address of the static staging array of feel_hip_mat_add_matrix_noalloc,
+// looked up once per device with that device current (noalloc.hip)
+int noallocSetup(int device);
 // element-by-element FP32 operator of P2 diffusion matrices (elemop.hip)
 struct ElemOp;
 int elemOpAssembled(feel_hip_mat* A, bool qualifies);

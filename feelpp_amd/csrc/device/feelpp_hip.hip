@@ -203,6 +203,7 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
     FEEL_HIP_CHECK(hipGetDeviceCount(&ndev));
     if (device < 0 || device >= ndev) return setError(FEEL_HIP_EINVAL, "no such HIP device");
     FEEL_HIP_CHECK(hipSetDevice(device));
+    if (int rs = noallocSetup(device); rs != FEEL_HIP_OK) return rs;
     auto* c = new feel_hip_ctx();
     c->device = device;
     hipDeviceProp_t prop;

@@ -228,6 +228,15 @@ int feel_hip_mat_set_matrix(feel_hip_mat* A, int nrows, const int32_t* rows, int
                             const double* data);
 /* operator()(i, j), :591: m entries (rows[k], cols[k]); entries outside the pattern read 0 */
 int feel_hip_mat_get_entries(const feel_hip_mat* A, int64_t m, const int32_t* rows, const int32_t* cols, double* out);
+/* feel_hip_mat_add_matrix without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same ordered ADD and the same errors;
+ * rows, columns, values and CSR positions are staged in one static device array
+ * per GPU (1.5 MiB, held until the process ends), so nrows * ncols must not
+ * exceed 65536 (FEEL_HIP_EINVAL otherwise).  Calls that overlap on one GPU share
+ * that array and race: order them. */
+int feel_hip_mat_add_matrix_noalloc(feel_hip_mat* A, int nrows, const int32_t* rows, int ncols, const int32_t* cols,
+                                    const double* data);
 
 /* ---- vectors (Vector<double>, feelalg/vector.hpp:143-576) ------------------ */
 int feel_hip_vec_create(feel_hip_ctx* ctx, int64_t n, feel_hip_vec** out);      /* Backend::newVector, backend.hpp:571-592 */
