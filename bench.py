@@ -80,8 +80,8 @@ class Problem:
 
 
 # solver forms: (precision, operator of the FP32 inner iterations)
-FORMS = {"fp64": ("FP64", "INNER_AUTO"), "mixed": ("FP32", "INNER_AUTO"),
-         "mixed_assembled": ("FP32", "INNER_ASSEMBLED")}
+FORMS = {"fp64": ("FP64", "INNER_AUTO"), "mixed": ("FP32", "INNER_ASSEMBLED"),
+         "mixed_element": ("FP32", "INNER_ELEMENT")}
 
 
 def time_form(prob, form, rtol, steps, warmup, inner_rtol):
@@ -361,8 +361,9 @@ def bench_beam(fa, args):
 def bench_kernels(fa, args):
     """Short fixed runs of the hot kernels for hardware-counter passes: a plain
     device copy of known size (calibrates the HBM byte counters), then on the
-    C2 problem 3 assemblies (scatter kernel) and 20-iteration FP64 and FP32
-    solves (SELL SpMV kernels)."""
+    C2 problem 3 assemblies (scatter kernel) and 20-iteration solves: FP64 (SELL
+    SpMV), mixed with the assembled FP32 matrix (FP32 SELL SpMV) and mixed with
+    the element-by-element FP32 operator."""
     import torch
     nbytes = 1 << 30
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
@@ -381,12 +382,14 @@ def bench_kernels(fa, args):
         prob.A.assemble(fa.DIFFUSION, qorder, kdiff=prob.k, precision=fa.FP64_FP32)
     fa.assemble_linear(prob.space, prob.b, fa.SRC_CONST, order, (1.0,))
     prob.A.zero_rows(prob.rows, None, prob.b, mode=fa.ELIMINATION_SYMMETRIC)
-    for prec in (fa.FP64, fa.FP32):
+    for prec, op in ((fa.FP64, fa.INNER_AUTO), (fa.FP32, fa.INNER_ASSEMBLED), (fa.FP32, fa.INNER_ELEMENT)):
         prob.x.zero()
-        st = prob.A.solve(prob.b, prob.x, rtol=1e-30, maxit=20, precision=prec, inner_rtol=1e-30, check_every=20)
-        log(f"precision {prec}: {st}")
+        st = prob.A.solve(prob.b, prob.x, rtol=1e-30, maxit=20, precision=prec, inner_rtol=1e-30, check_every=20,
+                          inner_operator=op)
+        log(f"precision {prec} inner operator {op}: {st}")
     ctx.synchronize()
-    print(json.dumps({"kernels": "copy 1 GiB x5; C2 assembly x3; FP64 and FP32 solves of 20 iterations",
+    print(json.dumps({"kernels": "copy 1 GiB x5; C2 assembly x3; solves of 20 iterations: FP64, mixed with "
+                                 "the assembled FP32 matrix, mixed with the element operator",
                       "nnz": prob.nnz, "dofs": prob.ndof, "elements": prob.ne}), flush=True)
 
 
@@ -413,6 +416,9 @@ def bench_beam_kernels(fa, args):
     ndof = space.ndof
     isb = fa.boundary_dofs(3, k, 3, coords, elems, space.elem2dof, ndof, sel=1)
     A = fa.Matrix(g, fa.FP64_FP32)
+    if args.spmv_shape >= 0:   # a fixed launch shape instead of the first-solve tuning
+        for prec in (fa.FP64, fa.FP32):
+            g.set_spmv_shape(prec, args.spmv_shape)
     young = ctx.vector(ne, fa.uniform(ne, 1.0, 100.0, args.seed + 1, log=True))
     poisson = ctx.vector(ne, fa.uniform(ne, 0.2, 0.35, args.seed + 2))
     bvec, x = ctx.vector(ndof), ctx.vector(ndof)
@@ -570,7 +576,7 @@ def main():
                     help="batched/serving: one SpMV per instance instead of one per family group")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="residual reduction between FP64 reliable updates of the FP32 solves (default 1e-2)")
-    ap.add_argument("--forms", default="fp64,mixed,mixed_assembled")
+    ap.add_argument("--forms", default="fp64,mixed,mixed_element")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="c2", choices=["c2", "batched", "serving", "beam", "kernels", "elastic", "beamkernels"])
     ap.add_argument("--orders", default="1,2,3,4")
@@ -580,6 +586,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=16)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--rates", default="5,10,20,30,50,200,1000,2000")
+    ap.add_argument("--spmv-shape", type=int, default=-1,
+                    help="beamkernels: pin the SpMV launch shape (index into the candidate list)")
     ap.add_argument("--backfill", type=int, default=8,
                     help="serving: steps a request that does not fit the HBM budget lets smaller ones pass it")
     ap.add_argument("--requests", type=int, default=200)
@@ -665,7 +673,7 @@ def main():
     else:
         roof = spmv_roofline(best, "spmv_fp32")
     others = {}
-    for f, timer in (("mixed_assembled", "spmv_fp32"), ("fp64", "spmv_fp64")):
+    for f, timer in (("mixed", "spmv_fp32"), ("mixed_element", "elemop_fp32"), ("fp64", "spmv_fp64")):
         if f in results and f != best and results[f]["timers"][timer][0] > 0:
             others[f] = spmv_roofline(f, timer)
     # scatter (owner-computes gather of element contributions into CSR values, FP64 + FP32 copies):

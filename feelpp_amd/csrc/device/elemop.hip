@@ -9,9 +9,10 @@
 // the assembled matrix:
 //   pass 1, one thread per element: gather p at the element's ten dofs
 //     (Dirichlet columns read as 0), y_e = sum_q w_q G_q F_e G_q^T p_e with the
-//     reference gradients G_q at the four points of a degree-2 rule and the
-//     element's factor F_e = k J K^{-1} K^{-T} (six FP32 numbers), written to
-//     ten planes Y[l][e];
+//     reference gradients G_q at the four points of a degree-2 rule (exact for
+//     the quadratic integrand, so the same integrals as the assembly's rule)
+//     and the element's factor F_e = k J K^{-1} K^{-T} (six FP32 numbers),
+//     written to ten planes Y[l][e];
 //   pass 2, one lane per solver row (64-row slices): the row's incident
 //     entries of Y summed in a fixed order (a SELL image of Y indices); a
 //     Dirichlet row returns p_i (unit diagonal, zero row and column).
@@ -23,13 +24,11 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
-#include <cstring>
 #include <string>
 #include <vector>
 
 #include "common.hpp"
 #include "kernels.hpp"
-#include "../host/fe.hpp"
 
 using namespace feelpp_amd;
 
@@ -59,11 +58,17 @@ namespace {
 constexpr int kNq = 4;      // degree-2 rule on the tetrahedron
 constexpr int kN = 10;      // P2 local dofs
 
-// reference gradients at the rule's points, scaled weights
-struct P2Table {
-    float g[kNq][kN][3];
-    float w[kNq];
-};
+// P2 nodal basis on the reference tetrahedron [-1,1]^3 (vertices (-1,-1,-1),
+// (1,-1,-1), (-1,1,-1), (-1,-1,1); local dofs: the vertices, then the edges
+// (1,2) (0,2) (0,1) (0,3) (1,3) (2,3), the order of the P2 node set):
+// phi_v = l_v (2 l_v - 1), phi_(a,b) = 4 l_a l_b, with l_k = (xi_k + 1) / 2 for
+// k = 1..3 and l_0 = 1 - l_1 - l_2 - l_3, so grad l_0 = -(1,1,1)/2 and
+// grad l_k = e_k / 2.  The degree-2 rule: the four points with barycentric
+// coordinates (a, b, b, b) and permutations, weight |T_ref| / 4 = 1/3 each.
+constexpr float kAlpha = 0.58541019662496845446f;   // (5 + 3 sqrt 5) / 20
+constexpr float kBeta = 0.13819660112501051518f;    // (5 - sqrt 5) / 20
+constexpr float kWq = 1.0f / 3.0f;
+constexpr int kEdge[6][2] = {{1, 2}, {0, 2}, {0, 1}, {0, 3}, {1, 3}, {2, 3}};
 
 #define EO_CHECK(expr)                                                                                   \
     do {                                                                                                 \
@@ -183,8 +188,13 @@ __global__ __launch_bounds__(kBlock) void kEoColFlags(int64_t m, const int32_t* 
     }
 }
 
-// pass 1: Y[l][pos] = (A_e p_e)_l, Dirichlet columns excluded
-__global__ __launch_bounds__(kBlock) void kElemApplyP2(int64_t ne, P2Table T, const float* __restrict__ F,
+// pass 1: Y[l][pos] = (A_e p_e)_l, Dirichlet columns excluded.  At each
+// point of the rule: grad u = sum_v s_v grad l_v with
+// s_v = p_v (4 l_v - 1) + 4 sum_{edges (v,o)} p_e l_o, then h = w F grad u and
+// (A_e p)_i += grad phi_i . h, i.e. (4 l_v - 1) t_v for a vertex and
+// 4 (l_a t_b + l_b t_a) for an edge, t_v = grad l_v . h.  The barycentric
+// coordinates are constants, so no table is loaded.
+__global__ __launch_bounds__(kBlock) void kElemApplyP2(int64_t ne, const float* __restrict__ F,
                                                        const int32_t* __restrict__ edof, const float* __restrict__ p,
                                                        float* __restrict__ Y, const CgState* __restrict__ st) {
     if (st && blockDone(st)) return;
@@ -202,18 +212,30 @@ __global__ __launch_bounds__(kBlock) void kElemApplyP2(int64_t ne, P2Table T, co
         for (int l = 0; l < kN; ++l) y[l] = 0.0f;
 #pragma unroll
         for (int q = 0; q < kNq; ++q) {
-            float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+            float lam[4], sv[4];
 #pragma unroll
-            for (int l = 0; l < kN; ++l) {
-                g0 += T.g[q][l][0] * pe[l];
-                g1 += T.g[q][l][1] * pe[l];
-                g2 += T.g[q][l][2] * pe[l];
+            for (int v = 0; v < 4; ++v) {
+                lam[v] = v == q ? kAlpha : kBeta;
+                sv[v] = pe[v] * (4.0f * lam[v] - 1.0f);
             }
-            const float h0 = T.w[q] * (f00 * g0 + f01 * g1 + f02 * g2);
-            const float h1 = T.w[q] * (f01 * g0 + f11 * g1 + f12 * g2);
-            const float h2 = T.w[q] * (f02 * g0 + f12 * g1 + f22 * g2);
 #pragma unroll
-            for (int l = 0; l < kN; ++l) y[l] += T.g[q][l][0] * h0 + T.g[q][l][1] * h1 + T.g[q][l][2] * h2;
+            for (int k = 0; k < 6; ++k) {
+                const int a = kEdge[k][0], b = kEdge[k][1];
+                sv[a] += (4.0f * lam[b]) * pe[4 + k];
+                sv[b] += (4.0f * lam[a]) * pe[4 + k];
+            }
+            const float g0 = 0.5f * (sv[1] - sv[0]), g1 = 0.5f * (sv[2] - sv[0]), g2 = 0.5f * (sv[3] - sv[0]);
+            const float h0 = kWq * (f00 * g0 + f01 * g1 + f02 * g2);
+            const float h1 = kWq * (f01 * g0 + f11 * g1 + f12 * g2);
+            const float h2 = kWq * (f02 * g0 + f12 * g1 + f22 * g2);
+            const float t[4] = {-0.5f * (h0 + h1 + h2), 0.5f * h0, 0.5f * h1, 0.5f * h2};
+#pragma unroll
+            for (int v = 0; v < 4; ++v) y[v] += (4.0f * lam[v] - 1.0f) * t[v];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const int a = kEdge[k][0], b = kEdge[k][1];
+                y[4 + k] += 4.0f * (lam[a] * t[b] + lam[b] * t[a]);
+            }
         }
 #pragma unroll
         for (int l = 0; l < kN; ++l) {
@@ -269,34 +291,8 @@ __global__ void kEoZeroTail(float* __restrict__ y) {
     if (blockIdx.x == 0 && threadIdx.x == 0) y[0] = 0.0f;
 }
 
-// ok = false when the degree-2 rule or the P2 basis do not have the expected sizes
-P2Table makeTable(bool* ok) {
-    P2Table T;
-    std::memset(&T, 0, sizeof(T));
-    const QuadratureRule q = quadrature(3, 2);
-    const Tabulation t = tabulate(3, 2, q.pts);
-    *ok = q.npts == kNq && t.n == kN;
-    if (!*ok) return T;
-    for (int p = 0; p < kNq; ++p) {
-        T.w[p] = (float)q.w[p];
-        for (int l = 0; l < kN; ++l)
-            for (int a = 0; a < 3; ++a) T.g[p][l][a] = (float)t.dphi[((size_t)l * q.npts + p) * 3 + a];
-    }
-    return T;
-}
-
 constexpr int kGatherBlock = 256;
 constexpr int kGatherGrid = 2048;
-
-struct TableHolder {
-    bool ok = false;
-    P2Table T;
-    TableHolder() { T = makeTable(&ok); }
-};
-const TableHolder& table() {
-    static const TableHolder h;
-    return h;
-}
 
 }  // namespace
 
@@ -304,7 +300,7 @@ namespace feelpp_amd {
 
 bool elemOpCandidate(const feel_hip_mat* A) {
     const feel_hip_space* s = A->g->space;
-    return s->dim == 3 && s->order == 2 && s->ncomp == 1 && !A->g->block && A->g->perm && A->v32 && table().ok;
+    return s->dim == 3 && s->order == 2 && s->ncomp == 1 && !A->g->block && A->g->perm && A->v32;
 }
 
 // Structure of the element operator from the space's elem2dof and the
@@ -472,9 +468,8 @@ int elemOpApply(feel_hip_mat* A, const float* p, float* w, double* part, const C
         FEEL_HIP_CHECK(hipGetLastError());
         E->flagsDirty = false;
     }
-    const P2Table& T = table().T;
     const int g1 = (int)std::min<int64_t>((E->ne + kBlock - 1) / kBlock, 16384);
-    hipLaunchKernelGGL(kElemApplyP2, dim3(g1), dim3(kBlock), 0, c->stream, E->ne, T, E->F, E->edof, p, E->Y, st);
+    hipLaunchKernelGGL(kElemApplyP2, dim3(g1), dim3(kBlock), 0, c->stream, E->ne, E->F, E->edof, p, E->Y, st);
     hipLaunchKernelGGL(kElemGather<kGatherBlock>, dim3(kGatherGrid), dim3(kGatherBlock), 0, c->stream, E->n,
                        E->nslices, E->incPtr, E->inc, E->Y, E->brow, p, w, part, st);
     FEEL_HIP_CHECK(hipGetLastError());

@@ -1097,19 +1097,17 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
 }
 
 // y = A x on the 3 x 3 block image of a vector space: one wavefront per slice
-// of 64 node rows, a lane computes its node's three rows.  Per quad of block
-// slots the lane reads four node columns with one 16-byte load and the nine
-// value planes with one 16-byte (FP32, non-temporal: the values are streamed
-// once and L2 is kept for the x gathers) or 32-byte (FP64) load each, then
-// gathers three consecutive x entries per block.  Twelve partial sums (row c,
-// slot of the quad) per lane, reduced in a fixed order.  With part, also the
-// block partial of x.y (p.Ap).
+// of 64 node rows, a lane computes its node's three rows; per block slot one
+// column index, nine values (planes of 64, read non-temporally) and three
+// consecutive x entries.  UB block slots per step (4 for FP32, 2 for FP64:
+// measured) keep independent partial sums.  With part, also the block partial
+// of x.y (p.Ap).
 template <typename T, int BS>
 __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
                                                      const int32_t* __restrict__ bcol, const T* __restrict__ val,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      double* __restrict__ part, const CgState* __restrict__ st) {
-    typedef typename Quad<T>::type V;
+    constexpr int UB = sizeof(T) == 4 ? 4 : 2;
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
@@ -1118,38 +1116,45 @@ __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nsli
     double dot = 0.0;
     for (int64_t s = wave; s < nslices; s += nwaves) {
         const uint32_t base = slicePtr[s];
-        const int nq = (int)((slicePtr[s + 1] - base) >> 8);
-        const i32x4* c = reinterpret_cast<const i32x4*>(bcol + base) + lane;
-        const V* v = reinterpret_cast<const V*>(val + 9 * (size_t)base) + lane;
-        T acc[3][4];
+        const int width = (int)((slicePtr[s + 1] - base) >> 6);
+        const int32_t* c = bcol + base + lane;
+        const T* v = val + 9 * (size_t)base + lane;
+        T acc[UB][3];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) acc[r][0] = acc[r][1] = acc[r][2] = acc[r][3] = 0;
-        for (int k = 0; k < nq; ++k) {
-            const i32x4 j = c[(size_t)k * 64];
-            V vv[9];
+        for (int u = 0; u < UB; ++u) acc[u][0] = acc[u][1] = acc[u][2] = 0;
+        int k = 0;
+        for (; k + UB <= width; k += UB) {
+            int32_t j[UB];
+            T vv[UB][9];
 #pragma unroll
-            for (int q = 0; q < 9; ++q)
-                vv[q] = sizeof(T) == 4 ? __builtin_nontemporal_load(v + ((size_t)k * 9 + q) * 64)
-                                       : v[((size_t)k * 9 + q) * 64];
-            T xx[4][3];
+            for (int u = 0; u < UB; ++u) {
+                j[u] = c[(size_t)(k + u) * 64];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int64_t jt = 3 * (int64_t)j[t];
-                xx[t][0] = x[jt];
-                xx[t][1] = x[jt + 1];
-                xx[t][2] = x[jt + 2];
+                for (int q = 0; q < 9; ++q) vv[u][q] = __builtin_nontemporal_load(v + ((size_t)(k + u) * 9 + q) * 64);
             }
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-                    acc[r][t] += vv[3 * r][t] * xx[t][0] + vv[3 * r + 1][t] * xx[t][1] + vv[3 * r + 2][t] * xx[t][2];
+            for (int u = 0; u < UB; ++u) {
+                const T x0 = x[3 * (int64_t)j[u]], x1 = x[3 * (int64_t)j[u] + 1], x2 = x[3 * (int64_t)j[u] + 2];
+                acc[u][0] += vv[u][0] * x0 + vv[u][1] * x1 + vv[u][2] * x2;
+                acc[u][1] += vv[u][3] * x0 + vv[u][4] * x1 + vv[u][5] * x2;
+                acc[u][2] += vv[u][6] * x0 + vv[u][7] * x1 + vv[u][8] * x2;
+            }
+        }
+        for (; k < width; ++k) {
+            const int32_t jj = c[(size_t)k * 64];
+            const T x0 = x[3 * (int64_t)jj], x1 = x[3 * (int64_t)jj + 1], x2 = x[3 * (int64_t)jj + 2];
+            const T* w = v + (size_t)k * 9 * 64;
+            acc[0][0] += w[0] * x0 + w[64] * x1 + w[128] * x2;
+            acc[0][1] += w[192] * x0 + w[256] * x1 + w[320] * x2;
+            acc[0][2] += w[384] * x0 + w[448] * x1 + w[512] * x2;
         }
         const int64_t r = s * 64 + lane;
         if (r < nb) {
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                const T a = (acc[cc][0] + acc[cc][1]) + (acc[cc][2] + acc[cc][3]);
+                T a = 0;
+#pragma unroll
+                for (int u = 0; u < UB; ++u) a += acc[u][cc];
                 y[3 * r + cc] = a;
                 if (part) dot += (double)x[3 * r + cc] * (double)a;
             }
@@ -1215,8 +1220,8 @@ static __global__ __launch_bounds__(kBlock) void kDiagInvSell(int64_t n, const i
             const double d = s64[sdiag[i]];
             d64[i] = d != 0.0 ? 1.0 / d : 1.0;
         }
-        if (d32) {
-            const double d = s32 ? (double)s32[sdiag[i]] : s64[sdiag[i]];
+        if (d32) {   // from the FP64 diagonal when there is one, so D^{-1} does not depend on the FP32 image
+            const double d = s64 ? s64[sdiag[i]] : (double)s32[sdiag[i]];
             d32[i] = (float)(d != 0.0 ? 1.0 / d : 1.0);
         }
     }

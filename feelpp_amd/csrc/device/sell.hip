@@ -246,14 +246,11 @@ done:
 // ------------------------------------------------------------------------
 // 3 x 3 block image for vector spaces (ncomp = 3, dofs interleaved node * 3 +
 // c, COUPLED pattern: the rows of a node share their columns, which come in
-// triples).  One lane per node row, 64 node rows per slice, slice width (block
-// slots per lane) rounded up to a multiple of 4 and cut into quads as in the
-// scalar image: the lane's k-th block has its permuted node column at
-// base + 256 (k / 4) + 4 lane + k % 4 and its value (c, d) at
-// 9 base + 256 (9 (k / 4) + 3 c + d) + 4 lane + k % 4, so a lane reads a quad's
-// four columns with one load and each of its nine value planes with one
-// 16-byte (FP32) or 32-byte (FP64) load.  One column index serves nine values
-// and the x gathers read three consecutive entries.
+// triples).  One lane per node row, 64 node rows per slice; block slot
+// (s, k, lane) holds the permuted node column of the lane's k-th block, and its
+// nine values sit in planes, value (c, d) of block slot k at
+// 9 base + (9 k + 3 c + d) 64 + lane.  One column index serves nine values and
+// the x gathers read three consecutive entries.
 
 namespace {
 
@@ -267,7 +264,7 @@ __global__ __launch_bounds__(kSlBlock) void kBlockSliceSlots(int64_t nb, int64_t
         int len = 0;
         if (r < nb) len = (ia[3 * (int64_t)nperm[r] + 1] - ia[3 * (int64_t)nperm[r]]) / 3;
         for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
-        if (lane == 0) slots[s] = (uint64_t)((len + 3) & ~3) * kC;
+        if (lane == 0) slots[s] = (uint64_t)len * kC;
     }
 }
 
@@ -288,21 +285,21 @@ __global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nb, int64_t n
         const int32_t old = r < nb ? nperm[r] : 0;
         const int len = r < nb ? (ia[3 * (int64_t)old + 1] - ia[3 * (int64_t)old]) / 3 : 0;
         for (int k = 0; k < width; ++k) {
-            const int64_t bs = b + (int64_t)(k >> 2) * 4 * kC + 4 * lane + (k & 3);
-            const size_t v0 = 9 * (size_t)b + (size_t)(k >> 2) * 9 * 4 * kC + 4 * lane + (k & 3);
+            const int64_t bs = b + (int64_t)k * kC + lane;
+            const size_t v0 = 9 * (size_t)b + (size_t)k * 9 * kC + lane;
             if (k < len) {
                 const int32_t bj = ja[ia[3 * (int64_t)old] + 3 * k] / 3;
                 bcol[bs] = niperm[bj];
                 for (int c = 0; c < 3; ++c)
                     for (int d = 0; d < 3; ++d) {
                         const int32_t p = ia[3 * (int64_t)old + c] + 3 * k + d;
-                        const size_t vs = v0 + (size_t)(3 * c + d) * 4 * kC;
+                        const size_t vs = v0 + (size_t)(3 * c + d) * kC;
                         slot2csr[vs] = p;
                         if (bj == old && c == d) diagSlot[3 * r + c] = (int32_t)vs;
                     }
             } else {
                 bcol[bs] = self;
-                for (int q = 0; q < 9; ++q) slot2csr[v0 + (size_t)q * 4 * kC] = -1;
+                for (int q = 0; q < 9; ++q) slot2csr[v0 + (size_t)q * kC] = -1;
             }
         }
     }

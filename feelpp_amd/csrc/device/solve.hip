@@ -495,7 +495,10 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
     if (o.inner_operator == FEEL_HIP_INNER_ELEMENT && !elemOpReady(A))
         return setError(FEEL_HIP_EUNSUPPORTED,
                         "element-by-element inner operator needs a P2 diffusion assembly (symmetric on() only)");
-    const bool elemOp = o.inner_operator != FEEL_HIP_INNER_ASSEMBLED && elemOpReady(A);
+    // AUTO takes the assembled FP32 values: on a P2 Poisson matrix the element
+    // operator moves about as many bytes per product (its element results are
+    // written, then gathered by row) and takes longer
+    const bool elemOp = o.inner_operator == FEEL_HIP_INNER_ELEMENT;
     // the element operator replaces the FP32 SELL values of the inner iterations
     TRY_CALL(ensureSell(A, true, !elemOp));
     hipLaunchKernelGGL(kPermuteIn, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, g->perm, b->d, A->bp64);

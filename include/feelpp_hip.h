@@ -106,7 +106,10 @@ typedef struct {
  * per-element factors k J K^{-1} K^{-T} in FP32, available while the matrix
  * holds a P2 tetrahedral diffusion assembly (operator=, quadrature order >= 2)
  * followed only by symmetric unit-diagonal eliminations; FP64 residuals still
- * use the assembled FP64 values.  AUTO: ELEMENT when available, else ASSEMBLED.
+ * use the assembled FP64 values; the FP32 matrix image is then not needed.
+ * AUTO: ASSEMBLED (faster on MI355X: the element operator writes its element
+ * results and gathers them back by row, about as many bytes per product as the
+ * assembled FP32 values and columns).
  * No counterpart in the reference (PETSc applies the assembled matrix). */
 enum { FEEL_HIP_INNER_AUTO = 0, FEEL_HIP_INNER_ASSEMBLED = 1, FEEL_HIP_INNER_ELEMENT = 2 };
 

@@ -289,9 +289,9 @@ def test_element_operator_inner_iterations(ctx, orc, seed):
         assert rel_fro(x.get(), xt) < 1e-8, op
         assert st["its"] <= 1.5 * its_o, (op, st, its_o)
         out[op] = (st, x.get())
-    # AUTO takes the element operator for this matrix
-    assert out[fa.INNER_AUTO][0] == out[fa.INNER_ELEMENT][0]
-    assert np.array_equal(out[fa.INNER_AUTO][1], out[fa.INNER_ELEMENT][1])
+    # AUTO takes the assembled FP32 values
+    assert out[fa.INNER_AUTO][0] == out[fa.INNER_ASSEMBLED][0]
+    assert np.array_equal(out[fa.INNER_AUTO][1], out[fa.INNER_ASSEMBLED][1])
 
 
 def test_element_operator_nonzero_dirichlet(ctx, orc):
@@ -342,11 +342,12 @@ def test_element_operator_follows_the_values(ctx, orc):
     # rows of the eliminated identity were scaled too: compare with the scaled oracle system
     assert st["converged"] and rel_fro(x.get(), xt) < 1e-8
     # re-assembled: available again; a nonsymmetric elimination takes it away
-    kd = P["keep"][2]
+    kd, rows = P["keep"][2], P["keep"][3]
     A.assemble(fa.DIFFUSION, 4, kdiff=kd)
+    A.zero_rows(rows, None, None, mode=fa.ELIMINATION_SYMMETRIC)
     x.zero()
     assert A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)["converged"]
-    A.zero_rows(P["keep"][3], None, None, mode=fa.ELIMINATION)
+    A.zero_rows(rows, None, None, mode=fa.ELIMINATION)
     with pytest.raises(fa.FeelHipError):
         A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
     # mass + diffusion is not covered

@@ -241,3 +241,28 @@ def test_p2_stiffness_tensors_exact_at_degree_two():
         assert np.max(np.abs(R2 - Rq)) < 1e-13 * np.max(np.abs(Rq))
     # stiffness rows sum to zero (constants in the kernel) for every a, b
     assert np.max(np.abs(R2.sum(axis=2))) < 1e-13
+    # the operator's closed form: nodal P2 basis phi_v = l_v (2 l_v - 1),
+    # phi_(a,b) = 4 l_a l_b on the [-1,1] tetrahedron with the edge order
+    # (1,2) (0,2) (0,1) (0,3) (1,3) (2,3), at the symmetric 4-point rule
+    # (barycentric (a, b, b, b), weights 1/3) gives the reference tensors
+    alpha, beta = (5 + 3 * np.sqrt(5)) / 20, (5 - np.sqrt(5)) / 20
+    edges = [(1, 2), (0, 2), (0, 1), (0, 3), (1, 3), (2, 3)]
+    glam = np.array([[-0.5, -0.5, -0.5], [0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5]])
+    R = np.zeros((3, 3, 10, 10))
+    for q in range(4):
+        lam = np.full(4, beta)
+        lam[q] = alpha
+        G = np.zeros((10, 3))
+        for v in range(4):
+            G[v] = (4 * lam[v] - 1) * glam[v]
+        for k, (a, b) in enumerate(edges):
+            G[4 + k] = 4 * (lam[a] * glam[b] + lam[b] * glam[a])
+        R += (1.0 / 3.0) * np.einsum("ia,jb->abij", G, G)
+    R4, _, _ = fa.reference_tensors(3, 2, 4)
+    assert np.max(np.abs(R.reshape(9, 10, 10) - R4)) < 1e-13 * np.max(np.abs(R4))
+    # and the nodes are the vertices and those edge midpoints
+    nodes = fa.lagrange_nodes(3, 2)
+    ref = np.array([[-1, -1, -1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]], dtype=float)
+    assert np.allclose(nodes[:4], ref)
+    for k, (a, b) in enumerate(edges):
+        assert np.allclose(nodes[4 + k], 0.5 * (ref[a] + ref[b]))
