@@ -39,6 +39,7 @@ constexpr int kSpmvBlock = 1024;
 constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
 constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
 constexpr int kSegTask = 256;         // scatter entries per task (whole slots)
+constexpr int kBlockLanes = 2;        // lanes per node row of the 3 x 3 block image (measured: 1, 2, 4)
 
 }  // namespace feelpp_amd
 
@@ -63,6 +64,7 @@ int elemOpAssembled(feel_hip_mat* A, bool qualifies);
 int elemOpZeroRows(feel_hip_mat* A, const feel_hip_rows* rows, int mode, double diag);
 void elemOpInvalidate(feel_hip_mat* A);
 bool elemOpReady(const feel_hip_mat* A);
+int elemOpPrepare(feel_hip_mat* A);
 void elemOpFree(feel_hip_mat* A);
 int elemOpApply(feel_hip_mat* A, const float* p, float* w, double* part, const CgState* st, int* npart);
 void retain(feel_hip_ctx* c);
@@ -126,6 +128,7 @@ struct feel_hip_space {
     // per-element scratch (geometric factors / element vectors)
     double* geo = nullptr;
     size_t geoBytes = 0;
+    uint64_t geoStamp = 0;        // bumped by every assembly that rewrites geo
     double* elemvec = nullptr;
     // reference tensors per quadrature order: [Rsym(NS) | M | m] device copies
     std::map<int, double*> rtab;
@@ -171,6 +174,7 @@ struct feel_hip_graph {
     bool compact = false;         // packed 16/32-bit column stream available
     bool block = false;           // 3 x 3 block image (vector spaces): slicePtr/scol in block slots
     int64_t nbslots = 0;          // block slots of the block image (nslots = 9 nbslots value slots)
+    int blockLanes = 2;           // block image: lanes per node row (64 / blockLanes rows per slice)
     int32_t* colBase = nullptr;   // nslices, -1 = slice stores int32 columns
     int64_t* ccptr = nullptr;     // nslices, slice start in packed (uint16 units)
     uint16_t* packed = nullptr;   // npacked
@@ -200,6 +204,8 @@ struct feel_hip_mat {
     // element-by-element FP32 operator: 1 while the values are a P2 diffusion
     // assembly followed only by symmetric unit-diagonal eliminations
     int mfState = 0;
+    uint64_t geoStamp = 0;        // the space's geoStamp after this matrix's last assembly
+    uint8_t* mfRows = nullptr;    // Dirichlet rows of its symmetric eliminations (Feel++ numbering)
     feelpp_amd::ElemOp* eop = nullptr;
 };
 

@@ -45,7 +45,7 @@ struct ElemOp {
     int64_t nslices = 0, nslots = 0;
     // values
     float* F = nullptr;             // [6][ne] k J (K^{-1} K^{-T}): 00 11 22 01 02 12, element order
-    uint8_t* bold = nullptr;        // n, Dirichlet rows in Feel++ numbering (union over on() calls)
+    bool factorsValid = false;      // F holds the factors of the matrix's last assembly
     uint8_t* brow = nullptr;        // n, Dirichlet rows in solver numbering
     float* Y = nullptr;             // 10 ne + 1
     bool flagsDirty = true;
@@ -77,6 +77,12 @@ constexpr int kEdge[6][2] = {{1, 2}, {0, 2}, {0, 1}, {0, 3}, {1, 3}, {2, 3}};
             rc = setError(FEEL_HIP_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
             goto done;                                                                                   \
         }                                                                                                \
+    } while (0)
+
+#define TRY_CALL_EO(expr)                    \
+    do {                                     \
+        int rc_ = (expr);                    \
+        if (rc_ != FEEL_HIP_OK) return rc_;  \
     } while (0)
 
 int gridOf(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, 65536)); }
@@ -377,7 +383,6 @@ static int buildStructure(feel_hip_mat* A, ElemOp* E) {
     rc = c->alloc((void**)&E->incPtr, sizeof(uint32_t) * (E->nslices + 1));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&E->inc, sizeof(int32_t) * std::max<int64_t>(1, E->nslots));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&E->F, sizeof(float) * 6 * ne);
-    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&E->bold, n);
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&E->brow, n);
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&E->Y, sizeof(float) * (m + 1));
     if (rc != FEEL_HIP_OK) goto done;
@@ -386,7 +391,6 @@ static int buildStructure(feel_hip_mat* A, ElemOp* E) {
                        start, ysorted, E->inc);
     hipLaunchKernelGGL(kEoZeroTail, dim3(1), dim3(64), 0, st, E->Y + m);
     EO_CHECK(hipGetLastError());
-    EO_CHECK(hipMemsetAsync(E->bold, 0, n, st));
     EO_CHECK(hipStreamSynchronize(st));
 done:
     for (void* p : {(void*)iperm, (void*)key, (void*)key2, (void*)id, (void*)ykey, (void*)ykey2, (void*)yval,
@@ -395,10 +399,12 @@ done:
     return rc;
 }
 
-void elemOpFree(feel_hip_mat* A) {
+// the operator's arrays (not the matrix's row set); sizes are zero for arrays
+// a failed build did not reach
+static void releaseOp(feel_hip_mat* A) {
+    feel_hip_ctx* c = A->g->space->ctx;
     ElemOp* E = A->eop;
     if (!E) return;
-    feel_hip_ctx* c = A->g->space->ctx;
     const int64_t m = E->ne * kN;
     c->release(E->edof, sizeof(int32_t) * m);
     c->release(E->edof0, sizeof(int32_t) * m);
@@ -406,62 +412,111 @@ void elemOpFree(feel_hip_mat* A) {
     c->release(E->incPtr, sizeof(uint32_t) * (E->nslices + 1));
     c->release(E->inc, sizeof(int32_t) * std::max<int64_t>(1, E->nslots));
     c->release(E->F, sizeof(float) * 6 * E->ne);
-    c->release(E->bold, E->n);
     c->release(E->brow, E->n);
     c->release(E->Y, sizeof(float) * (m + 1));
     delete E;
     A->eop = nullptr;
+}
+
+void elemOpFree(feel_hip_mat* A) {
+    A->g->space->ctx->release(A->mfRows, A->g->n);
+    A->mfRows = nullptr;
     A->mfState = 0;
+    releaseOp(A);
 }
 
 void elemOpInvalidate(feel_hip_mat* A) { A->mfState = 0; }
 
-// After a bilinear assembly: the operator stays available when the values are a
-// P2 diffusion assembly (qualifies); the space's factors are then copied.
-int elemOpAssembled(feel_hip_mat* A, bool qualifies) {
-    A->mfState = 0;
-    if (!qualifies || !elemOpCandidate(A)) return FEEL_HIP_OK;
+// the factors of the matrix's last assembly, still in the space's scratch, into F
+static int snapshotFactors(feel_hip_mat* A) {
     feel_hip_space* s = A->g->space;
-    feel_hip_ctx* c = s->ctx;
-    if (!A->eop) {
-        A->eop = new ElemOp();
-        int rc = buildStructure(A, A->eop);
-        if (rc != FEEL_HIP_OK) {
-            elemOpFree(A);
-            return rc;
-        }
-    }
     ElemOp* E = A->eop;
-    hipLaunchKernelGGL(kEoFactors, dim3(gridOf(E->ne)), dim3(kBlock), 0, c->stream, E->ne, E->eorder, s->geo, E->F);
-    FEEL_HIP_CHECK(hipMemsetAsync(E->bold, 0, E->n, c->stream));
+    hipLaunchKernelGGL(kEoFactors, dim3(gridOf(E->ne)), dim3(kBlock), 0, s->ctx->stream, E->ne, E->eorder, s->geo,
+                       E->F);
     FEEL_HIP_CHECK(hipGetLastError());
-    E->flagsDirty = true;
-    A->mfState = 1;
+    E->factorsValid = true;
+    return FEEL_HIP_OK;
+}
+
+// After a bilinear assembly: the operator stays available when the values are
+// a P2 diffusion assembly (qualifies).  Nothing is copied until an
+// element-operator solve asks for it; from then on each qualifying assembly
+// of the matrix copies its factors.
+int elemOpAssembled(feel_hip_mat* A, bool qualifies) {
+    feel_hip_space* s = A->g->space;
+    A->mfState = qualifies && elemOpCandidate(A) ? 1 : 0;
+    A->geoStamp = s->geoStamp;
+    if (A->mfRows) FEEL_HIP_CHECK(hipMemsetAsync(A->mfRows, 0, A->g->n, s->ctx->stream));
+    if (A->eop) {
+        A->eop->factorsValid = false;
+        A->eop->flagsDirty = true;
+        if (A->mfState == 1) return snapshotFactors(A);
+    }
     return FEEL_HIP_OK;
 }
 
 int elemOpZeroRows(feel_hip_mat* A, const feel_hip_rows* rows, int mode, double diag) {
-    if (A->mfState != 1 || !A->eop) return FEEL_HIP_OK;
+    if (A->mfState != 1) return FEEL_HIP_OK;
     if (mode != FEEL_HIP_ON_ELIMINATION_SYMMETRIC || diag != 1.0) {
         A->mfState = 0;
         return FEEL_HIP_OK;
     }
-    ElemOp* E = A->eop;
     feel_hip_ctx* c = A->g->space->ctx;
-    hipLaunchKernelGGL(kEoOrFlags, dim3(gridOf(E->n)), dim3(kBlock), 0, c->stream, E->n, rows->flag, E->bold);
+    const int64_t n = A->g->n;
+    if (!A->mfRows) {
+        if (c->alloc((void**)&A->mfRows, n) != FEEL_HIP_OK) {   // without the row set the operator is unavailable
+            A->mfState = 0;
+            return FEEL_HIP_OK;
+        }
+        FEEL_HIP_CHECK(hipMemsetAsync(A->mfRows, 0, n, c->stream));
+    }
+    hipLaunchKernelGGL(kEoOrFlags, dim3(gridOf(n)), dim3(kBlock), 0, c->stream, n, rows->flag, A->mfRows);
     FEEL_HIP_CHECK(hipGetLastError());
-    E->flagsDirty = true;
+    if (A->eop) A->eop->flagsDirty = true;
     return FEEL_HIP_OK;
 }
 
-bool elemOpReady(const feel_hip_mat* A) { return A->mfState == 1 && A->eop; }
+bool elemOpReady(const feel_hip_mat* A) { return A->mfState == 1; }
+
+// Before an element-operator solve: builds the structure on first use and
+// copies the factors if the space's scratch still holds this matrix's last
+// assembly (no other assembly on the space since); otherwise the matrix must
+// be assembled again.
+int elemOpPrepare(feel_hip_mat* A) {
+    feel_hip_space* s = A->g->space;
+    feel_hip_ctx* c = s->ctx;
+    if (A->mfState != 1)
+        return setError(FEEL_HIP_EUNSUPPORTED,
+                        "element-by-element inner operator needs a P2 diffusion assembly (symmetric on() only)");
+    if (!A->eop) {
+        A->eop = new ElemOp();
+        int rc = buildStructure(A, A->eop);
+        if (rc != FEEL_HIP_OK) {
+            releaseOp(A);   // buildStructure frees its temporaries, this the arrays it kept
+            return rc;
+        }
+    }
+    if (!A->eop->factorsValid) {
+        if (s->geoStamp != A->geoStamp)
+            return setError(FEEL_HIP_EUNSUPPORTED,
+                            "element-by-element inner operator: another assembly on this space replaced the matrix's "
+                            "element factors; assemble the matrix again");
+        TRY_CALL_EO(snapshotFactors(A));
+    }
+    if (!A->mfRows) {   // no on() since the assembly: no Dirichlet rows
+        TRY_CALL_EO(c->alloc((void**)&A->mfRows, A->g->n));
+        FEEL_HIP_CHECK(hipMemsetAsync(A->mfRows, 0, A->g->n, c->stream));
+        A->eop->flagsDirty = true;
+    }
+    return FEEL_HIP_OK;
+}
 
 // w = A p in the solver numbering (FP32), partials of p.w in part[0..grid)
 int elemOpApply(feel_hip_mat* A, const float* p, float* w, double* part, const CgState* st, int* npart) {
     ElemOp* E = A->eop;
     feel_hip_ctx* c = A->g->space->ctx;
     if (E->flagsDirty) {
-        hipLaunchKernelGGL(kEoRowFlags, dim3(gridOf(E->n)), dim3(kBlock), 0, c->stream, E->n, A->g->perm, E->bold,
+        hipLaunchKernelGGL(kEoRowFlags, dim3(gridOf(E->n)), dim3(kBlock), 0, c->stream, E->n, A->g->perm, A->mfRows,
                            E->brow);
         hipLaunchKernelGGL(kEoColFlags, dim3(gridOf(E->ne * kN)), dim3(kBlock), 0, c->stream, E->ne * kN, E->edof0,
                            E->brow, E->edof);

@@ -1000,6 +1000,7 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
         c->tic("assemble_geo");
         hipLaunchKernelGGL(kGeoElastic, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->coords, s->elems,
                            d->young->d, d->poisson->d, s->geo);
+        ++s->geoStamp;
         c->toc();
         FEEL_HIP_CHECK(hipGetLastError());
         const int grid = segGrid(g);
@@ -1067,6 +1068,7 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     else
         hipLaunchKernelGGL(kGeoScalar<2>, dim3(gridFor(s->ne)), dim3(kBlock), 0, c->stream, s->ne, s->coords, s->elems, kd,
                            kds, km, kms, s->geo);
+    ++s->geoStamp;
     c->toc();
     FEEL_HIP_CHECK(hipGetLastError());
     const int grid = segGrid(g);

@@ -1097,17 +1097,18 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
 }
 
 // y = A x on the 3 x 3 block image of a vector space: one wavefront per slice
-// of 64 node rows, a lane computes its node's three rows; per block slot one
-// column index, nine values (planes of 64, read non-temporally) and three
-// consecutive x entries.  UB block slots per step (4 for FP32, 2 for FP64:
-// measured) keep independent partial sums.  With part, also the block partial
-// of x.y (p.Ap).
-template <typename T, int BS>
+// of 64 / L node rows, L lanes per row each taking every L-th block of the
+// row, summed across the L lanes at the end; per block slot one column index,
+// nine values (planes of 64, read non-temporally) and three consecutive x
+// entries.  UB block slots per step (4 for FP32, 2 for FP64: measured) keep
+// independent partial sums.  With part, also the block partial of x.y (p.Ap).
+template <typename T, int BS, int L = 1>
 __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
                                                      const int32_t* __restrict__ bcol, const T* __restrict__ val,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      double* __restrict__ part, const CgState* __restrict__ st) {
     constexpr int UB = sizeof(T) == 4 ? 4 : 2;
+    constexpr int R = 64 / L;
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
@@ -1148,15 +1149,21 @@ __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nsli
             acc[0][1] += w[192] * x0 + w[256] * x1 + w[320] * x2;
             acc[0][2] += w[384] * x0 + w[448] * x1 + w[512] * x2;
         }
-        const int64_t r = s * 64 + lane;
-        if (r < nb) {
+        T a[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            a[cc] = 0;
+#pragma unroll
+            for (int u = 0; u < UB; ++u) a[cc] += acc[u][cc];
+#pragma unroll
+            for (int o = 1; o < L; o <<= 1) a[cc] += __shfl_xor(a[cc], o);   // the L lanes of the row
+        }
+        const int64_t r = s * R + lane / L;
+        if (r < nb && lane % L == 0) {
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                T a = 0;
-#pragma unroll
-                for (int u = 0; u < UB; ++u) a += acc[u][cc];
-                y[3 * r + cc] = a;
-                if (part) dot += (double)x[3 * r + cc] * (double)a;
+                y[3 * r + cc] = a[cc];
+                if (part) dot += (double)x[3 * r + cc] * (double)a[cc];
             }
         }
     }

@@ -246,47 +246,59 @@ done:
 // ------------------------------------------------------------------------
 // 3 x 3 block image for vector spaces (ncomp = 3, dofs interleaved node * 3 +
 // c, COUPLED pattern: the rows of a node share their columns, which come in
-// triples).  One lane per node row, 64 node rows per slice; block slot
-// (s, k, lane) holds the permuted node column of the lane's k-th block, and its
-// nine values sit in planes, value (c, d) of block slot k at
-// 9 base + (9 k + 3 c + d) 64 + lane.  One column index serves nine values and
-// the x gathers read three consecutive entries.
+// triples).  Two lanes per node row, 32 node rows per slice: the row's k-th
+// block goes to step k / 2 and lane 2 (row % 32) + k % 2, so a slice of S
+// steps holds 64 S block slots; the block slot's permuted node column sits at
+// base + 64 step + lane and its nine values in planes, value (c, d) at
+// 9 base + (9 step + 3 c + d) 64 + lane.  One column index serves nine values,
+// the x gathers read three consecutive entries, and a wave works half as many
+// steps per slice as with one lane per row (the two lanes' sums are added at
+// the end).
 
 namespace {
 
-__global__ __launch_bounds__(kSlBlock) void kBlockSliceSlots(int64_t nb, int64_t nslices, const int32_t* __restrict__ ia,
+// slots of slice s: 64 x the steps of its longest row, a row of len blocks
+// taking ceil(len / L) steps (L lanes per row, 64 / L rows per slice)
+__global__ __launch_bounds__(kSlBlock) void kBlockSliceSlots(int64_t nb, int64_t nslices, int L,
+                                                            const int32_t* __restrict__ ia,
                                                             const int32_t* __restrict__ nperm,
                                                             uint64_t* __restrict__ slots) {
     const int lane = threadIdx.x & 63;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int R = kC / L;
     for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
-        const int64_t r = s * kC + lane;
+        const int64_t r = s * R + lane / L;
         int len = 0;
         if (r < nb) len = (ia[3 * (int64_t)nperm[r] + 1] - ia[3 * (int64_t)nperm[r]]) / 3;
         for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
-        if (lane == 0) slots[s] = (uint64_t)len * kC;
+        if (lane == 0) slots[s] = (uint64_t)((len + L - 1) / L) * kC;
     }
 }
 
-__global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nb, int64_t nrowsPadded, const int32_t* __restrict__ ia,
+// One thread per (padded) node row r of slice s = r / R: its k-th block goes
+// to step k / L, lane (r % R) L + k % L.
+__global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nb, int64_t nrowsPadded, int L,
+                                                          const int32_t* __restrict__ ia,
                                                           const int32_t* __restrict__ ja,
                                                           const int32_t* __restrict__ nperm,
                                                           const int32_t* __restrict__ niperm,
                                                           const uint32_t* __restrict__ slicePtr,
                                                           int32_t* __restrict__ bcol, int32_t* __restrict__ slot2csr,
                                                           int32_t* __restrict__ diagSlot) {
+    const int R = kC / L;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrowsPadded;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = r / kC;
-        const int lane = (int)(r - s * kC);
+        const int64_t s = r / R;
+        const int rr = (int)(r - s * R);
         const uint32_t b = slicePtr[s];
-        const int width = (int)((slicePtr[s + 1] - b) / kC);
+        const int steps = (int)((slicePtr[s + 1] - b) / kC);
         const int32_t self = (int32_t)(r < nb ? r : nb - 1);
         const int32_t old = r < nb ? nperm[r] : 0;
         const int len = r < nb ? (ia[3 * (int64_t)old + 1] - ia[3 * (int64_t)old]) / 3 : 0;
-        for (int k = 0; k < width; ++k) {
-            const int64_t bs = b + (int64_t)k * kC + lane;
-            const size_t v0 = 9 * (size_t)b + (size_t)k * 9 * kC + lane;
+        for (int k = 0; k < steps * L; ++k) {
+            const int m = k / L, lane = rr * L + k % L;
+            const int64_t bs = b + (int64_t)m * kC + lane;
+            const size_t v0 = 9 * (size_t)b + (size_t)m * 9 * kC + lane;
             if (k < len) {
                 const int32_t bj = ja[ia[3 * (int64_t)old] + 3 * k] / 3;
                 bcol[bs] = niperm[bj];
@@ -322,7 +334,9 @@ int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
     feel_hip_ctx* c = g->space->ctx;
     hipStream_t st = c->stream;
     const int64_t nb = g->n / 3;
-    const int64_t nslices = (nb + kC - 1) / kC;
+    const int L = g->blockLanes = kBlockLanes;
+    const int64_t R = kC / L;
+    const int64_t nslices = (nb + R - 1) / R;
     int rc = FEEL_HIP_OK;
     int32_t *dnperm = nullptr, *niperm = nullptr;
     uint64_t *cnt = nullptr, *off = nullptr;
@@ -341,8 +355,8 @@ int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
     SL_CHECK(hipMalloc(&cnt, sizeof(uint64_t) * (nslices + 1)));
     SL_CHECK(hipMalloc(&off, sizeof(uint64_t) * (nslices + 1)));
     SL_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint64_t) * (nslices + 1), st));
-    hipLaunchKernelGGL(kBlockSliceSlots, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nb, nslices, g->ia, dnperm,
-                       cnt);
+    hipLaunchKernelGGL(kBlockSliceSlots, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nb, nslices, L, g->ia,
+                       dnperm, cnt);
     SL_CHECK(hipGetLastError());
     SL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, (int)(nslices + 1), st));
     SL_CHECK(hipMalloc(&tmp, need));
@@ -358,7 +372,7 @@ int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->sdiag, sizeof(int32_t) * g->n);
     if (rc != FEEL_HIP_OK) goto done;
     hipLaunchKernelGGL(kSlicePtr, dim3(slGrid(nslices + 1)), dim3(kSlBlock), 0, st, nslices + 1, off, g->slicePtr);
-    hipLaunchKernelGGL(kBlockSellFill, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nb, nslices * kC, g->ia,
+    hipLaunchKernelGGL(kBlockSellFill, dim3(slGrid(nslices * R)), dim3(kSlBlock), 0, st, nb, nslices * R, L, g->ia,
                        g->ja, dnperm, niperm, g->slicePtr, g->scol, g->slot2csr, g->sdiag);
     SL_CHECK(hipGetLastError());
     SL_CHECK(hipStreamSynchronize(st));

@@ -47,16 +47,16 @@ void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, int grid, const T* val
 template <typename T>
 void launchShape(feel_hip_ctx* c, const feel_hip_graph* g, const SpmvShape& sh, const T* val, const T* x, T* y,
                  double* part, const CgState* st) {
-    if (g->block) {
+    if (g->block) {   // the block image has kBlockLanes lanes per node row
         if (sh.block == 1024)
-            hipLaunchKernelGGL((kBlockSellSpmv<T, 1024>), dim3(sh.grid), dim3(1024), 0, c->stream, g->n, g->nslices,
-                               g->slicePtr, g->scol, val, x, y, part, st);
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 1024, kBlockLanes>), dim3(sh.grid), dim3(1024), 0, c->stream, g->n,
+                               g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
         else if (sh.block == 512)
-            hipLaunchKernelGGL((kBlockSellSpmv<T, 512>), dim3(sh.grid), dim3(512), 0, c->stream, g->n, g->nslices,
-                               g->slicePtr, g->scol, val, x, y, part, st);
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 512, kBlockLanes>), dim3(sh.grid), dim3(512), 0, c->stream, g->n,
+                               g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
         else
-            hipLaunchKernelGGL((kBlockSellSpmv<T, 256>), dim3(sh.grid), dim3(256), 0, c->stream, g->n, g->nslices,
-                               g->slicePtr, g->scol, val, x, y, part, st);
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 256, kBlockLanes>), dim3(sh.grid), dim3(256), 0, c->stream, g->n,
+                               g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
         return;
     }
     if (g->compact) {
@@ -492,9 +492,7 @@ int feel_hip_solve_pcg(feel_hip_mat* A, const feel_hip_vec* b, feel_hip_vec* x, 
     TRY_CALL(allocWork(c, &A->xp64, n));
     if (o.inner_operator < FEEL_HIP_INNER_AUTO || o.inner_operator > FEEL_HIP_INNER_ELEMENT)
         return setError(FEEL_HIP_EINVAL, "unknown inner operator");
-    if (o.inner_operator == FEEL_HIP_INNER_ELEMENT && !elemOpReady(A))
-        return setError(FEEL_HIP_EUNSUPPORTED,
-                        "element-by-element inner operator needs a P2 diffusion assembly (symmetric on() only)");
+    if (o.inner_operator == FEEL_HIP_INNER_ELEMENT) TRY_CALL(elemOpPrepare(A));
     // AUTO takes the assembled FP32 values: on a P2 Poisson matrix the element
     // operator moves about as many bytes per product (its element results are
     // written, then gathered by row) and takes longer

@@ -354,3 +354,30 @@ def test_element_operator_follows_the_values(ctx, orc):
     A.assemble(fa.DIFFUSION | fa.MASS, 4, kdiff=kd, mass=1.0)
     with pytest.raises(fa.FeelHipError):
         A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
+
+
+def test_element_operator_needs_the_matrix_own_factors(ctx, orc):
+    # the factors are taken from the assembly's per-element scratch on the first
+    # element-operator solve: an assembly of another matrix on the same space in
+    # between replaces them, and the matrix must be assembled again
+    P = problem(ctx, orc, 3, 2, 4, 4, orc.FK_CONST, 2, 0.15, 12)
+    A, bb, n = P["A"], P["bb"], P["ndof"]
+    space, g, kd, rows = P["keep"]
+    B = fa.Matrix(g, fa.FP64_FP32)
+    B.assemble(fa.DIFFUSION, 4)
+    x = ctx.vector(n)
+    with pytest.raises(fa.FeelHipError):
+        A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
+    A.assemble(fa.DIFFUSION, 4, kdiff=kd)
+    A.zero_rows(rows, None, None, mode=fa.ELIMINATION_SYMMETRIC)
+    st = A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
+    xt, _, _, _ = orc.pcg(P["ia"], P["ja"], P["val"], P["b"], rtol=1e-14, maxit=10000)
+    assert st["converged"] and rel_fro(x.get(), xt) < 1e-8
+    # once used, later assemblies keep their own copy: B's assembly no longer matters
+    B.assemble(fa.DIFFUSION, 4)
+    A.assemble(fa.DIFFUSION, 4, kdiff=kd)
+    A.zero_rows(rows, None, None, mode=fa.ELIMINATION_SYMMETRIC)
+    B.assemble(fa.DIFFUSION, 4)
+    x.zero()
+    st = A.solve(bb, x, rtol=1e-10, maxit=5000, precision=fa.FP32, inner_operator=fa.INNER_ELEMENT)
+    assert st["converged"] and rel_fro(x.get(), xt) < 1e-8

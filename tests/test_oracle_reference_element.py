@@ -214,3 +214,16 @@ def test_p4_solution_independent_of_face_node_position(orc, t):
     fa = np.setdiff1d(np.unique(e2d[:, 22:34]), np.nonzero(isb)[0])   # interior face dofs
     assert np.abs(x_t[ve] - x_ref[ve]).max() < 1e-10 * np.abs(x_ref).max()
     assert np.abs(x_t[fa] - x_ref[fa]).max() > 1e-6 * np.abs(x_ref).max()   # the bases do differ
+
+
+def test_quadrature_table_copies_agree():
+    # the oracle and the product each compile a copy of the quadrature tables
+    # transcribed from imsimplex.cpp by gen_im_tables.py; apart from the line
+    # naming the copy they must be the same data
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    a = open(os.path.join(root, "oracle", "im_simplex_tables.inc")).read().splitlines()
+    b = open(os.path.join(root, "feelpp_amd", "csrc", "host", "im_simplex_tables.inc")).read().splitlines()
+    assert len(a) == len(b) > 100
+    diff = [i for i, (x, y) in enumerate(zip(a, b)) if x != y]
+    assert len(diff) <= 1 and all("Generated by" in a[i] for i in diff), diff
