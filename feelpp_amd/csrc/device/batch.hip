@@ -188,6 +188,30 @@ __global__ __launch_bounds__(kBlock) void kBMask(int64_t n, int64_t nslices, con
     }
 }
 
+// Per slice, the terms with a nonzero FP32 value on a kept slot (mask 0): bit t
+// of tmask[s].  A term that is zero over a slice adds only zeros to every
+// a = sum_t mu_t a_t there, so the grouped SpMV does not read its plane.
+struct TermPtrs {
+    const float* t[kMaxTerms];
+};
+__global__ __launch_bounds__(kBlock) void kBTermMask(int64_t nslices, int nterms, const uint32_t* __restrict__ slicePtr,
+                                                    TermPtrs T, const uint8_t* __restrict__ mask,
+                                                    uint8_t* __restrict__ tmask) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * kBlock) >> 6;
+    for (int64_t s = wave; s < nslices; s += nwaves) {
+        const uint32_t b0 = slicePtr[s], b1 = slicePtr[s + 1];
+        unsigned m = 0;
+        for (int t = 0; t < nterms; ++t) {
+            bool nz = false;
+            for (uint32_t q = b0 + lane; q < b1; q += 64) nz |= mask[q] == 0 && T.t[t][q] != 0.0f;
+            if (__ballot(nz)) m |= 1u << t;
+        }
+        if (lane == 0) tmask[s] = (uint8_t)m;
+    }
+}
+
 // permuted rhs with zero Dirichlet rows
 __global__ __launch_bounds__(kBlock) void kBRhs(int64_t n, const int32_t* __restrict__ perm, const uint8_t* __restrict__ flag,
                                                const double* __restrict__ b, double* __restrict__ out) {
@@ -299,6 +323,7 @@ struct BGroup {
     float mu[kGroupMax][kMaxTerms];
     const float* terms[kMaxTerms];       // family terms, FP32 SELL values
     const uint8_t* mask;
+    const uint8_t* tmask;                // per slice: terms nonzero there (bit t)
     SellCols cols;
 };
 struct BGItem {
@@ -317,7 +342,8 @@ struct GroupQuad {
 };
 
 template <int NT, bool NARROW>
-__device__ inline void loadQuad(const BGroup& G, size_t q0, const void* cp, int32_t cb, int kq, GroupQuad<NT>& d) {
+__device__ inline void loadQuad(const BGroup& G, unsigned tm, size_t q0, const void* cp, int32_t cb, int kq,
+                                GroupQuad<NT>& d) {
     // q0: slot of the lane's first quad; quad kq starts 256 kq slots further
     const size_t q = q0 + ((size_t)kq << 8);
     if (NARROW) {
@@ -332,7 +358,9 @@ __device__ inline void loadQuad(const BGroup& G, size_t q0, const void* cp, int3
     float tv[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const f32x4 v = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(G.terms[t] + q));
+        // tm is wave-uniform: a term that is zero over the slice is not read
+        const f32x4 v = (tm >> t) & 1u ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(G.terms[t] + q))
+                                       : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         tv[t][0] = v.x; tv[t][1] = v.y; tv[t][2] = v.z; tv[t][3] = v.w;
     }
 #pragma unroll
@@ -353,38 +381,43 @@ __device__ inline void groupP(const float* __restrict__ P, int64_t r, float* pv)
 
 // acc_b += A(mu_b) p_b on the quad for every active instance b of the group
 template <int NT>
-__device__ inline void applyQuad(const BGroup& G, unsigned amask, const GroupQuad<NT>& d, float* acc) {
+__device__ inline void applyQuad(const BGroup& G, unsigned amask, unsigned tm, const GroupQuad<NT>& d, float* acc) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         float pv[kGroupMax];
         groupP(G.P, d.col[u], pv);
+        float v[kGroupMax];
 #pragma unroll
-        for (int b = 0; b < kGroupMax; ++b) {
-            if ((amask >> b) & 1u) {
-                float v = d.e[u];
+        for (int b = 0; b < kGroupMax; ++b) v[b] = d.e[u];
+        // terms in order; one that is zero over the slice (tm, wave-uniform)
+        // would only add zeros and is skipped
 #pragma unroll
-                for (int t = 0; t < NT; ++t) v += G.mu[b][t] * d.a[u][t];
-                acc[b] += v * pv[b];
-            }
-        }
+        for (int t = 0; t < NT; ++t)
+            if ((tm >> t) & 1u)
+#pragma unroll
+                for (int b = 0; b < kGroupMax; ++b)
+                    if ((amask >> b) & 1u) v[b] += G.mu[b][t] * d.a[u][t];
+#pragma unroll
+        for (int b = 0; b < kGroupMax; ++b)
+            if ((amask >> b) & 1u) acc[b] += v[b] * pv[b];
     }
 }
 
 // one slice row, the next quad's matrix data loaded while the current quad's
 // gathers and products run
 template <int NT, bool NARROW>
-__device__ inline void groupRow(const BGroup& G, unsigned amask, size_t q0, const void* cp, int32_t cb, int nq,
-                                float* acc) {
+__device__ inline void groupRow(const BGroup& G, unsigned amask, unsigned tm, size_t q0, const void* cp, int32_t cb,
+                                int nq, float* acc) {
     if (nq <= 0) return;
     GroupQuad<NT> cur;
-    loadQuad<NT, NARROW>(G, q0, cp, cb, 0, cur);
+    loadQuad<NT, NARROW>(G, tm, q0, cp, cb, 0, cur);
     for (int k = 1; k < nq; ++k) {
         GroupQuad<NT> nxt;
-        loadQuad<NT, NARROW>(G, q0, cp, cb, k, nxt);
-        applyQuad<NT>(G, amask, cur, acc);
+        loadQuad<NT, NARROW>(G, tm, q0, cp, cb, k, nxt);
+        applyQuad<NT>(G, amask, tm, cur, acc);
         cur = nxt;
     }
-    applyQuad<NT>(G, amask, cur, acc);
+    applyQuad<NT>(G, amask, tm, cur, acc);
 }
 
 // Grouped SpMV of an affine family (FP32 inner iterations): the in-flight
@@ -426,14 +459,15 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b) acc[b] = 0.0f;
         const int32_t cb = G.compact ? G.cols.colBase[sl] : -1;
+        const unsigned tm = __builtin_amdgcn_readfirstlane((unsigned)G.tmask[sl]);
         if (cb >= 0)
-            groupRow<NT, true>(G, amask, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb,
+            groupRow<NT, true>(G, amask, tm, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb,
                                nq, acc);
         else if (G.compact)
-            groupRow<NT, false>(G, amask, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
+            groupRow<NT, false>(G, amask, tm, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
                                 cb, nq, acc);
         else
-            groupRow<NT, false>(G, amask, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
+            groupRow<NT, false>(G, amask, tm, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
         const int64_t r = (int64_t)sl * 64 + lane;
         if (r < G.n) {
             float pr[kGroupMax];
@@ -689,6 +723,7 @@ struct BatchFamily {
     std::vector<double*> terms;   // SELL FP64 values per term
     std::vector<float*> termsF;   // the same rounded to FP32 (grouped FP32 SpMV)
     uint8_t* mask = nullptr;
+    uint8_t* tmask = nullptr;     // per slice: FP32 terms nonzero on a kept slot (grouped SpMV)
     double* b64 = nullptr;        // permuted rhs with zero Dirichlet rows
     int64_t n = 0, nslices = 0, nslots = 0;
     int nitems = 0;
@@ -917,6 +952,7 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
             }
             for (int t = 0; t < kMaxTerms; ++t) G.terms[t] = f.termsF[t < f.nterms ? t : 0];
             G.mask = f.mask;
+            G.tmask = f.tmask;
             G.cols = SellCols{f.g->slicePtr, f.g->scol, f.g->colBase, f.g->ccptr, f.g->packed};
             const int g = (int)grp.size();
             grp.push_back(G);
@@ -1374,6 +1410,7 @@ int feel_hip_batch_destroy(feel_hip_batch* B) {
         for (double* p : f.terms) c->release(p, sizeof(double) * f.nslots);
         for (float* p : f.termsF) c->release(p, sizeof(float) * f.nslots);
         c->release(f.mask, f.nslots);
+        if (f.tmask) c->release(f.tmask, f.nslices);
         c->release(f.b64, sizeof(double) * f.n);
         unref(f.g);
     }
@@ -1435,6 +1472,14 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
     TRY_CALL(alloc(c, &f.mask, f.nslots));
     hipLaunchKernelGGL(kBMask, dim3(grid), dim3(kBlock), 0, c->stream, f.n, f.nslices, g->slicePtr, g->scol, g->slot2csr,
                        g->perm, flags, f.mask);
+    if (!f.termsF.empty()) {
+        TermPtrs T{};
+        for (int t = 0; t < nterms; ++t) T.t[t] = f.termsF[t];
+        TRY_CALL(alloc(c, &f.tmask, f.nslices));
+        const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>((f.nslices * 64 + kBlock - 1) / kBlock, 8192));
+        hipLaunchKernelGGL(kBTermMask, dim3(sgrid), dim3(kBlock), 0, c->stream, f.nslices, nterms, g->slicePtr, T, f.mask,
+                           f.tmask);
+    }
     TRY_CALL(alloc(c, &f.b64, f.n));
     hipLaunchKernelGGL(kBRhs, dim3(grid), dim3(kBlock), 0, c->stream, f.n, g->perm, flags, rhs->d, f.b64);
     FEEL_HIP_CHECK(hipGetLastError());
