@@ -39,7 +39,11 @@ constexpr int kSpmvBlock = 1024;
 constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
 constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
 constexpr int kSegTask = 256;         // scatter entries per task (whole slots)
-constexpr int kBlockLanes = 2;        // lanes per node row of the 3 x 3 block image (measured: 1, 2, 4)
+// 3 x 3 block image: lanes per node row, chosen per slice: 2 (32 rows per
+// slice) unless the slice's longest row would then take more than
+// kBlockStepCap steps, then 4 or 8 (16 or 8 rows per slice)
+constexpr int kBlockMinShift = 1, kBlockMaxShift = 3;
+constexpr int kBlockStepCap = 48;
 
 }  // namespace feelpp_amd
 
@@ -54,7 +58,7 @@ int buildIncidenceDevice(feel_hip_space* s);
 // the solver's SELL-C-sigma image of g for the row order perm (sell.hip)
 int buildSellDevice(feel_hip_graph* g, const std::vector<int32_t>& perm);
 // the 3 x 3 block image of a vector-space graph for the node order nperm (sell.hip)
-int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm);
+int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm, const std::vector<int32_t>& nodeLen);
 // device address of the static staging array of feel_hip_mat_add_matrix_noalloc,
 // looked up once per device with that device current (noalloc.hip)
 int noallocSetup(int device);
@@ -174,7 +178,8 @@ struct feel_hip_graph {
     bool compact = false;         // packed 16/32-bit column stream available
     bool block = false;           // 3 x 3 block image (vector spaces): slicePtr/scol in block slots
     int64_t nbslots = 0;          // block slots of the block image (nslots = 9 nbslots value slots)
-    int blockLanes = 2;           // block image: lanes per node row (64 / blockLanes rows per slice)
+    int32_t* sliceRow = nullptr;  // block image: first node row of each slice (nslices + 1)
+    uint8_t* sliceShift = nullptr;  // block image: log2 of the lanes per node row of each slice
     int32_t* colBase = nullptr;   // nslices, -1 = slice stores int32 columns
     int64_t* ccptr = nullptr;     // nslices, slice start in packed (uint16 units)
     uint16_t* packed = nullptr;   // npacked

@@ -176,6 +176,8 @@ void unref(feel_hip_graph* g) {
     c->release(g->taskK, sizeof(uint32_t) * (g->ntasks + 1));
     if (g->segSlot) c->release(g->segSlot, sizeof(int32_t) * g->nseg);
     c->release(g->slicePtr, sizeof(uint32_t) * (g->nslices + 1));
+    if (g->sliceRow) c->release(g->sliceRow, sizeof(int32_t) * (g->nslices + 1));
+    if (g->sliceShift) c->release(g->sliceShift, sizeof(uint8_t) * g->nslices);
     c->release(g->scol, sizeof(int32_t) * (g->block ? g->nbslots : g->nslots));
     c->release(g->slot2csr, sizeof(int32_t) * g->nslots);
     c->release(g->perm, sizeof(int32_t) * g->n);
@@ -535,7 +537,9 @@ int finishGraph(feel_hip_graph* g) {
         ng.ja.resize(ng.nnz);
         for (int64_t i = 0; i < nb; ++i)
             for (int32_t p = ia[3 * i], q = ng.ia[i]; p < ia[3 * i + 1]; p += 3, ++q) ng.ja[q] = ja[p] / 3;
-        TRY_CALL(buildBlockSellDevice(g, sellPermutation(ng, 1024)));
+        std::vector<int32_t> nodeLen(nb);
+        for (int64_t i = 0; i < nb; ++i) nodeLen[i] = ng.ia[i + 1] - ng.ia[i];
+        TRY_CALL(buildBlockSellDevice(g, sellPermutation(ng, 1024), nodeLen));
         return FEEL_HIP_OK;
     }
     GraphCSR gc;   // borrows the host pattern for the row ordering

@@ -1096,28 +1096,30 @@ __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, 
     }
 }
 
-// y = A x on the 3 x 3 block image of a vector space: one wavefront per slice
-// of 64 / L node rows, L lanes per row each taking every L-th block of the
-// row, summed across the L lanes at the end; per block slot one column index,
-// nine values (planes of 64, read non-temporally) and three consecutive x
-// entries.  UB block slots per step (4 for FP32, 2 for FP64: measured) keep
-// independent partial sums.  With part, also the block partial of x.y (p.Ap).
-template <typename T, int BS, int L = 1>
-__global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nslices, const uint32_t* __restrict__ slicePtr,
+// y = A x on the 3 x 3 block image of a vector space: one wavefront per slice,
+// L = 2^sliceShift[s] lanes per node row (64 / L rows from sliceRow[s]), each
+// lane taking every L-th block of its row, the L lanes' sums added at the end;
+// per block slot one column index, nine values (planes of 64, read
+// non-temporally) and three consecutive x entries.  UB block slots per step (4
+// for FP32, 2 for FP64: measured) keep independent partial sums.  With part,
+// also the block partial of x.y (p.Ap).
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nslices, const uint32_t* __restrict__ slicePtr,
+                                                     const int32_t* __restrict__ sliceRow,
+                                                     const uint8_t* __restrict__ sliceShift,
                                                      const int32_t* __restrict__ bcol, const T* __restrict__ val,
                                                      const T* __restrict__ x, T* __restrict__ y,
                                                      double* __restrict__ part, const CgState* __restrict__ st) {
     constexpr int UB = sizeof(T) == 4 ? 4 : 2;
-    constexpr int R = 64 / L;
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * BS) >> 6;
-    const int64_t nb = nrows / 3;
     double dot = 0.0;
     for (int64_t s = wave; s < nslices; s += nwaves) {
         const uint32_t base = slicePtr[s];
         const int width = (int)((slicePtr[s + 1] - base) >> 6);
+        const int sh = __builtin_amdgcn_readfirstlane((int)sliceShift[s]);
         const int32_t* c = bcol + base + lane;
         const T* v = val + 9 * (size_t)base + lane;
         T acc[UB][3];
@@ -1155,11 +1157,10 @@ __global__ __launch_bounds__(BS) void kBlockSellSpmv(int64_t nrows, int64_t nsli
             a[cc] = 0;
 #pragma unroll
             for (int u = 0; u < UB; ++u) a[cc] += acc[u][cc];
-#pragma unroll
-            for (int o = 1; o < L; o <<= 1) a[cc] += __shfl_xor(a[cc], o);   // the L lanes of the row
+            for (int o = 1; o < (1 << sh); o <<= 1) a[cc] += __shfl_xor(a[cc], o);   // the L lanes of the row
         }
-        const int64_t r = s * R + lane / L;
-        if (r < nb && lane % L == 0) {
+        const int64_t r = (int64_t)sliceRow[s] + (lane >> sh);
+        if (r < sliceRow[s + 1] && (lane & ((1 << sh) - 1)) == 0) {
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
                 y[3 * r + cc] = a[cc];

@@ -246,38 +246,44 @@ done:
 // ------------------------------------------------------------------------
 // 3 x 3 block image for vector spaces (ncomp = 3, dofs interleaved node * 3 +
 // c, COUPLED pattern: the rows of a node share their columns, which come in
-// triples).  Two lanes per node row, 32 node rows per slice: the row's k-th
-// block goes to step k / 2 and lane 2 (row % 32) + k % 2, so a slice of S
-// steps holds 64 S block slots; the block slot's permuted node column sits at
+// triples).  Slice s holds the node rows sliceRow[s] .. sliceRow[s+1] - 1 with
+// L = 2^sliceShift[s] lanes per row (64 / L rows): the row's k-th block goes to
+// step k / L and lane L (row - sliceRow[s]) + k % L, so a slice of S steps
+// holds 64 S block slots; the block slot's permuted node column sits at
 // base + 64 step + lane and its nine values in planes, value (c, d) at
 // 9 base + (9 step + 3 c + d) 64 + lane.  One column index serves nine values,
-// the x gathers read three consecutive entries, and a wave works half as many
-// steps per slice as with one lane per row (the two lanes' sums are added at
-// the end).
+// the x gathers read three consecutive entries, and the L lanes of a row add
+// their sums at the end.  Rows are ordered longest first inside windows, and a
+// slice whose longest row would take more than kBlockStepCap steps with two
+// lanes takes four or eight, so no wavefront streams far longer than the rest
+// (P4 tetrahedra: rows of 35 to 369 blocks).
 
 namespace {
 
 // slots of slice s: 64 x the steps of its longest row, a row of len blocks
-// taking ceil(len / L) steps (L lanes per row, 64 / L rows per slice)
-__global__ __launch_bounds__(kSlBlock) void kBlockSliceSlots(int64_t nb, int64_t nslices, int L,
+// taking ceil(len / L) steps
+__global__ __launch_bounds__(kSlBlock) void kBlockSliceSlots(int64_t nslices, const int32_t* __restrict__ sliceRow,
+                                                            const uint8_t* __restrict__ sliceShift,
                                                             const int32_t* __restrict__ ia,
                                                             const int32_t* __restrict__ nperm,
                                                             uint64_t* __restrict__ slots) {
     const int lane = threadIdx.x & 63;
     const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    const int R = kC / L;
     for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
-        const int64_t r = s * R + lane / L;
+        const int sh = sliceShift[s], L = 1 << sh;
+        const int64_t r = (int64_t)sliceRow[s] + (lane >> sh);
         int len = 0;
-        if (r < nb) len = (ia[3 * (int64_t)nperm[r] + 1] - ia[3 * (int64_t)nperm[r]]) / 3;
+        if ((lane & (L - 1)) == 0 && r < sliceRow[s + 1])
+            len = (ia[3 * (int64_t)nperm[r] + 1] - ia[3 * (int64_t)nperm[r]]) / 3;
         for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
         if (lane == 0) slots[s] = (uint64_t)((len + L - 1) / L) * kC;
     }
 }
 
-// One thread per (padded) node row r of slice s = r / R: its k-th block goes
-// to step k / L, lane (r % R) L + k % L.
-__global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nb, int64_t nrowsPadded, int L,
+// One wavefront per slice, one thread per lane of the slice: lane L i + j
+// fills the blocks k = m L + j (m = 0 .. steps - 1) of the slice's row i.
+__global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nslices, const int32_t* __restrict__ sliceRow,
+                                                          const uint8_t* __restrict__ sliceShift,
                                                           const int32_t* __restrict__ ia,
                                                           const int32_t* __restrict__ ja,
                                                           const int32_t* __restrict__ nperm,
@@ -285,18 +291,20 @@ __global__ __launch_bounds__(kSlBlock) void kBlockSellFill(int64_t nb, int64_t n
                                                           const uint32_t* __restrict__ slicePtr,
                                                           int32_t* __restrict__ bcol, int32_t* __restrict__ slot2csr,
                                                           int32_t* __restrict__ diagSlot) {
-    const int R = kC / L;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrowsPadded;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t s = r / R;
-        const int rr = (int)(r - s * R);
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
+        const int sh = sliceShift[s], L = 1 << sh, j = lane & (L - 1);
+        const int64_t r0 = sliceRow[s], r1 = sliceRow[s + 1];
+        const int64_t r = r0 + (lane >> sh);
+        const bool real = r < r1;
         const uint32_t b = slicePtr[s];
         const int steps = (int)((slicePtr[s + 1] - b) / kC);
-        const int32_t self = (int32_t)(r < nb ? r : nb - 1);
-        const int32_t old = r < nb ? nperm[r] : 0;
-        const int len = r < nb ? (ia[3 * (int64_t)old + 1] - ia[3 * (int64_t)old]) / 3 : 0;
-        for (int k = 0; k < steps * L; ++k) {
-            const int m = k / L, lane = rr * L + k % L;
+        const int32_t self = (int32_t)(real ? r : r1 - 1);   // padding reads a row of this slice
+        const int32_t old = real ? nperm[r] : 0;
+        const int len = real ? (ia[3 * (int64_t)old + 1] - ia[3 * (int64_t)old]) / 3 : 0;
+        for (int m = 0; m < steps; ++m) {
+            const int k = m * L + j;
             const int64_t bs = b + (int64_t)m * kC + lane;
             const size_t v0 = 9 * (size_t)b + (size_t)m * 9 * kC + lane;
             if (k < len) {
@@ -327,16 +335,43 @@ __global__ __launch_bounds__(kSlBlock) void kExpandPerm(int64_t nb, const int32_
 
 namespace feelpp_amd {
 
-// Fills the block image (g->block, perm, slicePtr in block slots, scol = block
-// columns, slot2csr and sdiag over value slots, nslots = value slots) from the
-// device ia/ja and the host node order nperm (new -> old node).
-int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
+// Slices of the block image over the node order nperm: each takes the next
+// 64 / L rows, L = 2 unless the longest of them would take more than
+// kBlockStepCap steps (then 4, then 8).  Fills the first row of each slice
+// (and nb at the end) and log2 L per slice.
+static void blockSlices(const std::vector<int32_t>& nperm, const std::vector<int32_t>& nodeLen, std::vector<int32_t>& row0,
+                 std::vector<uint8_t>& shift) {
+    const int64_t nb = (int64_t)nperm.size();
+    row0.clear();
+    shift.clear();
+    for (int64_t r = 0; r < nb;) {
+        int sh = kBlockMinShift;
+        for (;; ++sh) {
+            const int L = 1 << sh;
+            const int64_t r1 = std::min<int64_t>(nb, r + kC / L);
+            int maxlen = 0;
+            for (int64_t i = r; i < r1; ++i) maxlen = std::max(maxlen, nodeLen[nperm[i]]);
+            if ((maxlen + L - 1) / L <= kBlockStepCap || sh == kBlockMaxShift) break;
+        }
+        row0.push_back((int32_t)r);
+        shift.push_back((uint8_t)sh);
+        r += kC >> sh;
+    }
+    row0.push_back((int32_t)nb);
+}
+
+// Fills the block image (g->block, perm, sliceRow/sliceShift, slicePtr in
+// block slots, scol = block columns, slot2csr and sdiag over value slots,
+// nslots = value slots) from the device ia/ja, the host node order nperm
+// (new -> old node) and the node row lengths in blocks (old numbering).
+int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm, const std::vector<int32_t>& nodeLen) {
     feel_hip_ctx* c = g->space->ctx;
     hipStream_t st = c->stream;
     const int64_t nb = g->n / 3;
-    const int L = g->blockLanes = kBlockLanes;
-    const int64_t R = kC / L;
-    const int64_t nslices = (nb + R - 1) / R;
+    std::vector<int32_t> row0;
+    std::vector<uint8_t> shift;
+    blockSlices(nperm, nodeLen, row0, shift);
+    const int64_t nslices = (int64_t)shift.size();
     int rc = FEEL_HIP_OK;
     int32_t *dnperm = nullptr, *niperm = nullptr;
     uint64_t *cnt = nullptr, *off = nullptr;
@@ -346,7 +381,11 @@ int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
     g->block = true;
     g->nslices = nslices;
     rc = c->alloc((void**)&g->perm, sizeof(int32_t) * g->n);
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->sliceRow, sizeof(int32_t) * (nslices + 1));
+    if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->sliceShift, sizeof(uint8_t) * nslices);
     if (rc != FEEL_HIP_OK) return rc;
+    SL_CHECK(hipMemcpyAsync(g->sliceRow, row0.data(), sizeof(int32_t) * (nslices + 1), hipMemcpyHostToDevice, st));
+    SL_CHECK(hipMemcpyAsync(g->sliceShift, shift.data(), sizeof(uint8_t) * nslices, hipMemcpyHostToDevice, st));
     SL_CHECK(hipMalloc(&dnperm, sizeof(int32_t) * nb));
     SL_CHECK(hipMemcpyAsync(dnperm, nperm.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, st));
     SL_CHECK(hipMalloc(&niperm, sizeof(int32_t) * nb));
@@ -355,8 +394,8 @@ int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
     SL_CHECK(hipMalloc(&cnt, sizeof(uint64_t) * (nslices + 1)));
     SL_CHECK(hipMalloc(&off, sizeof(uint64_t) * (nslices + 1)));
     SL_CHECK(hipMemsetAsync(cnt, 0, sizeof(uint64_t) * (nslices + 1), st));
-    hipLaunchKernelGGL(kBlockSliceSlots, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nb, nslices, L, g->ia,
-                       dnperm, cnt);
+    hipLaunchKernelGGL(kBlockSliceSlots, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nslices, g->sliceRow,
+                       g->sliceShift, g->ia, dnperm, cnt);
     SL_CHECK(hipGetLastError());
     SL_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, need, cnt, off, (int)(nslices + 1), st));
     SL_CHECK(hipMalloc(&tmp, need));
@@ -372,8 +411,8 @@ int buildBlockSellDevice(feel_hip_graph* g, const std::vector<int32_t>& nperm) {
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&g->sdiag, sizeof(int32_t) * g->n);
     if (rc != FEEL_HIP_OK) goto done;
     hipLaunchKernelGGL(kSlicePtr, dim3(slGrid(nslices + 1)), dim3(kSlBlock), 0, st, nslices + 1, off, g->slicePtr);
-    hipLaunchKernelGGL(kBlockSellFill, dim3(slGrid(nslices * R)), dim3(kSlBlock), 0, st, nb, nslices * R, L, g->ia,
-                       g->ja, dnperm, niperm, g->slicePtr, g->scol, g->slot2csr, g->sdiag);
+    hipLaunchKernelGGL(kBlockSellFill, dim3(slGrid(nslices * kC)), dim3(kSlBlock), 0, st, nslices, g->sliceRow,
+                       g->sliceShift, g->ia, g->ja, dnperm, niperm, g->slicePtr, g->scol, g->slot2csr, g->sdiag);
     SL_CHECK(hipGetLastError());
     SL_CHECK(hipStreamSynchronize(st));
 done:

@@ -47,16 +47,16 @@ void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, int grid, const T* val
 template <typename T>
 void launchShape(feel_hip_ctx* c, const feel_hip_graph* g, const SpmvShape& sh, const T* val, const T* x, T* y,
                  double* part, const CgState* st) {
-    if (g->block) {   // the block image has kBlockLanes lanes per node row
+    if (g->block) {   // the block image: per-slice lanes per node row
         if (sh.block == 1024)
-            hipLaunchKernelGGL((kBlockSellSpmv<T, 1024, kBlockLanes>), dim3(sh.grid), dim3(1024), 0, c->stream, g->n,
-                               g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 1024>), dim3(sh.grid), dim3(1024), 0, c->stream, g->nslices,
+                               g->slicePtr, g->sliceRow, g->sliceShift, g->scol, val, x, y, part, st);
         else if (sh.block == 512)
-            hipLaunchKernelGGL((kBlockSellSpmv<T, 512, kBlockLanes>), dim3(sh.grid), dim3(512), 0, c->stream, g->n,
-                               g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 512>), dim3(sh.grid), dim3(512), 0, c->stream, g->nslices,
+                               g->slicePtr, g->sliceRow, g->sliceShift, g->scol, val, x, y, part, st);
         else
-            hipLaunchKernelGGL((kBlockSellSpmv<T, 256, kBlockLanes>), dim3(sh.grid), dim3(256), 0, c->stream, g->n,
-                               g->nslices, g->slicePtr, g->scol, val, x, y, part, st);
+            hipLaunchKernelGGL((kBlockSellSpmv<T, 256>), dim3(sh.grid), dim3(256), 0, c->stream, g->nslices,
+                               g->slicePtr, g->sliceRow, g->sliceShift, g->scol, val, x, y, part, st);
         return;
     }
     if (g->compact) {
