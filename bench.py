@@ -358,22 +358,38 @@ def bench_beam(fa, args):
     print(json.dumps(line), flush=True)
 
 
+def memory_reference():
+    """A plain 1 GiB device copy, 5 timed launches after one warm-up (1 GiB
+    read + 1 GiB written each): it calibrates the HBM byte counters of the
+    FETCH_SIZE / WRITE_SIZE passes and gives a copy rate beside the kernels."""
+    import torch
+    nbytes = 1 << 30
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(5):
+        b.copy_(a)
+    ev[1].record()
+    torch.cuda.synchronize()
+    out = {"copy_TBps": round(2 * nbytes / (ev[0].elapsed_time(ev[1]) / 5e3) / 1e12, 3)}
+    log(f"copy kernel: {nbytes} bytes read + {nbytes} bytes written per launch, 6 launches; "
+        f"{out['copy_TBps']} TB/s")
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_kernels(fa, args):
     """Short fixed runs of the hot kernels for hardware-counter passes: a plain
     device copy of known size (calibrates the HBM byte counters), then on the
     C2 problem 3 assemblies (scatter kernel) and 20-iteration solves: FP64 (SELL
     SpMV), mixed with the assembled FP32 matrix (FP32 SELL SpMV) and mixed with
     the element-by-element FP32 operator."""
-    import torch
-    nbytes = 1 << 30
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    log(f"copy kernel: {nbytes} bytes read + {nbytes} bytes written per launch, 5 launches")
-    del a, b
+    stream = memory_reference()
     order, qorder = args.order, 2 * args.order
     ctx = fa.Context(0)
     prob = Problem(fa, ctx, build_problem(fa, ctx, args.n, order, 0.15, 2024), order, qorder)
@@ -388,8 +404,9 @@ def bench_kernels(fa, args):
                           inner_operator=op)
         log(f"precision {prec} inner operator {op}: {st}")
     ctx.synchronize()
-    print(json.dumps({"kernels": "copy 1 GiB x5; C2 assembly x3; solves of 20 iterations: FP64, mixed with "
-                                 "the assembled FP32 matrix, mixed with the element operator",
+    print(json.dumps({"kernels": "copy 1 GiB x6; C2 assembly x3; solves of 20 "
+                                 "iterations: FP64, mixed with the assembled FP32 matrix, mixed with the element "
+                                 "operator", "memory": stream,
                       "nnz": prob.nnz, "dofs": prob.ndof, "elements": prob.ne}), flush=True)
 
 
@@ -398,15 +415,7 @@ def bench_beam_kernels(fa, args):
     1 GiB device copy (counter calibration), then on the P3 or P4 elasticity
     beam (40x10x10 Kuhn cells, 24 000 tets) one assembly and 40-iteration FP64
     and FP32 solves (3x3 block SELL SpMV kernels)."""
-    import torch
-    nbytes = 1 << 30
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    del a, b
+    stream = memory_reference()
     ctx = fa.Context(0)
     k = args.order if args.order in (3, 4) else 4
     coords, elems = fa.mesh_kuhn_box(40, 10, 10, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=args.seed)
@@ -438,8 +447,9 @@ def bench_beam_kernels(fa, args):
         out[name] = {"ms": round((time.perf_counter() - t) * 1e3, 2), "its": st["its"]}
         log(f"beam P{k} {name}: {st}")
     nnz = g.nnz
-    print(json.dumps({"kernels": f"copy 1 GiB x5; P{k} beam elasticity: one assembly, {its}-iteration FP64 and "
-                                 "FP32 solves", "dofs": ndof, "nnz": nnz, "block_slots": nnz // 9,
+    print(json.dumps({"kernels": f"copy 1 GiB x6; P{k} beam elasticity: one assembly, "
+                                 f"{its}-iteration FP64 and FP32 solves", "memory": stream,
+                      "dofs": ndof, "nnz": nnz, "block_slots": nnz // 9,
                       "algorithmic_bytes": {"fp64": 8 * nnz + 4 * (nnz // 9) + 20 * ndof,
                                             "fp32": 4 * nnz + 4 * (nnz // 9) + 12 * ndof},
                       "solves": out}), flush=True)
@@ -452,15 +462,7 @@ def bench_elastic(fa, args):
     scatter; materialised on the FP64 matrix cores then scattered) for the P4
     elasticity beam (40x10x10 Kuhn cells, 24 000 tets, 811 923 dofs) and for P4
     diffusion on the Kuhn 32^3 cube (196 608 tets, quadrature order 6)."""
-    import torch
-    nbytes = 1 << 30
-    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
-    a.fill_(1.0)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    del a, b
+    stream = memory_reference()
     ctx = fa.Context(0)
     k = args.order if args.order in (3, 4) else 4
     coords, elems = fa.mesh_kuhn_box(40, 10, 10, lx=1.0, ly=0.25, lz=0.25, jitter=0.0, seed=args.seed)
@@ -492,8 +494,8 @@ def bench_elastic(fa, args):
             S.assemble(fa.DIFFUSION, 6, kdiff=kd, precision=fa.FP64_FP32, matrices=m)
         ctx.synchronize()
         sout[name] = round((time.perf_counter() - t) / 3 * 1e3, 3)
-    print(json.dumps({"kernels": f"copy 1 GiB x5; P{k} beam elasticity and P4 Kuhn 32^3 diffusion assemblies, "
-                                 "x4 per element-matrix path",
+    print(json.dumps({"kernels": f"copy 1 GiB x6; P{k} beam elasticity and P4 Kuhn 32^3 "
+                                 "diffusion assemblies, x4 per element-matrix path", "memory": stream,
                       "elements": ne, "dofs": space.ndof, "bilinear_ms": out,
                       "scalar_p4": {"elements": len(se), "dofs": sspace.ndof, "bilinear_ms": sout}}), flush=True)
 

@@ -41,9 +41,11 @@ constexpr int kElasticF = 90;         // elasticity factors per element: 9 block
 constexpr int kSegTask = 256;         // scatter entries per task (whole slots)
 // 3 x 3 block image: lanes per node row, chosen per slice: 2 (32 rows per
 // slice) unless the slice's longest row would then take more than
-// kBlockStepCap steps, then 4 or 8 (16 or 8 rows per slice)
-constexpr int kBlockMinShift = 1, kBlockMaxShift = 3;
-constexpr int kBlockStepCap = 48;
+// kBlockStepCap steps, then 4, 8 or 16 (16, 8 or 4 rows per slice).
+// Measured on the P3/P4 beams (cap, largest L): (48, 8), (32, 16), (24, 16),
+// (32, 8).
+constexpr int kBlockMinShift = 1, kBlockMaxShift = 4;
+constexpr int kBlockStepCap = 32;
 
 }  // namespace feelpp_amd
 

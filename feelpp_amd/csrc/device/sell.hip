@@ -255,7 +255,7 @@ done:
 // the x gathers read three consecutive entries, and the L lanes of a row add
 // their sums at the end.  Rows are ordered longest first inside windows, and a
 // slice whose longest row would take more than kBlockStepCap steps with two
-// lanes takes four or eight, so no wavefront streams far longer than the rest
+// lanes takes 4, 8 or 16, so no wavefront streams far longer than the rest
 // (P4 tetrahedra: rows of 35 to 369 blocks).
 
 namespace {
@@ -337,7 +337,7 @@ namespace feelpp_amd {
 
 // Slices of the block image over the node order nperm: each takes the next
 // 64 / L rows, L = 2 unless the longest of them would take more than
-// kBlockStepCap steps (then 4, then 8).  Fills the first row of each slice
+// kBlockStepCap steps (then 4, 8, 16).  Fills the first row of each slice
 // (and nb at the end) and log2 L per slice.
 static void blockSlices(const std::vector<int32_t>& nperm, const std::vector<int32_t>& nodeLen, std::vector<int32_t>& row0,
                  std::vector<uint8_t>& shift) {
