@@ -231,8 +231,9 @@ def bench_batched(fa, args):
     ok = all(s["converged"] for s in stats.values())
     log(f"batched mixed: {len(draws) / wall:.2f} solves/s, mean {np.mean(its):.0f} iterations")
     del B
-    # the same engine and draws, plain FP64 Jacobi-CG (like for like: batching without the precision change)
-    B64 = fa.Batch(ctx, rtol=args.rtol, precision=fa.FP64, use_graph=True, group_spmv=False)
+    # the same engine and draws, plain FP64 Jacobi-CG (like for like: the same batching and grouped term
+    # stream, without the precision change)
+    B64 = fa.Batch(ctx, rtol=args.rtol, precision=fa.FP64, use_graph=True, group_spmv=not args.no_group)
     ids64 = {k: B64.add_family(F.terms, F.rhs, F.rows) for k, F in fams.items()}
     run_batch(fa, B64, fams, ids64, draws[:min(8, len(draws))], args.chunk)
     lat64, stats64, wall64, sols64 = run_batch(fa, B64, fams, ids64, draws, args.chunk, keep=keep)
@@ -265,6 +266,7 @@ def bench_batched(fa, args):
                        "p99": round(float(np.percentile(L, 99)) * 1e3, 2)},
         "batched_fp64_solves_per_s": round(len(draws) / wall64, 3),
         "batched_fp64_converged": ok64,
+        "batched_fp64_grouped": not args.no_group,
         "speedup_vs_batched_fp64": round(wall64 / wall, 3),
         "rel_err_mixed_vs_batched_fp64": err_mixed,
         "rel_err_sample": keep,

@@ -191,11 +191,13 @@ __global__ __launch_bounds__(kBlock) void kBMask(int64_t n, int64_t nslices, con
 // Per slice, the terms with a nonzero FP32 value on a kept slot (mask 0): bit t
 // of tmask[s].  A term that is zero over a slice adds only zeros to every
 // a = sum_t mu_t a_t there, so the grouped SpMV does not read its plane.
+template <typename T>
 struct TermPtrs {
-    const float* t[kMaxTerms];
+    const T* t[kMaxTerms];
 };
+template <typename V>
 __global__ __launch_bounds__(kBlock) void kBTermMask(int64_t nslices, int nterms, const uint32_t* __restrict__ slicePtr,
-                                                    TermPtrs T, const uint8_t* __restrict__ mask,
+                                                    TermPtrs<V> T, const uint8_t* __restrict__ mask,
                                                     uint8_t* __restrict__ tmask) {
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * kBlock + threadIdx.x) >> 6;
@@ -205,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void kBTermMask(int64_t nslices, int nterms
         unsigned m = 0;
         for (int t = 0; t < nterms; ++t) {
             bool nz = false;
-            for (uint32_t q = b0 + lane; q < b1; q += 64) nz |= mask[q] == 0 && T.t[t][q] != 0.0f;
+            for (uint32_t q = b0 + lane; q < b1; q += 64) nz |= mask[q] == 0 && T.t[t][q] != (V)0;
             if (__ballot(nz)) m |= 1u << t;
         }
         if (lane == 0) tmask[s] = (uint8_t)m;
@@ -312,16 +314,18 @@ constexpr int kGroupMax = 8;
 constexpr int kGroupBlock = 256;   // 4 wavefronts
 constexpr int kXcds = 8;           // accelerator complex dies, each with its own L2
 constexpr int kGroupSlices = 16;   // slices per grouped-SpMV block (4 per wavefront)
-// A family group: up to kGroupMax in-flight FP32 instances of one family whose
-// search directions are stored interleaved, P[r * 8 + slot], so that one
-// 32-byte load brings every instance's p at a column.
+// A family group: up to kGroupMax in-flight instances of one family (all FP32
+// or all FP64) whose search directions are stored interleaved, P[r * 8 +
+// slot], so that one 32-byte (FP32) or 64-byte (FP64) row brings every
+// instance's p at a column.
 struct BGroup {
     int nterms, compact, nitems;
     int64_t n;
-    const float* P;                      // interleaved search directions of the group
+    const void* P;                       // interleaved search directions of the group (T)
     int inst[kGroupMax];                 // per slot: position in the launch's instance array, -1 if none
-    float mu[kGroupMax][kMaxTerms];
-    const float* terms[kMaxTerms];       // family terms, FP32 SELL values
+    float muF[kGroupMax][kMaxTerms];     // mu rounded to FP32 (FP32 groups)
+    double muD[kGroupMax][kMaxTerms];    // mu (FP64 groups)
+    const void* terms[kMaxTerms];        // family terms, SELL values in T
     const uint8_t* mask;
     const uint8_t* tmask;                // per slice: terms nonzero there (bit t)
     SellCols cols;
@@ -331,19 +335,38 @@ struct BGItem {
 };
 
 typedef Quad<float>::type f32x4;
+typedef Quad<double>::type f64x4;
+
+template <typename T>
+__device__ inline T groupMu(const BGroup& G, int b, int t);
+template <>
+__device__ inline float groupMu<float>(const BGroup& G, int b, int t) { return G.muF[b][t]; }
+template <>
+__device__ inline double groupMu<double>(const BGroup& G, int b, int t) { return G.muD[b][t]; }
+
+// four consecutive T values: FP32 non-temporally (one 16-byte load; the term
+// stream is read once), FP64 through L2 (two 16-byte loads of one line)
+__device__ inline void load4(const float* p, float* v) {
+    const f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(p));
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
+__device__ inline void load4(const double* p, double* v) {
+    const f64x4 q = *reinterpret_cast<const f64x4*>(p);
+    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+}
 
 // One quad (4 consecutive entries) of one slice row: columns, unit diagonal
 // of a Dirichlet row, and the term values (zero on Dirichlet slots)
-template <int NT>
+template <typename T, int NT>
 struct GroupQuad {
     int32_t col[4];
-    float e[4];
-    float a[4][NT];
+    T e[4];
+    T a[4][NT];
 };
 
-template <int NT, bool NARROW>
+template <typename T, int NT, bool NARROW>
 __device__ inline void loadQuad(const BGroup& G, unsigned tm, size_t q0, const void* cp, int32_t cb, int kq,
-                                GroupQuad<NT>& d) {
+                                GroupQuad<T, NT>& d) {
     // q0: slot of the lane's first quad; quad kq starts 256 kq slots further
     const size_t q = q0 + ((size_t)kq << 8);
     if (NARROW) {
@@ -355,19 +378,18 @@ __device__ inline void loadQuad(const BGroup& G, unsigned tm, size_t q0, const v
     }
     const u8x4 m = *reinterpret_cast<const u8x4*>(G.mask + q);
     const uint8_t mm[4] = {m.x, m.y, m.z, m.w};
-    float tv[NT][4];
+    T tv[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         // tm is wave-uniform: a term that is zero over the slice is not read
-        const f32x4 v = (tm >> t) & 1u ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(G.terms[t] + q))
-                                       : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        tv[t][0] = v.x; tv[t][1] = v.y; tv[t][2] = v.z; tv[t][3] = v.w;
+        if ((tm >> t) & 1u) load4(static_cast<const T*>(G.terms[t]) + q, tv[t]);
+        else tv[t][0] = tv[t][1] = tv[t][2] = tv[t][3] = (T)0;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        d.e[u] = mm[u] == 2 ? 1.0f : 0.0f;
+        d.e[u] = mm[u] == 2 ? (T)1 : (T)0;
 #pragma unroll
-        for (int t = 0; t < NT; ++t) d.a[u][t] = mm[u] == 0 ? tv[t][u] : 0.0f;
+        for (int t = 0; t < NT; ++t) d.a[u][t] = mm[u] == 0 ? tv[t][u] : (T)0;
     }
 }
 
@@ -378,15 +400,31 @@ __device__ inline void groupP(const float* __restrict__ P, int64_t r, float* pv)
     pv[0] = lo.x; pv[1] = lo.y; pv[2] = lo.z; pv[3] = lo.w;
     pv[4] = hi.x; pv[5] = hi.y; pv[6] = hi.z; pv[7] = hi.w;
 }
+__device__ inline void groupP(const double* __restrict__ P, int64_t r, double* pv) {
+    const f64x4* q = reinterpret_cast<const f64x4*>(P) + 2 * r;
+    const f64x4 lo = q[0], hi = q[1];
+    pv[0] = lo.x; pv[1] = lo.y; pv[2] = lo.z; pv[3] = lo.w;
+    pv[4] = hi.x; pv[5] = hi.y; pv[6] = hi.z; pv[7] = hi.w;
+}
+__device__ inline void groupPStore(float* __restrict__ P, int64_t r, const float* pv) {
+    f32x4* q = reinterpret_cast<f32x4*>(P) + 2 * r;
+    q[0] = f32x4{pv[0], pv[1], pv[2], pv[3]};
+    q[1] = f32x4{pv[4], pv[5], pv[6], pv[7]};
+}
+__device__ inline void groupPStore(double* __restrict__ P, int64_t r, const double* pv) {
+    f64x4* q = reinterpret_cast<f64x4*>(P) + 2 * r;
+    q[0] = f64x4{pv[0], pv[1], pv[2], pv[3]};
+    q[1] = f64x4{pv[4], pv[5], pv[6], pv[7]};
+}
 
 // acc_b += A(mu_b) p_b on the quad for every active instance b of the group
-template <int NT>
-__device__ inline void applyQuad(const BGroup& G, unsigned amask, unsigned tm, const GroupQuad<NT>& d, float* acc) {
+template <typename T, int NT>
+__device__ inline void applyQuad(const BGroup& G, unsigned amask, unsigned tm, const GroupQuad<T, NT>& d, T* acc) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        float pv[kGroupMax];
-        groupP(G.P, d.col[u], pv);
-        float v[kGroupMax];
+        T pv[kGroupMax];
+        groupP(static_cast<const T*>(G.P), d.col[u], pv);
+        T v[kGroupMax];
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b) v[b] = d.e[u];
         // terms in order; one that is zero over the slice (tm, wave-uniform)
@@ -396,7 +434,7 @@ __device__ inline void applyQuad(const BGroup& G, unsigned amask, unsigned tm, c
             if ((tm >> t) & 1u)
 #pragma unroll
                 for (int b = 0; b < kGroupMax; ++b)
-                    if ((amask >> b) & 1u) v[b] += G.mu[b][t] * d.a[u][t];
+                    if ((amask >> b) & 1u) v[b] += groupMu<T>(G, b, t) * d.a[u][t];
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b)
             if ((amask >> b) & 1u) acc[b] += v[b] * pv[b];
@@ -405,30 +443,30 @@ __device__ inline void applyQuad(const BGroup& G, unsigned amask, unsigned tm, c
 
 // one slice row, the next quad's matrix data loaded while the current quad's
 // gathers and products run
-template <int NT, bool NARROW>
+template <typename T, int NT, bool NARROW>
 __device__ inline void groupRow(const BGroup& G, unsigned amask, unsigned tm, size_t q0, const void* cp, int32_t cb,
-                                int nq, float* acc) {
+                                int nq, T* acc) {
     if (nq <= 0) return;
-    GroupQuad<NT> cur;
-    loadQuad<NT, NARROW>(G, tm, q0, cp, cb, 0, cur);
+    GroupQuad<T, NT> cur;
+    loadQuad<T, NT, NARROW>(G, tm, q0, cp, cb, 0, cur);
     for (int k = 1; k < nq; ++k) {
-        GroupQuad<NT> nxt;
-        loadQuad<NT, NARROW>(G, tm, q0, cp, cb, k, nxt);
-        applyQuad<NT>(G, amask, tm, cur, acc);
+        GroupQuad<T, NT> nxt;
+        loadQuad<T, NT, NARROW>(G, tm, q0, cp, cb, k, nxt);
+        applyQuad<T, NT>(G, amask, tm, cur, acc);
         cur = nxt;
     }
-    applyQuad<NT>(G, amask, tm, cur, acc);
+    applyQuad<T, NT>(G, amask, tm, cur, acc);
 }
 
-// Grouped SpMV of an affine family (FP32 inner iterations): the in-flight
+// Grouped SpMV of an affine family (the CG iterations in T): the in-flight
 // instances of one family share the sparsity pattern and the term matrices, so
-// a group streams each slot's FP32 term values, Dirichlet mask and column once
-// and every instance b combines a = sum_t mu_bt a_t on the fly (mask 1: zero,
+// a group streams each slot's term values, Dirichlet mask and column once and
+// every instance b combines a = sum_t mu_bt a_t on the fly (mask 1: zero,
 // mask 2: unit diagonal) before w_b += a * p_b[col], the p of all instances at
-// a column coming from one 32-byte load of the interleaved directions.  Each
-// instance's arithmetic is independent of the others in the group, so a result
-// is the same alone and in any mix.
-template <int NT>
+// a column coming from one row of the interleaved directions.  Each instance's
+// arithmetic is independent of the others in the group, so a result is the
+// same alone and in any mix.
+template <typename T, int NT>
 __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restrict__ gitems,
                                                           const BGroup* __restrict__ groups,
                                                           const BInst* __restrict__ insts) {
@@ -455,27 +493,27 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
         const uint32_t base = G.cols.slicePtr[sl];
         const int nq = (int)((G.cols.slicePtr[sl + 1] - base) >> 8);
         const size_t q0 = (size_t)base + ((size_t)lane << 2);
-        float acc[kGroupMax];
+        T acc[kGroupMax];
 #pragma unroll
-        for (int b = 0; b < kGroupMax; ++b) acc[b] = 0.0f;
+        for (int b = 0; b < kGroupMax; ++b) acc[b] = (T)0;
         const int32_t cb = G.compact ? G.cols.colBase[sl] : -1;
         const unsigned tm = __builtin_amdgcn_readfirstlane((unsigned)G.tmask[sl]);
         if (cb >= 0)
-            groupRow<NT, true>(G, amask, tm, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb,
+            groupRow<T, NT, true>(G, amask, tm, q0, reinterpret_cast<const u16x4*>(G.cols.packed + G.cols.cptr[sl]) + lane, cb,
                                nq, acc);
         else if (G.compact)
-            groupRow<NT, false>(G, amask, tm, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
+            groupRow<T, NT, false>(G, amask, tm, q0, reinterpret_cast<const i32x4*>(G.cols.packed + G.cols.cptr[sl]) + lane,
                                 cb, nq, acc);
         else
-            groupRow<NT, false>(G, amask, tm, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
+            groupRow<T, NT, false>(G, amask, tm, q0, reinterpret_cast<const i32x4*>(G.cols.col + base) + lane, cb, nq, acc);
         const int64_t r = (int64_t)sl * 64 + lane;
         if (r < G.n) {
-            float pr[kGroupMax];
-            groupP(G.P, r, pr);
+            T pr[kGroupMax];
+            groupP(static_cast<const T*>(G.P), r, pr);
 #pragma unroll
             for (int b = 0; b < kGroupMax; ++b)
                 if ((amask >> b) & 1u) {
-                    tp<float>(insts[G.inst[b]].wT)[r] = acc[b];
+                    tp<T>(insts[G.inst[b]].wT)[r] = acc[b];
                     dot[b] += (double)pr[b] * (double)acc[b];
                 }
         }
@@ -500,9 +538,10 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
 
 // Direction and update kernels of a family group: one block covers one item
 // (64 slices) of every instance of the group, so the interleaved P is read
-// and written in whole 32-byte rows.  Per instance the operations, their
-// order and the rows each thread handles are those of kBDirection and
-// kBUpdate, so the results are the same as theirs, bit for bit.
+// and written in whole rows.  Per instance the operations, their order and the
+// rows each thread handles are those of kBDirection and kBUpdate, so the
+// results are the same as theirs, bit for bit.
+template <typename T>
 __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restrict__ gitems,
                                                           const BGroup* __restrict__ groups,
                                                           const BInst* __restrict__ insts, const int* __restrict__ gcount) {
@@ -547,34 +586,32 @@ __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restr
         }
     }
     __syncthreads();
-    const float* z[kGroupMax];
+    const T* z[kGroupMax];
     int mode[kGroupMax];
     double bb[kGroupMax];
 #pragma unroll
     for (int b = 0; b < kGroupMax; ++b) {
         mode[b] = smode[b];
         bb[b] = sbb[b];
-        z[b] = mode[b] ? tp<float>(insts[G.inst[b]].zT) : nullptr;
+        z[b] = mode[b] ? tp<T>(insts[G.inst[b]].zT) : nullptr;
     }
     const int64_t r0 = (int64_t)item.s0 * 64, r1 = std::min<int64_t>((int64_t)item.s1 * 64, G.n);
-    f32x4* P = reinterpret_cast<f32x4*>(const_cast<float*>(G.P));
+    T* P = static_cast<T*>(const_cast<void*>(G.P));
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
         // every load of the row before any store: the instances' loads overlap
-        float zv[kGroupMax];
+        T zv[kGroupMax];
 #pragma unroll
-        for (int b = 0; b < kGroupMax; ++b) zv[b] = mode[b] ? z[b][r] : 0.0f;
-        f32x4 lo = P[2 * r], hi = P[2 * r + 1];
-        float pv[kGroupMax] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        for (int b = 0; b < kGroupMax; ++b) zv[b] = mode[b] ? z[b][r] : (T)0;
+        T pv[kGroupMax];
+        groupP(P, r, pv);
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b)
-            if (mode[b]) pv[b] = mode[b] == 2 ? zv[b] : (float)((double)zv[b] + bb[b] * (double)pv[b]);
-        lo = f32x4{pv[0], pv[1], pv[2], pv[3]};
-        hi = f32x4{pv[4], pv[5], pv[6], pv[7]};
-        P[2 * r] = lo;
-        P[2 * r + 1] = hi;
+            if (mode[b]) pv[b] = mode[b] == 2 ? zv[b] : (T)((double)zv[b] + bb[b] * (double)pv[b]);
+        groupPStore(P, r, pv);
     }
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict__ gitems,
                                                        const BGroup* __restrict__ groups,
                                                        const BInst* __restrict__ insts, const int* __restrict__ gcount) {
@@ -621,23 +658,23 @@ __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict
         zz[b] = 0.0;
     }
     const int64_t r0 = (int64_t)item.s0 * 64, r1 = std::min<int64_t>((int64_t)item.s1 * 64, G.n);
-    const f32x4* P = reinterpret_cast<const f32x4*>(G.P);
-    float *X[kGroupMax], *Rv[kGroupMax], *Z[kGroupMax];
-    const float *W[kGroupMax], *D[kGroupMax];
+    const T* P = static_cast<const T*>(G.P);
+    T *X[kGroupMax], *Rv[kGroupMax], *Z[kGroupMax];
+    const T *W[kGroupMax], *D[kGroupMax];
 #pragma unroll
     for (int b = 0; b < kGroupMax; ++b) {
         const BInst* I = act[b] ? &insts[G.inst[b]] : nullptr;
-        X[b] = I ? tp<float>(I->xT) : nullptr;
-        Rv[b] = I ? tp<float>(I->rT) : nullptr;
-        Z[b] = I ? tp<float>(I->zT) : nullptr;
-        W[b] = I ? tp<float>(I->wT) : nullptr;
-        D[b] = I ? tp<float>(I->dinvT) : nullptr;
+        X[b] = I ? tp<T>(I->xT) : nullptr;
+        Rv[b] = I ? tp<T>(I->rT) : nullptr;
+        Z[b] = I ? tp<T>(I->zT) : nullptr;
+        W[b] = I ? tp<T>(I->wT) : nullptr;
+        D[b] = I ? tp<T>(I->dinvT) : nullptr;
     }
     for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-        const f32x4 lo = P[2 * i], hi = P[2 * i + 1];
-        const float pv[kGroupMax] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        T pv[kGroupMax];
+        groupP(P, i, pv);
         // every load of the row before any store: the instances' loads overlap
-        float xv[kGroupMax], rv[kGroupMax], wv[kGroupMax], dv[kGroupMax];
+        T xv[kGroupMax], rv[kGroupMax], wv[kGroupMax], dv[kGroupMax];
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b) {
             if (!act[b]) continue;
@@ -649,10 +686,10 @@ __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict
 #pragma unroll
         for (int b = 0; b < kGroupMax; ++b) {
             if (!act[b]) continue;
-            X[b][i] = (float)((double)xv[b] + a[b] * (double)pv[b]);
-            const float ri = (float)((double)rv[b] - a[b] * (double)wv[b]);
+            X[b][i] = (T)((double)xv[b] + a[b] * (double)pv[b]);
+            const T ri = (T)((double)rv[b] - a[b] * (double)wv[b]);
             Rv[b][i] = ri;
-            const float zi = dv[b] * ri;
+            const T zi = dv[b] * ri;
             Z[b][i] = zi;
             zr[b] += (double)zi * (double)ri;
             zz[b] += (double)zi * (double)zi;
@@ -721,7 +758,7 @@ struct BatchFamily {
     feel_hip_graph* g = nullptr;
     int nterms = 0;
     std::vector<double*> terms;   // SELL FP64 values per term
-    std::vector<float*> termsF;   // the same rounded to FP32 (grouped FP32 SpMV)
+    std::vector<float*> termsF;   // the same rounded to FP32 (grouped SpMV of FP32 batches)
     uint8_t* mask = nullptr;
     uint8_t* tmask = nullptr;     // per slice: FP32 terms nonzero on a kept slot (grouped SpMV)
     double* b64 = nullptr;        // permuted rhs with zero Dirichlet rows
@@ -729,11 +766,11 @@ struct BatchFamily {
     int nitems = 0;
     size_t instBytes = 0;
     std::vector<int> freeBufs;    // reusable instance buffer sets
-    // family groups of the grouped FP32 SpMV: interleaved search directions
+    // family groups of the grouped SpMV: interleaved search directions
     // P[n x 8] and the request holding each slot (-1 = free); an instance keeps
     // its slot from admission until it finishes
     struct PGroup {
-        float* P = nullptr;
+        void* P = nullptr;   // T rows of kGroupMax
         int64_t req[kGroupMax];
     };
     std::vector<PGroup> pgroups;
@@ -762,7 +799,7 @@ struct Request {
     int fresh = 0;         // iteration at which the search direction last (re)started
     double devRnorm0 = 0;  // the device's initial residual norm (FP32 units), for the dtol test
     bool unpermuted = false;   // solution copied to Feel++ numbering
-    int pgroup = -1, pslot = -1;   // family group slot of the grouped FP32 SpMV
+    int pgroup = -1, pslot = -1;   // family group slot of the grouped SpMV
     int waited = 0;        // steps spent at the head of the queue without fitting (backfill)
 };
 
@@ -810,9 +847,9 @@ size_t partCount(const BatchFamily& f) {
     return 3 * (size_t)f.nitems + (size_t)((f.nslices + kGroupSlices - 1) / kGroupSlices);
 }
 
-// the grouped FP32 engine reads the family terms and keeps p in the family
+// the grouped engine reads the family terms and keeps p in the family
 // group: its instances hold no T copy of A(mu) and no p of their own
-bool groupedEngine(const feel_hip_batch* B) { return B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv; }
+bool groupedEngine(const feel_hip_batch* B) { return B->opts.group_spmv != 0; }
 
 size_t instanceBytes(const feel_hip_batch* B, const BatchFamily& f) {
     const size_t t = tsize(B);
@@ -868,7 +905,7 @@ int ensureStates(feel_hip_batch* B, int need) {
 
 // Work list (and per-instance descriptors) for a set of requests.
 // Work list of the requests rs: one BInst per request, its 64-slice items, and
-// with groups set the family groups of the grouped FP32 SpMV (instances of one
+// with groups set the family groups of the grouped SpMV (instances of one
 // family in request order, kGroupMax per group; group items ordered by item
 // within a family so the groups of a family stream the same slices together).
 int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, bool groups = false) {
@@ -896,7 +933,7 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
         I.wT = ib.wT; I.bT = ib.bT;
         I.pstride = 1;
         if (r->pgroup >= 0) {
-            I.pT = f.pgroups[r->pgroup].P + r->pslot;
+            I.pT = static_cast<char*>(f.pgroups[r->pgroup].P) + tsize(B) * r->pslot;
             I.pstride = kGroupMax;
         }
         I.st = B->states + r->state;
@@ -948,9 +985,15 @@ int buildWork(feel_hip_batch* B, const std::vector<Request*>& rs, int* nItems, b
                 auto it = pg.req[b] >= 0 ? pos.find(pg.req[b]) : pos.end();
                 G.inst[b] = it == pos.end() ? -1 : it->second;
                 if (G.inst[b] >= 0)
-                    for (int t = 0; t < f.nterms; ++t) G.mu[b][t] = (float)rs[G.inst[b]]->mu[t];
+                    for (int t = 0; t < f.nterms; ++t) {
+                        G.muD[b][t] = rs[G.inst[b]]->mu[t];
+                        G.muF[b][t] = (float)rs[G.inst[b]]->mu[t];
+                    }
             }
-            for (int t = 0; t < kMaxTerms; ++t) G.terms[t] = f.termsF[t < f.nterms ? t : 0];
+            for (int t = 0; t < kMaxTerms; ++t) {
+                const int u = t < f.nterms ? t : 0;
+                G.terms[t] = B->opts.precision == FEEL_HIP_FP32 ? (const void*)f.termsF[u] : (const void*)f.terms[u];
+            }
             G.mask = f.mask;
             G.tmask = f.tmask;
             G.cols = SellCols{f.g->slicePtr, f.g->scol, f.g->colBase, f.g->ccptr, f.g->packed};
@@ -1042,12 +1085,13 @@ int residualNorms(feel_hip_batch* B, const std::vector<Request*>& rs) {
 // grouped SpMV over every group of the current work list, instantiated for
 // the largest term count of the list: a family with fewer terms pads them with
 // its first term and mu = 0 (the repeated loads hit the cache lines just read)
+template <typename T>
 void launchGroupSpmv(feel_hip_batch* B, int nterms) {
     feel_hip_ctx* c = B->ctx;
     const dim3 grid(B->nGItems), block(kGroupBlock);
     switch (nterms) {
 #define FEEL_GROUP_CASE(NT) \
-    case NT: hipLaunchKernelGGL(kBSpmvGroup<NT>, grid, block, 0, c->stream, B->dGItems, B->dGroups, B->dInsts); break;
+    case NT: hipLaunchKernelGGL((kBSpmvGroup<T, NT>), grid, block, 0, c->stream, B->dGItems, B->dGroups, B->dInsts); break;
         FEEL_GROUP_CASE(1) FEEL_GROUP_CASE(2) FEEL_GROUP_CASE(3) FEEL_GROUP_CASE(4)
         FEEL_GROUP_CASE(5) FEEL_GROUP_CASE(6) FEEL_GROUP_CASE(7) FEEL_GROUP_CASE(8)
 #undef FEEL_GROUP_CASE
@@ -1057,14 +1101,14 @@ void launchGroupSpmv(feel_hip_batch* B, int nterms) {
 template <typename T>
 int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
     feel_hip_ctx* c = B->ctx;
-    const bool grouped = sizeof(T) == 4 && nterms > 0;
+    const bool grouped = nterms > 0;
     auto body = [&]() {
         for (int k = 0; k < chunk; ++k) {
             if (grouped) {
-                hipLaunchKernelGGL(kBDirectionGroup, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems,
+                hipLaunchKernelGGL(kBDirectionGroup<T>, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems,
                                    B->dGroups, B->dInsts, B->gcount);
-                launchGroupSpmv(B, nterms);
-                hipLaunchKernelGGL(kBUpdateGroup, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems, B->dGroups,
+                launchGroupSpmv<T>(B, nterms);
+                hipLaunchKernelGGL(kBUpdateGroup<T>, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems, B->dGroups,
                                    B->dInsts, B->gcount);
             } else {
                 hipLaunchKernelGGL(kBDirection<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts,
@@ -1105,7 +1149,7 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
     return FEEL_HIP_OK;
 }
 
-// a family group slot for a request of the grouped FP32 SpMV (a new group
+// a family group slot for a request of the grouped SpMV (a new group
 // when every slot of the family's groups is taken)
 bool hasFreeSlot(const BatchFamily& f) {
     for (const auto& pg : f.pgroups)
@@ -1115,7 +1159,7 @@ bool hasFreeSlot(const BatchFamily& f) {
     return false;
 }
 
-size_t groupBytes(const BatchFamily& f) { return sizeof(float) * kGroupMax * (size_t)f.n; }
+size_t groupBytes(const feel_hip_batch* B, const BatchFamily& f) { return tsize(B) * kGroupMax * (size_t)f.n; }
 
 int assignSlot(feel_hip_batch* B, Request& r) {
     BatchFamily& f = B->fams[r.family];
@@ -1129,10 +1173,10 @@ int assignSlot(feel_hip_batch* B, Request& r) {
             }
     BatchFamily::PGroup pg;
     for (int b = 0; b < kGroupMax; ++b) pg.req[b] = -1;
-    TRY_CALL(B->ctx->alloc((void**)&pg.P, groupBytes(f)));
-    // slots without an instance are read (one 32-byte load serves all eight) but never used
-    FEEL_HIP_CHECK(hipMemsetAsync(pg.P, 0, groupBytes(f), B->ctx->stream));
-    B->used += (int64_t)groupBytes(f);
+    TRY_CALL(B->ctx->alloc((void**)&pg.P, groupBytes(B, f)));
+    // slots without an instance are read (one row load serves all eight) but never used
+    FEEL_HIP_CHECK(hipMemsetAsync(pg.P, 0, groupBytes(B, f), B->ctx->stream));
+    B->used += (int64_t)groupBytes(B, f);
     pg.req[0] = r.id;
     f.pgroups.push_back(pg);
     r.pgroup = (int)f.pgroups.size() - 1;
@@ -1190,8 +1234,8 @@ bool releaseIdle(feel_hip_batch* B, int64_t need, int keep) {
             if (!idle) continue;
             (void)hipFree(pg.P);
             pg.P = nullptr;
-            B->used -= (int64_t)groupBytes(f);
-            c->bytes -= (int64_t)groupBytes(f);
+            B->used -= (int64_t)groupBytes(B, f);
+            c->bytes -= (int64_t)groupBytes(B, f);
             freed = true;
         }
     }
@@ -1212,9 +1256,9 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
         Request& r = B->reqs[*it];
         BatchFamily& f = B->fams[r.family];
         const bool reuse = !f.freeBufs.empty();
-        const bool grouped = sizeof(T) == 4 && B->opts.group_spmv;
+        const bool grouped = B->opts.group_spmv != 0;
         const int64_t need = (reuse ? 0 : (int64_t)instanceBytes(B, f)) +
-                             (grouped && !hasFreeSlot(f) ? (int64_t)groupBytes(f) : 0);
+                             (grouped && !hasFreeSlot(f) ? (int64_t)groupBytes(B, f) : 0);
         if (need > 0 && B->opts.hbm_budget > 0 && B->used + need > B->opts.hbm_budget &&
             !releaseIdle(B, need, r.family)) {
             if (B->used == 0) return setError(FEEL_HIP_ENOMEM, "one instance exceeds the batch HBM budget");
@@ -1302,7 +1346,7 @@ int stepImpl(feel_hip_batch* B, int iterations, int* nActive, int* nDone) {
     for (auto& kv : B->reqs)
         if (kv.second.phase == Phase::Inner) inner.push_back(&kv.second);
     if (!inner.empty() && iterations > 0) {
-        const bool grouped = sizeof(T) == 4 && B->opts.group_spmv;
+        const bool grouped = B->opts.group_spmv != 0;
         int maxTerms = 0;
         for (Request* r : inner) maxTerms = std::max(maxTerms, B->fams[r->family].nterms);
         TRY_CALL(buildWork(B, inner, &nItems, grouped));
@@ -1455,7 +1499,7 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
         hipLaunchKernelGGL(kCsrToSell<double>, dim3(grid), dim3(kBlock), 0, c->stream, f.nslots, g->slot2csr, terms[t]->v64,
                            (const float*)nullptr, v);
         f.terms.push_back(v);
-        if (B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv) {
+        if (B->opts.precision == FEEL_HIP_FP32 && B->opts.group_spmv) {   // FP32 groups stream FP32 terms
             float* vf = nullptr;
             TRY_CALL(alloc(c, &vf, f.nslots));
             hipLaunchKernelGGL(kCastF32, dim3(grid), dim3(kBlock), 0, c->stream, f.nslots, v, vf);
@@ -1472,13 +1516,20 @@ int feel_hip_batch_add_family(feel_hip_batch* B, int nterms, feel_hip_mat* const
     TRY_CALL(alloc(c, &f.mask, f.nslots));
     hipLaunchKernelGGL(kBMask, dim3(grid), dim3(kBlock), 0, c->stream, f.n, f.nslices, g->slicePtr, g->scol, g->slot2csr,
                        g->perm, flags, f.mask);
-    if (!f.termsF.empty()) {
-        TermPtrs T{};
-        for (int t = 0; t < nterms; ++t) T.t[t] = f.termsF[t];
+    if (B->opts.group_spmv) {
         TRY_CALL(alloc(c, &f.tmask, f.nslices));
         const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>((f.nslices * 64 + kBlock - 1) / kBlock, 8192));
-        hipLaunchKernelGGL(kBTermMask, dim3(sgrid), dim3(kBlock), 0, c->stream, f.nslices, nterms, g->slicePtr, T, f.mask,
-                           f.tmask);
+        if (B->opts.precision == FEEL_HIP_FP32) {
+            TermPtrs<float> T{};
+            for (int t = 0; t < nterms; ++t) T.t[t] = f.termsF[t];
+            hipLaunchKernelGGL(kBTermMask<float>, dim3(sgrid), dim3(kBlock), 0, c->stream, f.nslices, nterms,
+                               g->slicePtr, T, f.mask, f.tmask);
+        } else {
+            TermPtrs<double> T{};
+            for (int t = 0; t < nterms; ++t) T.t[t] = f.terms[t];
+            hipLaunchKernelGGL(kBTermMask<double>, dim3(sgrid), dim3(kBlock), 0, c->stream, f.nslices, nterms,
+                               g->slicePtr, T, f.mask, f.tmask);
+        }
     }
     TRY_CALL(alloc(c, &f.b64, f.n));
     hipLaunchKernelGGL(kBRhs, dim3(grid), dim3(kBlock), 0, c->stream, f.n, g->perm, flags, rhs->d, f.b64);

@@ -337,7 +337,7 @@ typedef struct {
     int precision;            /* FEEL_HIP_FP32 (refined) or FEEL_HIP_FP64 */
     int64_t hbm_budget;       /* bytes of instance buffers allowed at once, 0 = unlimited */
     int use_graph;            /* capture each iteration chunk in a hipGraph */
-    int group_spmv;           /* FP32: one SpMV pass per family group, A(mu) formed on the fly */
+    int group_spmv;           /* one SpMV pass per family group, A(mu) formed on the fly (FP32 and FP64) */
     int backfill;             /* 0: strict submission order; N > 0: requests that fit the budget may pass
                                  one that does not, until it has waited N steps */
 } feel_hip_batch_opts;

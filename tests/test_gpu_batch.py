@@ -275,15 +275,16 @@ def two_term(ctx, F):
     return terms
 
 
+@pytest.mark.parametrize("precision", [fa.FP32, fa.FP64])
 @pytest.mark.parametrize("group", [True, False])
-def test_groups_of_mixed_term_counts(ctx, group):
+def test_groups_of_mixed_term_counts(ctx, group, precision):
     # eleven instances of a 4-term family (two groups of the grouped SpMV) and
     # three of a 2-term family on the same graph, whose terms are padded to
     # four in the grouped pass; every solution against the FP64 solve, and a
     # 2-term instance bitwise equal to the same request solved alone
     F = family(ctx, 5, 2)
     t2 = two_term(ctx, F)
-    B = fa.Batch(ctx, rtol=1e-10, group_spmv=group)
+    B = fa.Batch(ctx, rtol=1e-10, group_spmv=group, precision=precision)
     f4 = B.add_family(F["terms"], F["b"], F["rows"])
     f2 = B.add_family(t2, F["b"], F["rows"])
     rng = np.random.default_rng(7)
@@ -301,7 +302,7 @@ def test_groups_of_mixed_term_counts(ctx, group):
         assert sd["converged"], sd
         assert rel_fro(x, reference(ctx, F, mu4)) < 1e-8
     rid2 = next(r for r, mu4 in reqs.items() if mu4[0] == mu4[1] and mu4[2] == mu4[3])
-    A = fa.Batch(ctx, rtol=1e-10, group_spmv=group)
+    A = fa.Batch(ctx, rtol=1e-10, group_spmv=group, precision=precision)
     a2 = A.add_family(t2, F["b"], F["rows"])
     r = A.submit(a2, reqs[rid2][::2])
     alone = run_all(A, {r: F["ndof"]}, chunk=9)[r][1]
