@@ -149,6 +149,7 @@ struct feel_hip_space {
 
 struct feel_hip_graph {
     int refs = 1;
+    uint64_t uid = 0;   // process-unique, never reused (keys per-graph caches)
     feel_hip_space* space = nullptr;
     int64_t n = 0, nnz = 0;
     int32_t* ia = nullptr;
@@ -227,4 +228,11 @@ struct feel_hip_rows {
     int64_t n = 0, nrows = 0;
     int32_t* rows = nullptr;
     uint8_t* flag = nullptr;
+    int32_t* flagged = nullptr;   // the distinct flagged rows, ascending
+    int64_t nflagged = 0;
+    // rows that symmetric elimination changes on one graph (flagged rows and
+    // rows holding a flagged column), built on the first such call there
+    mutable uint64_t touchedGraph = 0;
+    mutable int32_t* touched = nullptr;
+    mutable int64_t ntouched = 0;
 };

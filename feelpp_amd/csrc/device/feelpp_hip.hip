@@ -2,6 +2,7 @@
 // Dirichlet elimination and the Jacobi-preconditioned CG (FP64, and FP32 inside
 // FP64 iterative refinement).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -550,6 +551,41 @@ int finishGraph(feel_hip_graph* g) {
     return FEEL_HIP_OK;
 }
 
+uint64_t nextGraphUid() {
+    static std::atomic<uint64_t> next{1};
+    return next++;
+}
+
+// The rows of g that symmetric elimination by `rows` changes (flagged rows and
+// rows holding a flagged column), ascending; listed once per (rows, graph).
+int touchedRows(const feel_hip_graph* g, const feel_hip_rows* rows) {
+    if (rows->touchedGraph == g->uid) return FEEL_HIP_OK;
+    feel_hip_ctx* c = g->space->ctx;
+    constexpr int L = 16;
+    uint8_t* mark = nullptr;
+    TRY_CALL(c->alloc((void**)&mark, (size_t)g->n));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock / L - 1) / (kBlock / L), 16384));
+    hipLaunchKernelGGL(kMarkTouched<L>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, rows->flag, mark);
+    std::vector<uint8_t> h((size_t)g->n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), mark, (size_t)g->n, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    c->release(mark, (size_t)g->n);
+    if (e != hipSuccess) return setError(FEEL_HIP_EDEVICE, std::string("touched rows: ") + hipGetErrorString(e));
+    std::vector<int32_t> list;
+    for (int64_t r = 0; r < g->n; ++r)
+        if (h[r]) list.push_back((int32_t)r);
+    // the list belongs to the row set (its context allocates and frees it)
+    if (rows->touched) rows->ctx->release(rows->touched, sizeof(int32_t) * rows->ntouched);
+    rows->touched = nullptr;
+    rows->ntouched = 0;
+    rows->touchedGraph = 0;
+    if (!list.empty()) TRY_CALL(upload(rows->ctx, &rows->touched, list.data(), list.size()));
+    rows->ntouched = (int64_t)list.size();
+    rows->touchedGraph = g->uid;
+    return FEEL_HIP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -571,6 +607,7 @@ int feel_hip_graph_create(feel_hip_space* s, int64_t nrows, const int32_t* ia, c
     feel_hip_ctx* ctx = s->ctx;
     FEEL_HIP_CHECK(hipSetDevice(ctx->device));
     auto* g = new feel_hip_graph();
+    g->uid = nextGraphUid();
     g->space = s;
     retain(s);
     g->n = nrows;
@@ -598,6 +635,7 @@ int feel_hip_graph_create_from_space(feel_hip_space* s, feel_hip_graph** out) {
     feel_hip_ctx* ctx = s->ctx;
     FEEL_HIP_CHECK(hipSetDevice(ctx->device));
     auto* g = new feel_hip_graph();
+    g->uid = nextGraphUid();
     g->space = s;
     retain(s);
     int rc = buildGraphDevice(s, g);
@@ -1161,8 +1199,13 @@ int feel_hip_rows_create(feel_hip_ctx* c, int64_t n, int64_t nrows, const int32_
     retain(c);
     r->n = n;
     r->nrows = nrows;
+    std::vector<int32_t> flagged;
+    for (int64_t i = 0; i < n; ++i)
+        if (flag[i]) flagged.push_back((int32_t)i);
+    r->nflagged = (int64_t)flagged.size();
     int rc = upload(c, &r->rows, rows, (size_t)nrows);
     if (rc == FEEL_HIP_OK) rc = upload(c, &r->flag, flag.data(), flag.size());
+    if (rc == FEEL_HIP_OK && r->nflagged) rc = upload(c, &r->flagged, flagged.data(), flagged.size());
     if (rc != FEEL_HIP_OK) { feel_hip_rows_destroy(r); return rc; }
     *out = r;
     return FEEL_HIP_OK;
@@ -1173,6 +1216,8 @@ int feel_hip_rows_destroy(feel_hip_rows* r) {
     (void)hipStreamSynchronize(r->ctx->stream);
     r->ctx->release(r->rows, sizeof(int32_t) * r->nrows);
     r->ctx->release(r->flag, r->n);
+    if (r->flagged) r->ctx->release(r->flagged, sizeof(int32_t) * r->nflagged);
+    if (r->touched) r->ctx->release(r->touched, sizeof(int32_t) * r->ntouched);
     feel_hip_ctx* c = r->ctx;
     delete r;
     unref(c);
@@ -1190,13 +1235,18 @@ int feel_hip_mat_zero_rows(feel_hip_mat* A, const feel_hip_rows* rows, const fee
         return setError(FEEL_HIP_EINVAL, "unknown elimination mode");
     feel_hip_ctx* c = g->space->ctx;
     constexpr int L = 16;   // lanes per row
-    const int grid = (int)std::min<int64_t>((g->n + kBlock / L - 1) / (kBlock / L), 16384);
     const double* gv = values ? values->d : nullptr;
     double* b = rhs ? rhs->d : nullptr;
     const int sym = mode == FEEL_HIP_ON_ELIMINATION_SYMMETRIC;
-    if (A->v64 || A->v32)
-        hipLaunchKernelGGL(kDirichletRows<L>, dim3(grid), dim3(kBlock), 0, c->stream, g->n, g->ia, g->ja, A->v64,
-                           A->v32, b, rows->flag, gv, sym, diag);
+    // only the rows the call changes: the flagged ones, and with symmetric
+    // elimination the rows holding a flagged column (listed once per graph)
+    if (sym) TRY_CALL(touchedRows(g, rows));
+    const int32_t* list = sym ? rows->touched : rows->flagged;
+    const int64_t count = sym ? rows->ntouched : rows->nflagged;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((count + kBlock / L - 1) / (kBlock / L), 16384));
+    if ((A->v64 || A->v32) && count > 0)
+        hipLaunchKernelGGL(kDirichletRows<L>, dim3(grid), dim3(kBlock), 0, c->stream, count, list, g->ia, g->ja,
+                           A->v64, A->v32, b, rows->flag, gv, sym, diag);
     FEEL_HIP_CHECK(hipGetLastError());
     A->dinvValid = false;
     A->sell64Valid = A->sell32Valid = false;

@@ -646,15 +646,19 @@ __global__ __launch_bounds__(kBlock) void kGatherVec(int64_t n, const uint32_t* 
 // diag * e_r (rhs_r = diag g_r); with symmetric elimination the flagged
 // columns of the other rows are zeroed and rhs_r -= sum A_rc g_c, summed from
 // the FP64 values when present (else the FP32 ones) over the row's lanes.
+// Rows are list[0 .. count) (every row when list is null): rows outside the
+// list are rows the call leaves unchanged.
 template <int L>
-__global__ __launch_bounds__(kBlock) void kDirichletRows(int64_t n, const int32_t* __restrict__ ia,
+__global__ __launch_bounds__(kBlock) void kDirichletRows(int64_t count, const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ ia,
                                                          const int32_t* __restrict__ ja, double* __restrict__ v64,
                                                          float* __restrict__ v32, double* __restrict__ rhs,
                                                          const uint8_t* __restrict__ flag,
                                                          const double* __restrict__ g, int symmetric, double diag) {
     const int lane = threadIdx.x & (L - 1);
     constexpr int RPB = kBlock / L;
-    for (int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / L; r < n; r += (int64_t)gridDim.x * RPB) {
+    for (int64_t i = (int64_t)blockIdx.x * RPB + threadIdx.x / L; i < count; i += (int64_t)gridDim.x * RPB) {
+        const int64_t r = list ? (int64_t)list[i] : i;
         const int b = ia[r], e = ia[r + 1];
         if (flag[r]) {
             for (int p = b + lane; p < e; p += L) {
@@ -679,6 +683,23 @@ __global__ __launch_bounds__(kBlock) void kDirichletRows(int64_t n, const int32_
                 if (lane == 0) rhs[r] -= corr;
             }
         }
+    }
+}
+
+// mark[r] = 1 for the rows symmetric elimination changes: flagged rows and
+// rows holding a flagged column (L lanes per row)
+template <int L>
+__global__ __launch_bounds__(kBlock) void kMarkTouched(int64_t n, const int32_t* __restrict__ ia,
+                                                       const int32_t* __restrict__ ja, const uint8_t* __restrict__ flag,
+                                                       uint8_t* __restrict__ mark) {
+    const int lane = threadIdx.x & (L - 1);
+    constexpr int RPB = kBlock / L;
+    for (int64_t r = (int64_t)blockIdx.x * RPB + threadIdx.x / L; r < n; r += (int64_t)gridDim.x * RPB) {
+        int any = 0;
+        for (int p = ia[r] + lane; p < ia[r + 1]; p += L) any |= flag[ja[p]];
+#pragma unroll
+        for (int off = L / 2; off > 0; off >>= 1) any |= __shfl_xor(any, off, L);
+        if (lane == 0) mark[r] = (uint8_t)(any | flag[r]);
     }
 }
 

@@ -140,6 +140,19 @@ def test_dirichlet_vs_oracle(ctx, orc, mode):
     assert rel_fro(A.values(), val) < TOL64
     assert rel_fro(bb.get(), b) < TOL64
     assert np.array_equal(A.values(fa.FP32) == 0, A.values() == 0)   # FP32 copy eliminated alike
+    # the same row set on a second graph of the same pattern (its rows to touch
+    # are listed per graph) and again on the first: the same bits each time;
+    # duplicated row ids change nothing
+    g2 = fa.Graph(sp, ia, ja)
+    dup = np.concatenate([np.nonzero(isb)[0], np.nonzero(isb)[0][::3]])
+    for graph, rs in ((g2, rows), (g, rows), (g2, fa.Rows(ctx, ndof, dup))):
+        A2 = fa.Matrix(graph, fa.FP64_FP32)
+        A2.assemble(fa.DIFFUSION, 4)
+        b2 = ctx.vector(ndof)
+        fa.assemble_linear(sp, b2, fa.SRC_CONST, 2, (1.0,))
+        A2.zero_rows(rs, ctx.vector(ndof, gval), b2, mode=mode)
+        assert np.array_equal(A2.values(), A.values())
+        assert np.array_equal(b2.get(), bb.get())
 
 
 def test_full_size_p2_properties(ctx):
