@@ -686,12 +686,12 @@ def main():
     for f, timer in (("mixed", "spmv_fp32"), ("mixed_element", "elemop_fp32"), ("fp64", "spmv_fp64")):
         if f in results and f != best and results[f]["timers"][timer][0] > 0:
             others[f] = spmv_roofline(f, timer)
-    # scatter (owner-computes gather of element contributions into CSR values, FP64 + FP32 copies):
-    # algorithmic bytes = nnz*(8+4) values + E*nloc*4 elem2dof + E*8 coefficient + E*4*4 + V*3*8 geometry
+    # scatter (owner-computes gather of element contributions into the FP64 CSR values):
+    # algorithmic bytes = nnz*8 values + E*nloc*4 elem2dof + E*8 coefficient + E*4*4 + V*3*8 geometry
     ng, gms = R["timers"]["assemble_gather"]
     nv = (args.n + 1) ** 3
     nloc = (order + 1) * (order + 2) * (order + 3) // 6
-    bytes_scatter = 12 * nnz + prob.ne * nloc * 4 + prob.ne * 8 + prob.ne * 16 + nv * 24
+    bytes_scatter = 8 * nnz + prob.ne * nloc * 4 + prob.ne * 8 + prob.ne * 16 + nv * 24
     g_ms = gms / max(ng, 1)
     g_ach = bytes_scatter / (g_ms * 1e-3) / 1e9 if ng else 0.0
     scatter = {"kernel": "kGatherScalar<100,8,true>" if order == 2 else "kGatherScalar", "bound": "hbm",

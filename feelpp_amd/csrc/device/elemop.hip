@@ -306,7 +306,8 @@ namespace feelpp_amd {
 
 bool elemOpCandidate(const feel_hip_mat* A) {
     const feel_hip_space* s = A->g->space;
-    return s->dim == 3 && s->order == 2 && s->ncomp == 1 && !A->g->block && A->g->perm && A->v32;
+    return s->dim == 3 && s->order == 2 && s->ncomp == 1 && !A->g->block && A->g->perm &&
+           (A->precision & FEEL_HIP_FP32);
 }
 
 // Structure of the element operator from the space's elem2dof and the

@@ -681,8 +681,11 @@ int feel_hip_mat_create(feel_hip_graph* g, int precision, feel_hip_mat** out) {
     retain(g);
     A->precision = precision;
     int rc = FEEL_HIP_OK;
+    // FP32 CSR values are stored only by an FP32-only matrix: the FP32 values
+    // of an FP64 + FP32 matrix are its FP64 values rounded, formed where they
+    // are needed (the solver's FP32 SELL image, get_values)
     if (precision & FEEL_HIP_FP64) rc = devAlloc(c, &A->v64, (size_t)g->nnz);
-    if (rc == FEEL_HIP_OK && (precision & FEEL_HIP_FP32)) rc = devAlloc(c, &A->v32, (size_t)g->nnz);
+    if (rc == FEEL_HIP_OK && precision == FEEL_HIP_FP32) rc = devAlloc(c, &A->v32, (size_t)g->nnz);
     if (rc == FEEL_HIP_OK) rc = devAlloc(c, &A->dinv64, (size_t)g->n);
     if (rc == FEEL_HIP_OK) rc = devAlloc(c, &A->dinv32, (size_t)g->n);
     if (rc != FEEL_HIP_OK) {
@@ -767,8 +770,16 @@ int feel_hip_mat_get_values(feel_hip_mat* A, int precision, void* host) {
         if (!A->v64) return setError(FEEL_HIP_EINVAL, "matrix holds no FP64 values");
         FEEL_HIP_CHECK(hipMemcpyAsync(host, A->v64, sizeof(double) * A->g->nnz, hipMemcpyDeviceToHost, c->stream));
     } else if (precision == FEEL_HIP_FP32) {
-        if (!A->v32) return setError(FEEL_HIP_EINVAL, "matrix holds no FP32 values");
-        FEEL_HIP_CHECK(hipMemcpyAsync(host, A->v32, sizeof(float) * A->g->nnz, hipMemcpyDeviceToHost, c->stream));
+        if (!(A->precision & FEEL_HIP_FP32)) return setError(FEEL_HIP_EINVAL, "matrix holds no FP32 values");
+        if (A->v32) {
+            FEEL_HIP_CHECK(hipMemcpyAsync(host, A->v32, sizeof(float) * A->g->nnz, hipMemcpyDeviceToHost, c->stream));
+        } else {   // the FP64 values rounded
+            std::vector<double> v((size_t)A->g->nnz);
+            FEEL_HIP_CHECK(hipMemcpyAsync(v.data(), A->v64, sizeof(double) * v.size(), hipMemcpyDeviceToHost, c->stream));
+            FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+            float* out = static_cast<float*>(host);
+            for (size_t p = 0; p < v.size(); ++p) out[p] = (float)v[p];
+        }
     } else {
         return setError(FEEL_HIP_EINVAL, "precision must be FP64 or FP32");
     }
@@ -1026,7 +1037,9 @@ int feel_hip_assemble_bilinear(feel_hip_mat* A, const feel_hip_bilinear_desc* d,
     feel_hip_space* s = g->space;
     feel_hip_ctx* c = s->ctx;
     const int wantPrec = d->precision ? d->precision : A->precision;
-    double* v64 = (wantPrec & FEEL_HIP_FP64) ? A->v64 : nullptr;
+    // an FP64 + FP32 matrix keeps FP32 values as its FP64 values rounded, so
+    // either request assembles its FP64 values
+    double* v64 = (wantPrec & FEEL_HIP_FP64) || !A->v32 ? A->v64 : nullptr;
     float* v32 = (wantPrec & FEEL_HIP_FP32) ? A->v32 : nullptr;
     if (!v64 && !v32) return setError(FEEL_HIP_EINVAL, "matrix holds none of the requested precisions");
     elemOpInvalidate(A);

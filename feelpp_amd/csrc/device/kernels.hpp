@@ -1227,15 +1227,17 @@ __global__ __launch_bounds__(kBlock) void kCsrToSell(int64_t nslots, const int32
 }
 
 // Both SELL value images in one pass over the slot map: FP64 from the FP64
-// CSR values, FP32 from the FP32 ones.
+// CSR values, FP32 from the FP32 ones (the FP64 ones rounded when the matrix
+// stores none).
 static __global__ __launch_bounds__(kBlock) void kCsrToSellBoth(int64_t nslots, const int32_t* __restrict__ slot2csr,
                                                                 const double* __restrict__ c64,
                                                                 const float* __restrict__ c32,
                                                                 double* __restrict__ s64, float* __restrict__ s32) {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nslots; s += (int64_t)gridDim.x * blockDim.x) {
         const int32_t p = slot2csr[s];
-        s64[s] = p < 0 ? 0.0 : c64[p];
-        s32[s] = p < 0 ? 0.0f : c32[p];
+        const double v = p < 0 ? 0.0 : c64[p];
+        s64[s] = v;
+        s32[s] = c32 ? (p < 0 ? 0.0f : c32[p]) : (float)v;
     }
 }
 

@@ -127,7 +127,7 @@ int ensureSell(feel_hip_mat* A, bool need64, bool need32) {
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
     bool changed = false;
-    if (need64 && need32 && !A->sell64Valid && !A->sell32Valid && A->v64 && A->v32) {
+    if (need64 && need32 && !A->sell64Valid && !A->sell32Valid && A->v64) {
         TRY_CALL(allocWork(c, &A->s64, g->nslots));
         TRY_CALL(allocWork(c, &A->s32, g->nslots));
         hipLaunchKernelGGL(kCsrToSellBoth, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,

@@ -196,6 +196,8 @@ int64_t feel_hip_graph_nnz(const feel_hip_graph* g);
 
 /* ---- matrices (MatrixSparse<double>, feelalg/matrixsparse.hpp:97) -------- */
 int feel_hip_mat_create(feel_hip_graph* g, int precision, feel_hip_mat** out);   /* Backend::newMatrix, backend.hpp:349-440 */
+/* FEEL_HIP_FP64_FP32: the FP32 values are always the FP64 values rounded (no
+ * FP32 CSR copy is kept; get_values(FP32) and the FP32 solver image round). */
 int feel_hip_mat_destroy(feel_hip_mat* A);
 int feel_hip_mat_zero(feel_hip_mat* A);                                          /* zero(), matrixsparse.hpp:420 */
 /* addMatrix(rows, nrows, cols, ncols, data) ADD semantics, data row-major (matrixsparse.hpp:496-498) */
