@@ -91,9 +91,11 @@ def time_form(prob, form, rtol, steps, warmup, inner_rtol):
         prob.step(precision, rtol, inner_rtol, op)
     ctx.synchronize()
     ctx.reset_timing()
-    # events around every 16th launch of each timed kernel: the kernel averages
-    # come from the timed region while the events' own cost stays out of it
-    ctx.set_timing(16)
+    # events around every 64th launch of each timed kernel: the kernel averages
+    # come from the timed region (about 16 samples per solve) while the events
+    # cost little of it (each recorded pair leaves the queue idle ~8 us on
+    # either side; every 16th launch cost 1.2 % of a C2 step)
+    ctx.set_timing(64)
     lat = []
     its = []
     t0 = time.perf_counter()

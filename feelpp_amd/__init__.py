@@ -89,6 +89,7 @@ _SIGS = {
     "feel_quadrature": (C.c_int, [C.c_int, C.c_int, _vp, _vp]),
     "feel_reference_tensors": (C.c_int, [C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "feel_cg_condition_estimate": (C.c_int, [_vp, C.c_int, _vp]),
+    "feel_cg_condition_exceeds": (C.c_int, [_vp, C.c_int, C.c_double, _vp]),
     # device
     "feel_hip_last_error": (C.c_char_p, []),
     "feel_hip_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
@@ -313,6 +314,19 @@ def cg_condition_estimate(gamma, pap):
     out = C.c_double()
     _check(lib().feel_cg_condition_estimate(h.ctypes.data_as(_vp), k, C.byref(out)), host=True)
     return out.value
+
+
+def cg_condition_exceeds(gamma, pap, limit):
+    """Whether cg_condition_estimate(gamma, pap) exceeds limit, decided from
+    the largest eigenvalue to 1 % and one Sturm count (the FP32 solve's test)."""
+    gamma, pap = np.asarray(gamma, dtype=np.float64), np.asarray(pap, dtype=np.float64)
+    k = len(pap)
+    h = np.ones(2 * (k + 1))
+    h[0::2][:len(gamma)] = gamma[:k + 1]
+    h[1:2 * k:2] = pap
+    out = C.c_int()
+    _check(lib().feel_cg_condition_exceeds(h.ctypes.data_as(_vp), k, float(limit), C.byref(out)), host=True)
+    return bool(out.value)
 
 
 # ============================================================ device side

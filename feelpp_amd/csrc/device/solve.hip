@@ -317,8 +317,9 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
         std::vector<double> hh(2 * (size_t)k);
         FEEL_HIP_CHECK(hipMemcpy(hh.data(), c->hist + 2 * (size_t)h.fresh, sizeof(double) * hh.size(),
                                  hipMemcpyDeviceToHost));
-        // an order-of-magnitude test: three digits are plenty
-        ill = cgConditionEstimate(hh.data(), k, 1e-3) > kMixedKappaLimit;
+        // an order-of-magnitude test: the largest eigenvalue to 1 % and one
+        // Sturm count at lmax / limit, instead of resolving the smallest
+        ill = cgConditionExceeds(hh.data(), k, kMixedKappaLimit);
         return FEEL_HIP_OK;
     };
     while (true) {

@@ -212,4 +212,15 @@ int feel_cg_condition_estimate(const double* gamma_pap, int k, double* kappa) {
     HOST_CATCH
 }
 
+int feel_cg_condition_exceeds(const double* gamma_pap, int k, double limit, int* exceeds) {
+    HOST_TRY
+    if (!gamma_pap || !exceeds || k < 0 || !(limit > 0)) return (int)fail("bad argument");
+    for (int j = 0; j < k; ++j)
+        if (!(gamma_pap[2 * j] > 0) || !(gamma_pap[2 * j + 1] > 0))
+            return (int)fail("coefficients must be positive");
+    *exceeds = cgConditionExceeds(gamma_pap, k, limit) ? 1 : 0;
+    return 0;
+    HOST_CATCH
+}
+
 }  // extern "C"

@@ -66,6 +66,11 @@ int feel_reference_tensors(int dim, int order, int qorder, double* R, double* M,
  * the FP32 solve uses it to decide when the matrix is too ill-conditioned for
  * FP32 and the rest of the solve must run in FP64. */
 int feel_cg_condition_estimate(const double* gamma_pap, int k, double* kappa);
+/* Whether that estimate exceeds limit (the test the FP32 solve makes): the
+ * largest eigenvalue to 1 % and one Sturm count, so it agrees with
+ * estimate > limit except within about 1 % of the limit.  No Feel++
+ * counterpart. */
+int feel_cg_condition_exceeds(const double* gamma_pap, int k, double limit, int* exceeds);
 
 #ifdef __cplusplus
 }

@@ -199,6 +199,11 @@ def test_cg_condition_estimate(orc):
     assert fa.cg_condition_estimate(gam[:6], pap[:5]) <= kappa * (1 + 1e-12)
     with pytest.raises(RuntimeError):
         fa.cg_condition_estimate([1.0, -1.0, 1.0], [1.0, 1.0])
+    # the solver's threshold test agrees with the estimate away from the limit
+    for n in (4, 6, 12, len(pap)):
+        e = fa.cg_condition_estimate(gam[:n + 1], pap[:n])
+        for lim in (e / 3, e / 1.05, e * 1.05, e * 3):
+            assert fa.cg_condition_exceeds(gam[:n + 1], pap[:n], lim) == (e > lim), (n, e, lim)
 
 
 @pytest.mark.parametrize("dim,order,ncomp", [(2, 1, 1), (2, 2, 1), (3, 2, 1), (3, 3, 3)])
