@@ -35,6 +35,10 @@ constexpr int kReduceBlocks = 1024;   // grid of the reduction-carrying vector k
 constexpr int kBlock = 256;
 constexpr int kSpmvGrid = 2048;       // default grid of the solver SpMV (one p.Ap partial per block)
 constexpr int kSpmvPartCap = 4096;    // largest SpMV grid (partials buffer capacity)
+// the context's partials buffer: z.r and z.z (set A), FP64 residual, SpMV
+// p.Ap, SpMV w.D^{-1}w, z.r and z.z (set B: the pipelined FP32 iteration
+// reads one set and writes the other)
+constexpr int kPartDoubles = 5 * kReduceBlocks + 2 * kSpmvPartCap;
 constexpr int kSpmvBlock = 1024;
 constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
 constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
@@ -102,7 +106,7 @@ struct feel_hip_ctx {
     size_t evUsed = 0;
     std::map<std::string, std::pair<int64_t, double>> timers;
     // CG scratch: partial sums (3 arrays of kReduceBlocks doubles) and state
-    double* part = nullptr;   // 3 x kReduceBlocks + kSpmvPartCap doubles
+    double* part = nullptr;   // kPartDoubles doubles
     feelpp_amd::CgState* state = nullptr;
     feelpp_amd::CgState* hstate = nullptr;   // pinned host mirror
     double* hist = nullptr;   // 2 x kHistMax CG coefficients (beta, p.Ap) of the running solve
@@ -205,9 +209,15 @@ struct feel_hip_mat {
     double* s64 = nullptr;
     float* s32 = nullptr;
     bool sell64Valid = false, sell32Valid = false;
+    // scalar image: s32 holds D^{-1/2} A D^{-1/2} (rounded from the FP64
+    // values; every diagonal entry positive), dsq64 = D^{-1/2}
+    bool s32Scaled = false;
+    double* dsq64 = nullptr;
+    int* diagFlag = nullptr;   // set by kDiagSqrt when a diagonal entry is not positive
     // solver work vectors, allocated on first use and kept with the matrix
     double *r64 = nullptr, *z64 = nullptr, *p64 = nullptr, *w64 = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *w32 = nullptr, *x32 = nullptr, *b32 = nullptr;
+    float* q32 = nullptr;   // second search-direction buffer of the pipelined FP32 iteration
     double *bp64 = nullptr, *xp64 = nullptr;   // b and x in the solver numbering
     // element-by-element FP32 operator: 1 while the values are a P2 diffusion
     // assembly followed only by symmetric unit-diagonal eliminations

@@ -132,7 +132,7 @@ void unref(feel_hip_ctx* c) {
     if (--c->refs > 0) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->release(c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvPartCap));
+    c->release(c->part, sizeof(double) * kPartDoubles);
     c->release(c->state, sizeof(CgState) + 4 * sizeof(double));
     c->release(c->hist, sizeof(double) * 2 * kHistMax);
     c->release(c->elemmat, c->elemmatBytes);
@@ -217,7 +217,7 @@ int feel_hip_ctx_create(int device, feel_hip_ctx** out) {
         delete c;
         return setError(FEEL_HIP_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
     }
-    int rc = c->alloc((void**)&c->part, sizeof(double) * (3 * kReduceBlocks + kSpmvPartCap));
+    int rc = c->alloc((void**)&c->part, sizeof(double) * kPartDoubles);
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->state, sizeof(CgState) + 4 * sizeof(double));
     if (rc == FEEL_HIP_OK) rc = c->alloc((void**)&c->hist, sizeof(double) * 2 * kHistMax);
     if (rc == FEEL_HIP_OK && hipHostMalloc((void**)&c->hstate, sizeof(CgState), hipHostMallocDefault) != hipSuccess)
@@ -704,9 +704,11 @@ int feel_hip_mat_destroy(feel_hip_mat* A) {
     c->release(A->v32, sizeof(float) * A->g->nnz);
     c->release(A->dinv64, sizeof(double) * A->g->n);
     c->release(A->dinv32, sizeof(float) * A->g->n);
+    c->release(A->dsq64, sizeof(double) * A->g->n);
+    c->release(A->diagFlag, sizeof(int));
     const int64_t n = A->g->n;
     for (double* v : {A->r64, A->z64, A->p64, A->w64}) c->release(v, sizeof(double) * n);
-    for (float* v : {A->r32, A->z32, A->p32, A->w32, A->x32, A->b32}) c->release(v, sizeof(float) * n);
+    for (float* v : {A->r32, A->z32, A->p32, A->q32, A->w32, A->x32, A->b32}) c->release(v, sizeof(float) * n);
     for (double* v : {A->bp64, A->xp64}) c->release(v, sizeof(double) * n);
     c->release(A->s64, sizeof(double) * A->g->nslots);
     c->release(A->s32, sizeof(float) * A->g->nslots);

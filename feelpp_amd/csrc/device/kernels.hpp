@@ -48,10 +48,36 @@ __device__ inline bool blockDone(const CgState* st) {
     return flag != 0;
 }
 
-// Sum of count partials, identical in every block: four independent
-// accumulators per thread so that its loads are in flight together, then a
+// N block sums at once (one pair of barriers instead of N); each is combined
+// in blockReduceSum's order, so the results are the same bits; valid in every
+// thread
+template <int BS, int N>
+__device__ inline void blockReduceSums(double (&v)[N]) {
+    __shared__ double sh[N][BS / 64];
+    __shared__ double total[N];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double t = waveReduceSum(v[k]);
+        if (lane == 0) sh[k][w] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double t = 0;
+#pragma unroll
+        for (int i = 0; i < BS / 64; ++i) t += sh[threadIdx.x][i];
+        total[threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = total[k];
+    __syncthreads();
+}
+
+// This thread's share of the sum of count partials (kBlock threads): four
+// independent accumulators so that its loads are in flight together, then a
 // fixed combination order.
-__device__ inline double reducePartials(const double* __restrict__ part, int count = kReduceBlocks) {
+__device__ inline double partialShare(const double* __restrict__ part, int count) {
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
     int i = threadIdx.x;
     for (; i + 3 * kBlock < count; i += 4 * kBlock) {
@@ -61,7 +87,12 @@ __device__ inline double reducePartials(const double* __restrict__ part, int cou
         v3 += part[i + 3 * kBlock];
     }
     for (; i < count; i += kBlock) v0 += part[i];
-    return blockReduceSum((v0 + v1) + (v2 + v3));
+    return (v0 + v1) + (v2 + v3);
+}
+
+// Sum of count partials, identical in every block.
+__device__ inline double reducePartials(const double* __restrict__ part, int count = kReduceBlocks) {
+    return blockReduceSum(partialShare(part, count));
 }
 
 // 16-byte vector of T (four floats or two doubles) for the streaming CG
@@ -732,19 +763,22 @@ __global__ __launch_bounds__(kBlock) void kSpmv(int64_t n, const int32_t* __rest
     }
 }
 
-// p = z + (beta/beta_prev) p, preceded by the convergence test on the
-// preconditioned residual norm (KSPConvergedDefault with UIRNorm).  With
-// hist, block 0 records beta = (r, z) of iteration it at hist[2 it] (the
-// update kernel records p.Ap at hist[2 it + 1]): the Lanczos coefficients of
-// the run, from which the host estimates the condition number.
+// p = z + (beta/beta_prev) q, preceded by the convergence test on the
+// preconditioned residual norm (KSPConvergedDefault with UIRNorm); q is the
+// previous direction (p itself, or the other buffer of the pipelined FP32
+// iteration).  With hist, block 0 records beta = (r, z) of iteration it at
+// hist[2 it] (the update kernel records p.Ap at hist[2 it + 1]): the Lanczos
+// coefficients of the run, from which the host estimates the condition number.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const T* __restrict__ z,
-                                                       T* __restrict__ p, const double* __restrict__ zrPart,
+__global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const T* __restrict__ z, const T* q, T* p,
+                                                       const double* __restrict__ zrPart,
                                                        const double* __restrict__ zzPart, CgState* st,
                                                        double* betaSlots, double* hist) {
     if (blockDone(st)) return;
-    const double beta = reducePartials(zrPart);
-    const double dp = sqrt(reducePartials(zzPart));
+    double sums[2] = {partialShare(zrPart, kReduceBlocks), partialShare(zzPart, kReduceBlocks)};
+    blockReduceSums<kBlock, 2>(sums);
+    const double beta = sums[0];
+    const double dp = sqrt(sums[1]);
     const double ttol = it == 0 ? fmax(st->rtol * dp, st->atol) : st->ttol;
     const double rnorm0 = it == 0 ? dp : st->rnorm0;
     int reason = 0;
@@ -768,18 +802,19 @@ __global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const 
     const double bb = first ? 0.0 : beta / betaSlots[(it - 1) & 1];
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
     int64_t tail = 0;
-    if (aligned16(z, p)) {
+    if (aligned16(z, q, p)) {
         typedef typename Vec16<T>::type VT;
         constexpr int V = 16 / sizeof(T);
         const int64_t nv = n / V;
         const VT* zv = reinterpret_cast<const VT*>(z);
+        const VT* qv = reinterpret_cast<const VT*>(q);
         VT* pv = reinterpret_cast<VT*>(p);
         for (int64_t i = gid; i < nv; i += stride) {
             const VT zi = zv[i];
             if (first) {
                 pv[i] = zi;
             } else {
-                VT pi = pv[i];
+                VT pi = qv[i];
 #pragma unroll
                 for (int l = 0; l < V; ++l) pi[l] = (T)((double)zi[l] + bb * (double)pi[l]);
                 pv[i] = pi;
@@ -787,7 +822,7 @@ __global__ __launch_bounds__(kBlock) void kCgDirection(int64_t n, int it, const 
         }
         tail = nv * V;
     }
-    for (int64_t i = tail + gid; i < n; i += stride) p[i] = first ? z[i] : (T)((double)z[i] + bb * (double)p[i]);
+    for (int64_t i = tail + gid; i < n; i += stride) p[i] = first ? z[i] : (T)((double)z[i] + bb * (double)q[i]);
 }
 
 // x += a p, r -= a w, z = D^{-1} r; partials of z.r and z.z.  dpiPart holds
@@ -844,9 +879,111 @@ __global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __rest
         tail = nv * V;
     }
     for (int64_t i = tail + gid; i < n; i += stride) step(x[i], r[i], z[i], p[i], w[i], dinv[i]);
-    zr = blockReduceSum(zr);
-    zz = blockReduceSum(zz);
-    if (threadIdx.x == 0) { zrPart[blockIdx.x] = zr; zzPart[blockIdx.x] = zz; }
+    double sums[2] = {zr, zz};
+    blockReduceSums<kBlock, 2>(sums);
+    if (threadIdx.x == 0) { zrPart[blockIdx.x] = sums[0]; zzPart[blockIdx.x] = sums[1]; }
+}
+
+// One pipelined Jacobi-PCG iteration (the FP32 inner solve): the update of
+// iteration it (x += a p, r -= a w, z = D^{-1} r, partials of z.r and z.z into
+// zrOut/zzOut) fused with the next search direction q = z + b p.  With
+// w = Ap and a = (r, z) / (p, w):
+// (r', z') = (r, z) - 2 a (w, z) + a^2 (w, D^{-1} w), and (w, z) = (w, p) since
+// p = z + b_prev p_prev is A-conjugate to p_prev, so (r', z') =
+// a^2 (w, D^{-1} w) - (r, z) (the recurrence behind single-reduction CG
+// variants, Chronopoulos and Gear 1989).  b = (r', z') / (r, z) then needs only
+// the SpMV's partials, and the direction kernel of the classic iteration is
+// not launched.  The true (r', z')
+// and (z', z') partials go to the next launch, whose first step is the stopping
+// test on them (kCgDirection's test; at it == st->fresh that kernel ran it and
+// stored (r, z)).  The partial sets alternate with it, since every block reads
+// the previous launch's set while blocks that finished write theirs.
+// SCALED: the iteration runs on D^{-1/2} A D^{-1/2} (unit diagonal), the same
+// Jacobi-PCG in exact arithmetic (x~ = D^{1/2} x, r~ = D^{-1/2} r), where z = r
+// and dinv is not read.
+template <typename T, bool SCALED>
+__global__ __launch_bounds__(kBlock) void kCgStep(int64_t n, int it, T* __restrict__ x, T* __restrict__ r,
+                                                  const T* __restrict__ p, T* __restrict__ q, const T* __restrict__ w,
+                                                  const T* __restrict__ dinv, const double* __restrict__ dpiPart,
+                                                  const double* __restrict__ wdwPart, int npart,
+                                                  const double* __restrict__ zrIn, const double* __restrict__ zzIn,
+                                                  double* __restrict__ zrOut, double* __restrict__ zzOut, CgState* st,
+                                                  double* betaSlots, double* dpiSlots, double* hist) {
+    if (blockDone(st)) return;
+    // the four scalars of the step in one block reduction
+    const bool test = it > st->fresh;
+    double sums[4] = {test ? partialShare(zrIn, kReduceBlocks) : 0.0, test ? partialShare(zzIn, kReduceBlocks) : 0.0,
+                      partialShare(dpiPart, npart), partialShare(wdwPart, npart)};
+    blockReduceSums<kBlock, 4>(sums);
+    double rz;
+    if (test) {
+        rz = sums[0];
+        const double dp = sqrt(sums[1]);
+        int reason = 0;
+        if (dp != dp) reason = -9;
+        else if (dp <= st->ttol) reason = dp < st->atol ? 3 : 2;
+        else if (dp >= st->dtol * st->rnorm0) reason = -4;
+        else if (it >= st->maxit) reason = -3;
+        else if (rz == 0.0) reason = 3;
+        else if (rz * betaSlots[(it - 1) & 1] < 0.0) reason = -8;   // indefinite preconditioner
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->dp = dp;
+            if (reason) { st->done = 1; st->reason = reason; st->its = reason == -8 ? it + 1 : it; }
+            betaSlots[it & 1] = rz;
+            if (hist && it < kHistMax) hist[2 * it] = rz;
+        }
+        if (reason) return;
+    } else {
+        rz = betaSlots[it & 1];
+    }
+    const double dpi = sums[2];
+    const bool indefinite = dpi == 0.0 || (it > 0 && dpi * dpiSlots[(it - 1) & 1] <= 0.0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dpiSlots[it & 1] = dpi;
+        if (hist && it < kHistMax) hist[2 * it + 1] = dpi;
+        if (indefinite) { st->done = 1; st->reason = -10; st->its = it + 1; }
+    }
+    if (indefinite) return;
+    const double a = rz / dpi;
+    double b = (a * a * sums[3] - rz) / rz;
+    if (!(b > 0.0) || !(b < 1e300)) b = 0.0;   // the recurrence lost its sign: restart the direction
+    double zr = 0.0, zz = 0.0;
+    auto step = [&](T& xi, T& ri, T& qi, T pi, T wi, T di) {
+        xi = (T)((double)xi + a * (double)pi);
+        ri = (T)((double)ri - a * (double)wi);
+        const T zi = SCALED ? ri : di * ri;
+        qi = (T)((double)zi + b * (double)pi);
+        zr += (double)zi * (double)ri;
+        zz += (double)zi * (double)zi;
+    };
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t tail = 0;
+    if (aligned16(x, r, p, q, w, SCALED ? x : dinv)) {
+        typedef typename Vec16<T>::type VT;
+        constexpr int V = 16 / sizeof(T);
+        const int64_t nv = n / V;
+        for (int64_t i = gid; i < nv; i += stride) {
+            VT xv = reinterpret_cast<const VT*>(x)[i], rv = reinterpret_cast<const VT*>(r)[i], qv, dv;
+            const VT pv = reinterpret_cast<const VT*>(p)[i], wv = reinterpret_cast<const VT*>(w)[i];
+            if constexpr (!SCALED) dv = reinterpret_cast<const VT*>(dinv)[i];
+#pragma unroll
+            for (int l = 0; l < V; ++l) {
+                T xi = xv[l], ri = rv[l], qi;
+                step(xi, ri, qi, pv[l], wv[l], dv[l]);
+                xv[l] = xi;
+                rv[l] = ri;
+                qv[l] = qi;
+            }
+            reinterpret_cast<VT*>(x)[i] = xv;
+            reinterpret_cast<VT*>(r)[i] = rv;
+            reinterpret_cast<VT*>(q)[i] = qv;
+        }
+        tail = nv * V;
+    }
+    for (int64_t i = tail + gid; i < n; i += stride) step(x[i], r[i], q[i], p[i], w[i], SCALED ? (T)1 : dinv[i]);
+    double out[2] = {zr, zz};
+    blockReduceSums<kBlock, 2>(out);
+    if (threadIdx.x == 0) { zrOut[blockIdx.x] = out[0]; zzOut[blockIdx.x] = out[1]; }
 }
 
 // r = b - w (w = A x0), z = D^{-1} r; partials of z.r and z.z.
@@ -889,10 +1026,66 @@ static __global__ __launch_bounds__(kBlock) void kScaleToF32(int64_t n, const do
         out[i] = (float)(s * a[i]);
 }
 
+// x += a (s o d) (s = D^{-1/2} when the FP32 iteration ran on the scaled
+// image, else 1)
 static __global__ __launch_bounds__(kBlock) void kAxpyF32(int64_t n, double a, const float* __restrict__ d,
-                                                   double* __restrict__ x) {
+                                                   const double* __restrict__ s, double* __restrict__ x) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        x[i] += a * (double)d[i];
+        x[i] += a * (s ? s[i] * (double)d[i] : (double)d[i]);
+}
+
+// D^{-1/2} from D^{-1}; flag = 1 when some entry is not positive (the scaled
+// image then does not exist)
+static __global__ __launch_bounds__(kBlock) void kDiagSqrt(int64_t n, const double* __restrict__ dinv,
+                                                    double* __restrict__ dsq, int* __restrict__ flag) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double d = dinv[i];
+        if (d > 0.0) {
+            dsq[i] = sqrt(d);
+        } else {
+            dsq[i] = 1.0;
+            *flag = 1;
+        }
+    }
+}
+
+// The FP32 scalar image of D^{-1/2} A D^{-1/2} from the FP64 one: one
+// wavefront per slice, 64 consecutive slots per step (slot base + o of the
+// slice belongs to lane ((o >> 2) & 63), i.e. row 64 s + that lane; padding
+// slots hold 0 and stay 0)
+static __global__ __launch_bounds__(kBlock) void kSellScale32(int64_t n, int64_t nslices,
+                                                       const uint32_t* __restrict__ slicePtr,
+                                                       const int32_t* __restrict__ scol,
+                                                       const double* __restrict__ s64,
+                                                       const double* __restrict__ dsq, float* __restrict__ s32) {
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nslices; s += nw) {
+        const uint32_t base = slicePtr[s], len = slicePtr[s + 1] - base;
+        for (uint32_t o = lane; o < len; o += 64) {
+            const int64_t row = s * 64 + ((o >> 2) & 63);
+            const size_t q = (size_t)base + o;
+            const double ds = row < n ? dsq[row] : 0.0;
+            s32[q] = (float)(s64[q] * ds * dsq[scol[q]]);
+        }
+    }
+}
+
+// reliable-update restart of the scaled FP32 iteration: r~ = D^{-1/2} r64 / sigma,
+// x~ = 0, partials of r~.r~ into both z.r and z.z (z = r on the scaled image)
+static __global__ __launch_bounds__(kBlock) void kCgScaledReset(int64_t n, const double* __restrict__ r64,
+                                                         const double* __restrict__ dsq, double invSigma,
+                                                         float* __restrict__ r, float* __restrict__ xs,
+                                                         double* __restrict__ zrPart, double* __restrict__ zzPart) {
+    double rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float ri = (float)(r64[i] * dsq[i] * invSigma);
+        r[i] = ri;
+        xs[i] = 0.0f;
+        rr += (double)ri * (double)ri;
+    }
+    rr = blockReduceSum(rr);
+    if (threadIdx.x == 0) { zrPart[blockIdx.x] = rr; zzPart[blockIdx.x] = rr; }
 }
 
 // Jacobi: D^{-1} from the stored diagonal; PCJACOBI treats a zero diagonal as 1.
@@ -1093,25 +1286,36 @@ __device__ inline T sellRow(const SellCols& S, int64_t s, int lane, const T* __r
 
 // y = A x on the SELL-C-sigma image: one wavefront per 64-row slice, one lane
 // per row, grid-stride over slices with a fixed grid.  With part, also the
-// block partial of x.y (p.Ap).
+// block partial of x.y (p.Ap); with part2, that of y.D^{-1}y (dinv = D^{-1}),
+// which the pipelined CG iteration needs (y.y when dinv is null: the
+// symmetrically scaled image).
 template <typename T, int BS, int U, bool COMPACT>
 __global__ __launch_bounds__(BS) void kSellSpmv(int64_t nrows, int64_t nslices, SellCols S, const T* __restrict__ val,
                                                 const T* __restrict__ x, T* __restrict__ y, double* __restrict__ part,
-                                                const CgState* __restrict__ st) {
+                                                const CgState* __restrict__ st, const T* __restrict__ dinv,
+                                                double* __restrict__ part2) {
     if (st && blockDone(st)) return;
     const int lane = threadIdx.x & 63;
     const int64_t wave = ((int64_t)blockIdx.x * BS + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * BS) >> 6;
-    double dot = 0.0;
+    double dot = 0.0, dot2 = 0.0;
     for (int64_t s = wave; s < nslices; s += nwaves) {
         const T a = sellRow<T, U, COMPACT>(S, s, lane, val, x);
         const int64_t r = s * 64 + lane;
         if (r < nrows) {
             y[r] = a;
             if (part) dot += (double)x[r] * (double)a;
+            if (part2) dot2 += (double)a * (double)a * (dinv ? (double)dinv[r] : 1.0);
         }
     }
-    if (part) {
+    if (part2) {
+        double t[2] = {dot, dot2};
+        blockReduceSums<BS, 2>(t);
+        if (threadIdx.x == 0) {
+            if (part) part[blockIdx.x] = t[0];
+            part2[blockIdx.x] = t[1];
+        }
+    } else if (part) {
         const double t = blockReduceSumN<BS>(dot);
         if (threadIdx.x == 0) part[blockIdx.x] = t;
     }

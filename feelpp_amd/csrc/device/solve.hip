@@ -38,15 +38,16 @@ constexpr double kShapeMargin = 0.95;
 
 template <typename T, int BS, bool COMPACT>
 void launchSpmv(feel_hip_ctx* c, const feel_hip_graph* g, int grid, const T* val, const T* x, T* y, double* part,
-                const CgState* st) {
+                const CgState* st, const T* dinv, double* part2) {
     constexpr int U = 2;   // quads per step
     hipLaunchKernelGGL((kSellSpmv<T, BS, U, COMPACT>), dim3(grid), dim3(BS), 0, c->stream, g->n, g->nslices,
-                       sellCols(g), val, x, y, part, st);
+                       sellCols(g), val, x, y, part, st, dinv, part2);
 }
 
+// (dinv, part2: the y.D^{-1}y partials, scalar image only)
 template <typename T>
 void launchShape(feel_hip_ctx* c, const feel_hip_graph* g, const SpmvShape& sh, const T* val, const T* x, T* y,
-                 double* part, const CgState* st) {
+                 double* part, const CgState* st, const T* dinv = nullptr, double* part2 = nullptr) {
     if (g->block) {   // the block image: per-slice lanes per node row
         if (sh.block == 1024)
             hipLaunchKernelGGL((kBlockSellSpmv<T, 1024>), dim3(sh.grid), dim3(1024), 0, c->stream, g->nslices,
@@ -60,13 +61,13 @@ void launchShape(feel_hip_ctx* c, const feel_hip_graph* g, const SpmvShape& sh, 
         return;
     }
     if (g->compact) {
-        if (sh.block == 1024) launchSpmv<T, 1024, true>(c, g, sh.grid, val, x, y, part, st);
-        else if (sh.block == 512) launchSpmv<T, 512, true>(c, g, sh.grid, val, x, y, part, st);
-        else launchSpmv<T, 256, true>(c, g, sh.grid, val, x, y, part, st);
+        if (sh.block == 1024) launchSpmv<T, 1024, true>(c, g, sh.grid, val, x, y, part, st, dinv, part2);
+        else if (sh.block == 512) launchSpmv<T, 512, true>(c, g, sh.grid, val, x, y, part, st, dinv, part2);
+        else launchSpmv<T, 256, true>(c, g, sh.grid, val, x, y, part, st, dinv, part2);
     } else {
-        if (sh.block == 1024) launchSpmv<T, 1024, false>(c, g, sh.grid, val, x, y, part, st);
-        else if (sh.block == 512) launchSpmv<T, 512, false>(c, g, sh.grid, val, x, y, part, st);
-        else launchSpmv<T, 256, false>(c, g, sh.grid, val, x, y, part, st);
+        if (sh.block == 1024) launchSpmv<T, 1024, false>(c, g, sh.grid, val, x, y, part, st, dinv, part2);
+        else if (sh.block == 512) launchSpmv<T, 512, false>(c, g, sh.grid, val, x, y, part, st, dinv, part2);
+        else launchSpmv<T, 256, false>(c, g, sh.grid, val, x, y, part, st, dinv, part2);
     }
 }
 
@@ -100,16 +101,18 @@ int tuneSpmv(feel_hip_ctx* c, feel_hip_graph* g, const T* val, const T* x, T* y,
     return FEEL_HIP_OK;
 }
 
-// y = A x in the solver numbering (SELL image), optional p.Ap partials, with
-// the graph's tuned launch shape (tuned on the first call).  Returns the
-// number of partials written (the grid).
+// y = A x in the solver numbering (SELL image), optional p.Ap partials (and,
+// on the scalar image, y.D^{-1}y partials in part2), with the graph's tuned
+// launch shape (tuned on the first call).  Returns the number of partials
+// written (the grid).
 template <typename T>
 int spmv(feel_hip_ctx* c, feel_hip_graph* g, const T* val, const T* x, T* y, double* part, const CgState* st,
-         int* npart) {
+         int* npart, const T* dinv = nullptr, double* part2 = nullptr) {
     const int idx = sizeof(T) == 8 ? 0 : 1;
+    if (part2 && g->block) return setError(FEEL_HIP_EUNSUPPORTED, "y.D^-1y partials are computed on the scalar image only");
     if (g->spmvShape[idx] < 0) TRY_CALL(tuneSpmv<T>(c, g, val, x, y, c->part + 3 * kReduceBlocks));
     const SpmvShape& sh = kSpmvShapes[g->spmvShape[idx]];
-    launchShape<T>(c, g, sh, val, x, y, part, st);
+    launchShape<T>(c, g, sh, val, x, y, part, st, dinv, part2);
     if (npart) *npart = sh.grid;
     return FEEL_HIP_OK;
 }
@@ -122,36 +125,65 @@ int allocWork(feel_hip_ctx* c, T** p, int64_t n) {
     return c->alloc((void**)p, sizeof(T) * n);
 }
 
-// Refresh the SELL values (and the Jacobi diagonal, solver numbering) from the CSR values.
+// Refresh the SELL values (and the Jacobi diagonal, solver numbering) from the
+// CSR values.  On the scalar image the FP32 values are those of
+// D^{-1/2} A D^{-1/2}, rounded from the FP64 image, when every diagonal entry
+// is positive (s32Scaled: the FP32 iteration then runs on the scaled system);
+// otherwise, and on the block image, those of A.
 int ensureSell(feel_hip_mat* A, bool need64, bool need32) {
     feel_hip_graph* g = A->g;
     feel_hip_ctx* c = g->space->ctx;
     bool changed = false;
-    if (need64 && need32 && !A->sell64Valid && !A->sell32Valid && A->v64) {
+    const bool scalable = need32 && !g->block && A->v64;
+    if (need64 && need32 && !scalable && !A->sell64Valid && !A->sell32Valid && A->v64) {
         TRY_CALL(allocWork(c, &A->s64, g->nslots));
         TRY_CALL(allocWork(c, &A->s32, g->nslots));
         hipLaunchKernelGGL(kCsrToSellBoth, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,
                            A->v64, A->v32, A->s64, A->s32);
         A->sell64Valid = A->sell32Valid = changed = true;
+        A->s32Scaled = false;
     }
-    if (need64 && !A->sell64Valid) {
+    if ((need64 || scalable) && !A->sell64Valid) {
         if (!A->v64) return setError(FEEL_HIP_EINVAL, "matrix holds no FP64 values");
         TRY_CALL(allocWork(c, &A->s64, g->nslots));
         hipLaunchKernelGGL(kCsrToSell<double>, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,
                            A->v64, (const float*)nullptr, A->s64);
         A->sell64Valid = changed = true;
     }
-    if (need32 && !A->sell32Valid) {
+    if (need32 && !A->sell32Valid && !scalable) {
         TRY_CALL(allocWork(c, &A->s32, g->nslots));
         hipLaunchKernelGGL(kCsrToSell<float>, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots, g->slot2csr,
                            A->v32 ? (const double*)nullptr : A->v64, A->v32, A->s32);
         A->sell32Valid = changed = true;
+        A->s32Scaled = false;
     }
     if (changed || !A->dinvValid) {
         hipLaunchKernelGGL(kDiagInvSell, dim3(gridN(g->n)), dim3(kBlock), 0, c->stream, g->n, g->sdiag,
-                           A->sell64Valid ? A->s64 : nullptr, A->sell32Valid ? A->s32 : nullptr,
+                           A->sell64Valid ? A->s64 : nullptr,
+                           A->sell32Valid && !A->s32Scaled ? A->s32 : nullptr,
                            A->sell64Valid ? A->dinv64 : nullptr, A->dinv32);
         A->dinvValid = true;
+    }
+    if (scalable && !A->sell32Valid) {
+        TRY_CALL(allocWork(c, &A->s32, g->nslots));
+        TRY_CALL(allocWork(c, &A->dsq64, g->n));
+        TRY_CALL(allocWork(c, &A->diagFlag, 1));
+        FEEL_HIP_CHECK(hipMemsetAsync(A->diagFlag, 0, sizeof(int), c->stream));
+        hipLaunchKernelGGL(kDiagSqrt, dim3(gridN(g->n)), dim3(kBlock), 0, c->stream, g->n, A->dinv64, A->dsq64,
+                           A->diagFlag);
+        int bad = 0;
+        FEEL_HIP_CHECK(hipMemcpyAsync(&bad, A->diagFlag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+        if (!bad) {
+            const int grid = (int)std::min<int64_t>((g->nslices + kBlock / 64 - 1) / (kBlock / 64), 16384);
+            hipLaunchKernelGGL(kSellScale32, dim3(std::max(grid, 1)), dim3(kBlock), 0, c->stream, g->n, g->nslices,
+                               g->slicePtr, g->scol, A->s64, A->dsq64, A->s32);
+        } else {
+            hipLaunchKernelGGL(kCsrToSell<float>, dim3(gridN(g->nslots)), dim3(kBlock), 0, c->stream, g->nslots,
+                               g->slot2csr, A->v64, (const float*)nullptr, A->s32);
+        }
+        A->s32Scaled = !bad;
+        A->sell32Valid = true;
     }
     FEEL_HIP_CHECK(hipGetLastError());
     return FEEL_HIP_OK;
@@ -189,8 +221,8 @@ int runCg(feel_hip_mat* A, const T* val, const T* dinv, const T* b, T* x, bool x
     while (true) {
         const int chunk = checkEvery > 0 ? checkEvery : std::min(64, std::max(8, it / 4));
         for (int k = 0; k < chunk; ++k, ++it) {
-            hipLaunchKernelGGL(kCgDirection<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, z, p, zrPart, zzPart,
-                               st, betaSlots, (double*)nullptr);
+            hipLaunchKernelGGL(kCgDirection<T>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, z, (const T*)p, p,
+                               zrPart, zzPart, st, betaSlots, (double*)nullptr);
             c->tic(timerName);
             int npart = 0;
             TRY_CALL(spmv<T>(c, g, val, p, w, dpiPart, st, &npart));
@@ -250,9 +282,11 @@ constexpr int kKappaEvery = 128;
 constexpr int kKappaPollIts = 1024;
 
 // Jacobi-PCG in FP32 with reliable updates in FP64 (solver numbering, on
-// A->bp64 / A->xp64).  One FP32 CG (the direction / SpMV / update kernels of
-// the FP64 solver) runs on the scaled residual r / sigma and accumulates its
-// correction in xs.  Whenever its preconditioned residual norm has dropped by
+// A->bp64 / A->xp64).  One FP32 CG runs on the scaled residual r / sigma and
+// accumulates its correction in xs: on the assembled scalar image two kernels
+// per iteration (SpMV with the p.Ap and w.D^{-1}w partials, then kCgStep: the
+// update fused with the next direction), otherwise the direction / SpMV /
+// update kernels of the FP64 solver.  Whenever its preconditioned residual norm has dropped by
 // inner_rtol since the last reliable update, or reaches the tolerance, the
 // device stops and the host folds sigma * xs into the FP64 solution,
 // recomputes the true FP64 residual and tests it (KSPConvergedDefault on the
@@ -270,6 +304,20 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
     double* zrPart = c->part;
     double* zzPart = c->part + kReduceBlocks;
     double* dpiPart = c->part + 3 * kReduceBlocks;   // up to kSpmvPartCap partials
+    // the pipelined iteration (assembled FP32 values on the scalar image):
+    // w.D^{-1}w partials of the SpMV, and a second z.r / z.z set and search
+    // direction buffer, taken by iteration parity
+    const bool pipelined = !elemOp && !g->block;
+    // on the scaled FP32 image (every diagonal entry positive) the iteration
+    // runs on D^{-1/2} A D^{-1/2}: no D^{-1} stream in the SpMV or the update
+    const bool scaled = pipelined && A->s32Scaled;
+    double* wdwPart = dpiPart + kSpmvPartCap;
+    double* zrPartB = wdwPart + kSpmvPartCap;
+    double* zzPartB = zrPartB + kReduceBlocks;
+    auto zrSet = [&](int k) { return pipelined && (k & 1) ? zrPartB : zrPart; };
+    auto zzSet = [&](int k) { return pipelined && (k & 1) ? zzPartB : zzPart; };
+    if (pipelined) TRY_CALL(allocWork(c, &A->q32, n));
+    float* const dirs[2] = {A->p32, pipelined ? A->q32 : A->p32};
     CgState* st = c->state;
     double* betaSlots = reinterpret_cast<double*>(st + 1);
     double* dpiSlots = betaSlots + 2;
@@ -288,26 +336,48 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
     CgState h{};
     h.rtol = 0.0; h.dtol = o.dtol; h.maxit = o.maxit;
     int it = 0;
+    // scaled image: the device norm of a chain is ||r~|| = ||D^{-1/2} r|| / sigma,
+    // not the preconditioned one, so each chain's thresholds are taken relative
+    // to its starting norm chainStart, and chainScale converts its norms back
+    double chainStart = 1.0, chainScale = sigma;
+    std::vector<double> hpart(kReduceBlocks);
     auto reset = [&](double dpNow, bool restart) -> int {
-        const double stop = std::max(ttol, delta * dpNow) / sigma;
+        double stop = std::max(ttol, delta * dpNow) / sigma;
+        if (scaled) {
+            hipLaunchKernelGGL(kCgScaledReset, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, A->r64, A->dsq64,
+                               1.0 / sigma, A->r32, A->x32, zrSet(it), zzSet(it));
+            FEEL_HIP_CHECK(hipGetLastError());
+            FEEL_HIP_CHECK(hipMemcpyAsync(hpart.data(), zzSet(it), sizeof(double) * kReduceBlocks,
+                                          hipMemcpyDeviceToHost, c->stream));
+            FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
+            double rr = 0;
+            for (double v : hpart) rr += v;
+            chainStart = std::sqrt(rr);
+            chainScale = chainStart > 0 ? dpNow / chainStart : sigma;
+            stop = chainStart * (std::max(ttol, delta * dpNow) / dpNow);
+        }
         h.atol = stop;   // threshold at iteration 0
         h.ttol = stop;   // threshold afterwards
         h.done = 0; h.reason = 0; h.its = it;
         if (restart) h.fresh = it;
         FEEL_HIP_CHECK(hipMemcpyAsync(st, &h, sizeof(CgState), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(kCgStepReset, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, A->r64, 1.0 / sigma,
-                           A->dinv32, A->r32, A->z32, A->x32, zrPart, zzPart);
+        if (!scaled)
+            hipLaunchKernelGGL(kCgStepReset, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, A->r64, 1.0 / sigma,
+                               A->dinv32, A->r32, A->z32, A->x32, zrSet(it), zzSet(it));
         FEEL_HIP_CHECK(hipGetLastError());
         // h must stay valid until the copy has run
         FEEL_HIP_CHECK(hipStreamSynchronize(c->stream));
         return FEEL_HIP_OK;
     };
     TRY_CALL(reset(dp, true));
+    // the pipelined iteration starts each chain (and resumes after a reliable
+    // update) with the classic direction kernel: stopping test, p = z + b p_prev
+    bool needDirection = true;
     int reason = 0;
     // Launches queued past the iteration that stops the device are early
     // exits; the chunk is sized from the observed convergence rate so that
     // few are wasted before each reliable update.
-    double lastDp = 1.0;
+    double lastDp = scaled ? chainStart : 1.0;
     int lastIt = it, chunk = 16, lastKappa = 0;
     // condition estimate of the current Lanczos chain (since the last direction restart)
     auto tooIllConditioned = [&](bool& ill) -> int {
@@ -325,8 +395,34 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
     while (true) {
         if (o.check_every > 0) chunk = o.check_every;
         for (int k = 0; k < chunk && it <= o.maxit; ++k, ++it) {
+            if (pipelined) {
+                // direction it in dirs[it & 1], the previous one in the other buffer
+                float* p = dirs[it & 1];
+                if (needDirection) {
+                    hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it,
+                                       scaled ? A->r32 : A->z32, (const float*)dirs[(it + 1) & 1], p, zrSet(it),
+                                       zzSet(it), st, betaSlots, c->hist);
+                    needDirection = false;
+                }
+                int npart = 0;
+                c->tic("spmv_fp32");
+                TRY_CALL(spmv<float>(c, g, A->s32, p, A->w32, dpiPart, st, &npart,
+                                     scaled ? (const float*)nullptr : A->dinv32, wdwPart));
+                c->toc();
+                if (scaled)
+                    hipLaunchKernelGGL((kCgStep<float, true>), dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it,
+                                       A->x32, A->r32, (const float*)p, dirs[(it + 1) & 1], A->w32,
+                                       (const float*)nullptr, dpiPart, wdwPart, npart, zrSet(it), zzSet(it),
+                                       zrSet(it + 1), zzSet(it + 1), st, betaSlots, dpiSlots, c->hist);
+                else
+                    hipLaunchKernelGGL((kCgStep<float, false>), dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it,
+                                       A->x32, A->r32, (const float*)p, dirs[(it + 1) & 1], A->w32, A->dinv32,
+                                       dpiPart, wdwPart, npart, zrSet(it), zzSet(it), zrSet(it + 1), zzSet(it + 1), st,
+                                       betaSlots, dpiSlots, c->hist);
+                continue;
+            }
             hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->z32,
-                               A->p32, zrPart, zzPart, st, betaSlots, c->hist);
+                               (const float*)A->p32, A->p32, zrPart, zzPart, st, betaSlots, c->hist);
             int npart = 0;
             if (elemOp) {
                 c->tic("elemop_fp32");
@@ -364,9 +460,10 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
             reason = c->hstate->reason;
         }
         h.rnorm0 = c->hstate->rnorm0;   // set by the device at iteration 0
-        const double estimate = c->hstate->dp * sigma;
+        const double estimate = c->hstate->dp * (scaled ? chainScale : sigma);
         // fold the FP32 correction into the FP64 solution and take the true residual
-        hipLaunchKernelGGL(kAxpyF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, sigma, A->x32, A->xp64);
+        hipLaunchKernelGGL(kAxpyF32, dim3(gridN(n)), dim3(kBlock), 0, c->stream, n, sigma, A->x32,
+                           scaled ? (const double*)A->dsq64 : nullptr, A->xp64);
         FEEL_HIP_CHECK(hipGetLastError());
         TRY_CALL(residual64(A, A->bp64, A->xp64, A->r64, A->w64, dp));
         ++out.updates;
@@ -392,7 +489,8 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
             break;
         }
         TRY_CALL(reset(dp, dp > kDriftRestart * estimate));
-        lastDp = dp / sigma;
+        needDirection = true;
+        lastDp = scaled ? chainStart : dp / sigma;
         lastIt = it;
         chunk = 16;
     }
