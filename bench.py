@@ -593,7 +593,7 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--rates", default="5,10,20,30,50,200,1000,2000")
     ap.add_argument("--spmv-shape", type=int, default=-1,
-                    help="beamkernels: pin the SpMV launch shape (index into the candidate list)")
+                    help="pin the SpMV launch shape (index into the candidate list) instead of tuning it")
     ap.add_argument("--backfill", type=int, default=8,
                     help="serving: steps a request that does not fit the HBM budget lets smaller ones pass it")
     ap.add_argument("--requests", type=int, default=200)
@@ -627,6 +627,9 @@ def main():
     ctx = fa.Context(0)
     prob = Problem(fa, ctx, build_problem(fa, ctx, args.n, order, 0.15, 2024), order, qorder)
     host = prob.host
+    if args.spmv_shape >= 0:   # a fixed launch shape instead of the first-solve tuning
+        for prec in (fa.FP64, fa.FP32):
+            prob.graph.set_spmv_shape(prec, args.spmv_shape)
 
     # FP64 reference solution (tight) for the error figure
     st = prob.step(fa.FP64, 1e-13)

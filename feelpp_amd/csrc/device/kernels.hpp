@@ -74,19 +74,20 @@ __device__ inline void blockReduceSums(double (&v)[N]) {
     __syncthreads();
 }
 
-// This thread's share of the sum of count partials (kBlock threads): four
+// This thread's share of the sum of count partials (BS threads): four
 // independent accumulators so that its loads are in flight together, then a
 // fixed combination order.
+template <int BS = kBlock>
 __device__ inline double partialShare(const double* __restrict__ part, int count) {
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0;
     int i = threadIdx.x;
-    for (; i + 3 * kBlock < count; i += 4 * kBlock) {
+    for (; i + 3 * BS < count; i += 4 * BS) {
         v0 += part[i];
-        v1 += part[i + kBlock];
-        v2 += part[i + 2 * kBlock];
-        v3 += part[i + 3 * kBlock];
+        v1 += part[i + BS];
+        v2 += part[i + 2 * BS];
+        v3 += part[i + 3 * BS];
     }
-    for (; i < count; i += kBlock) v0 += part[i];
+    for (; i < count; i += BS) v0 += part[i];
     return (v0 + v1) + (v2 + v3);
 }
 
@@ -895,26 +896,32 @@ __global__ __launch_bounds__(kBlock) void kCgUpdate(int64_t n, int it, T* __rest
 // the SpMV's partials, and the direction kernel of the classic iteration is
 // not launched.  The true (r', z')
 // and (z', z') partials go to the next launch, whose first step is the stopping
-// test on them (kCgDirection's test; at it == st->fresh that kernel ran it and
+// test on them (kCgDirection's test; when that kernel ran in this iteration,
+// at the start of a chain or after a reliable update, it ran the test and
 // stored (r, z)).  The partial sets alternate with it, since every block reads
 // the previous launch's set while blocks that finished write theirs.
 // SCALED: the iteration runs on D^{-1/2} A D^{-1/2} (unit diagonal), the same
 // Jacobi-PCG in exact arithmetic (x~ = D^{1/2} x, r~ = D^{-1/2} r), where z = r
 // and dinv is not read.
 template <typename T, bool SCALED>
-__global__ __launch_bounds__(kBlock) void kCgStep(int64_t n, int it, T* __restrict__ x, T* __restrict__ r,
-                                                  const T* __restrict__ p, T* __restrict__ q, const T* __restrict__ w,
-                                                  const T* __restrict__ dinv, const double* __restrict__ dpiPart,
-                                                  const double* __restrict__ wdwPart, int npart,
-                                                  const double* __restrict__ zrIn, const double* __restrict__ zzIn,
-                                                  double* __restrict__ zrOut, double* __restrict__ zzOut, CgState* st,
-                                                  double* betaSlots, double* dpiSlots, double* hist) {
+__global__ __launch_bounds__(kStepBlock) void kCgStep(int64_t n, int it, T* __restrict__ x, T* __restrict__ r,
+                                                      const T* __restrict__ p, T* __restrict__ q,
+                                                      const T* __restrict__ w, const T* __restrict__ dinv,
+                                                      const double* __restrict__ dpiPart,
+                                                      const double* __restrict__ wdwPart, int npart,
+                                                      const double* __restrict__ zrIn, const double* __restrict__ zzIn,
+                                                      double* __restrict__ zrOut, double* __restrict__ zzOut,
+                                                      CgState* st, double* betaSlots, double* dpiSlots, double* hist,
+                                                      int tested) {
     if (blockDone(st)) return;
-    // the four scalars of the step in one block reduction
-    const bool test = it > st->fresh;
-    double sums[4] = {test ? partialShare(zrIn, kReduceBlocks) : 0.0, test ? partialShare(zzIn, kReduceBlocks) : 0.0,
-                      partialShare(dpiPart, npart), partialShare(wdwPart, npart)};
-    blockReduceSums<kBlock, 4>(sums);
+    // the four scalars of the step in one block reduction; unless the
+    // direction kernel ran this iteration's test (tested), zrIn / zzIn hold the
+    // kStepBlocks partials of the previous step
+    const bool test = !tested;
+    double sums[4] = {test ? partialShare<kStepBlock>(zrIn, kStepBlocks) : 0.0,
+                      test ? partialShare<kStepBlock>(zzIn, kStepBlocks) : 0.0,
+                      partialShare<kStepBlock>(dpiPart, npart), partialShare<kStepBlock>(wdwPart, npart)};
+    blockReduceSums<kStepBlock, 4>(sums);
     double rz;
     if (test) {
         rz = sums[0];
@@ -982,7 +989,7 @@ __global__ __launch_bounds__(kBlock) void kCgStep(int64_t n, int it, T* __restri
     }
     for (int64_t i = tail + gid; i < n; i += stride) step(x[i], r[i], q[i], p[i], w[i], SCALED ? (T)1 : dinv[i]);
     double out[2] = {zr, zz};
-    blockReduceSums<kBlock, 2>(out);
+    blockReduceSums<kStepBlock, 2>(out);
     if (threadIdx.x == 0) { zrOut[blockIdx.x] = out[0]; zzOut[blockIdx.x] = out[1]; }
 }
 

@@ -398,6 +398,7 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
             if (pipelined) {
                 // direction it in dirs[it & 1], the previous one in the other buffer
                 float* p = dirs[it & 1];
+                const int tested = needDirection ? 1 : 0;
                 if (needDirection) {
                     hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it,
                                        scaled ? A->r32 : A->z32, (const float*)dirs[(it + 1) & 1], p, zrSet(it),
@@ -410,15 +411,15 @@ int runMixedCg(feel_hip_mat* A, const feel_hip_solve_opts& o, bool elemOp, Mixed
                                      scaled ? (const float*)nullptr : A->dinv32, wdwPart));
                 c->toc();
                 if (scaled)
-                    hipLaunchKernelGGL((kCgStep<float, true>), dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it,
+                    hipLaunchKernelGGL((kCgStep<float, true>), dim3(kStepBlocks), dim3(kStepBlock), 0, c->stream, n, it,
                                        A->x32, A->r32, (const float*)p, dirs[(it + 1) & 1], A->w32,
                                        (const float*)nullptr, dpiPart, wdwPart, npart, zrSet(it), zzSet(it),
-                                       zrSet(it + 1), zzSet(it + 1), st, betaSlots, dpiSlots, c->hist);
+                                       zrSet(it + 1), zzSet(it + 1), st, betaSlots, dpiSlots, c->hist, tested);
                 else
-                    hipLaunchKernelGGL((kCgStep<float, false>), dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it,
+                    hipLaunchKernelGGL((kCgStep<float, false>), dim3(kStepBlocks), dim3(kStepBlock), 0, c->stream, n, it,
                                        A->x32, A->r32, (const float*)p, dirs[(it + 1) & 1], A->w32, A->dinv32,
                                        dpiPart, wdwPart, npart, zrSet(it), zzSet(it), zrSet(it + 1), zzSet(it + 1), st,
-                                       betaSlots, dpiSlots, c->hist);
+                                       betaSlots, dpiSlots, c->hist, tested);
                 continue;
             }
             hipLaunchKernelGGL(kCgDirection<float>, dim3(kReduceBlocks), dim3(kBlock), 0, c->stream, n, it, A->z32,
