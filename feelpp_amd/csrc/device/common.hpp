@@ -40,10 +40,11 @@ constexpr int kSpmvPartCap = 4096;    // largest SpMV grid (partials buffer capa
 // reads one set and writes the other)
 constexpr int kPartDoubles = 5 * kReduceBlocks + 2 * kSpmvPartCap;
 constexpr int kSpmvBlock = 1024;
-// the fused CG step kernel's launch (256 blocks of 1024 threads, which read
-// the prologue's partials a quarter as often, measured 1 % slower on C2)
-constexpr int kStepBlock = kBlock;
-constexpr int kStepBlocks = kReduceBlocks;
+// the fused CG step kernel's launch: 512 blocks of 512 threads read the
+// prologue's partials half as often as 1024 x 256 (C2 mixed +0.75 %; 256 x 1024
+// measured no better than 1024 x 256)
+constexpr int kStepBlock = 512;
+constexpr int kStepBlocks = 512;
 constexpr int kHistMax = 8192;        // CG coefficient history kept for the condition estimate
 constexpr int kElasticF = 90;         // elasticity factors per element: 9 blocks of 9 + pad
 constexpr int kSegTask = 256;         // scatter entries per task (whole slots)
