@@ -248,13 +248,13 @@ __global__ __launch_bounds__(kBlock) void kBInnerInit(const BItem* __restrict__ 
 // direction (re)starts as p = z at it == st->fresh
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBDirection(const BItem* __restrict__ items, const BInst* __restrict__ insts,
-                                                     const int* __restrict__ gcount) {
+                                                     const int* __restrict__ gcount, int k) {
     const int bid = blockIdx.x;
     const BItem item = items[bid];
     const BInst& I = insts[item.inst];
     CgState* st = I.st;
     if (instDone(st)) return;
-    const int it = *gcount - I.start;
+    const int it = *gcount + k - I.start;
     const double beta = instReduce(I.part, I.nitems);
     const double dp = sqrt(instReduce(I.part + I.nitems, I.nitems));
     const double ttol = it == 0 ? fmax(st->rtol * dp, st->atol) : st->ttol;
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(kGroupBlock) void kBSpmvGroup(const BGItem* __restr
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restrict__ gitems,
                                                           const BGroup* __restrict__ groups,
-                                                          const BInst* __restrict__ insts, const int* __restrict__ gcount) {
+                                                          const BInst* __restrict__ insts, const int* __restrict__ gcount, int k) {
     const BGItem item = gitems[blockIdx.x];
     if (item.group < 0) return;
     const BGroup& G = groups[item.group];
@@ -561,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restr
         if (sdone[b]) continue;
         const BInst& I = insts[G.inst[b]];
         CgState* st = I.st;
-        const int it = *gcount - I.start;
+        const int it = *gcount + k - I.start;
         const double beta = instReduce(I.part, I.nitems);
         const double dp = sqrt(instReduce(I.part + I.nitems, I.nitems));
         const double ttol = it == 0 ? fmax(st->rtol * dp, st->atol) : st->ttol;
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void kBDirectionGroup(const BGItem* __restr
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict__ gitems,
                                                        const BGroup* __restrict__ groups,
-                                                       const BInst* __restrict__ insts, const int* __restrict__ gcount) {
+                                                       const BInst* __restrict__ insts, const int* __restrict__ gcount, int k) {
     const BGItem item = gitems[blockIdx.x];
     if (item.group < 0) return;
     const BGroup& G = groups[item.group];
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict
         if (sdone[b]) continue;
         const BInst& I = insts[G.inst[b]];
         CgState* st = I.st;
-        const int it = *gcount - I.start;
+        const int it = *gcount + k - I.start;
         const double dpi = instReduce(I.part + I.apOff, I.nap);
         const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
         if (item.item == 0 && threadIdx.x == 0) {
@@ -706,13 +706,13 @@ __global__ __launch_bounds__(kBlock) void kBUpdateGroup(const BGItem* __restrict
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ items, const BInst* __restrict__ insts,
-                                                  const int* __restrict__ gcount) {
+                                                  const int* __restrict__ gcount, int k) {
     const int bid = blockIdx.x;
     const BItem item = items[bid];
     const BInst& I = insts[item.inst];
     CgState* st = I.st;
     if (instDone(st)) return;
-    const int it = *gcount - I.start;
+    const int it = *gcount + k - I.start;
     const double dpi = instReduce(I.part + I.apOff, I.nap);
     const bool indefinite = dpi == 0.0 || (it > 0 && dpi * I.slots[2 + ((it - 1) & 1)] <= 0.0);
     if (item.item == 0 && threadIdx.x == 0) {
@@ -741,8 +741,8 @@ __global__ __launch_bounds__(kBlock) void kBUpdate(const BItem* __restrict__ ite
     if (threadIdx.x == 0) { I.part[item.item] = zr; I.part[I.nitems + item.item] = zz; }
 }
 
-__global__ void kBBump(int* gcount) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *gcount += 1;
+__global__ void kBBump(int* gcount, int iterations) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *gcount += iterations;
 }
 
 __global__ __launch_bounds__(kBlock) void kBPermuteOut(int64_t n, const int32_t* __restrict__ perm,
@@ -1104,21 +1104,22 @@ int launchIterations(feel_hip_batch* B, int nItems, int chunk, int nterms) {
     const bool grouped = nterms > 0;
     auto body = [&]() {
         for (int k = 0; k < chunk; ++k) {
+            // iteration k of the chunk: the counter moves once per chunk
             if (grouped) {
                 hipLaunchKernelGGL(kBDirectionGroup<T>, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems,
-                                   B->dGroups, B->dInsts, B->gcount);
+                                   B->dGroups, B->dInsts, B->gcount, k);
                 launchGroupSpmv<T>(B, nterms);
                 hipLaunchKernelGGL(kBUpdateGroup<T>, dim3(B->nGDItems), dim3(kBlock), 0, c->stream, B->dGDItems, B->dGroups,
-                                   B->dInsts, B->gcount);
+                                   B->dInsts, B->gcount, k);
             } else {
                 hipLaunchKernelGGL(kBDirection<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts,
-                                   B->gcount);
+                                   B->gcount, k);
                 hipLaunchKernelGGL(kBSpmv<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts);
                 hipLaunchKernelGGL(kBUpdate<T>, dim3(nItems), dim3(kBlock), 0, c->stream, B->dItems, B->dInsts,
-                                   B->gcount);
+                                   B->gcount, k);
             }
-            hipLaunchKernelGGL(kBBump, dim3(1), dim3(64), 0, c->stream, B->gcount);
         }
+        hipLaunchKernelGGL(kBBump, dim3(1), dim3(64), 0, c->stream, B->gcount, chunk);
     };
     if (!B->opts.use_graph) {
         body();
